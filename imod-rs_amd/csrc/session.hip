@@ -1,0 +1,981 @@
+// libtiltgpu session: the C ABI of include/tiltgpu.h over the gfx950 kernels.
+//
+// A session holds one tilt series and its volume resident in HBM and runs the
+// pipeline of imod-rs apps/tilt/src/main.rs:601-1090 on one HIP stream:
+//   log + per-view means -> R-weighting + cosine stretch (+ FP16/BF16 storage)
+//   -> WBP (output transform fused)  or  SIRT: FP -> (O - R) filter -> BP += relax
+//   -> fake-SIRT low-pass -> output transform.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tiltgpu.h"
+#include "tiltgpu_internal.h"
+
+using namespace tiltgpu;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(TILTGPU_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+struct TimedRegion {
+    hipEvent_t a, b;
+    int cls;
+};
+
+}  // namespace
+
+enum { kClsPrep = 0, kClsFilter, kClsBp, kClsFp, kClsOther, kNumCls };
+
+struct tiltgpu_session {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tiltgpu_dims d{};
+    tiltgpu_options o{};
+    bool seam = false;  // created by tiltgpu_bp_session_new
+
+    std::vector<float> angles, xtilt, zfac;
+    bool has_xtilt = false, has_zfac = false, has_angles = false;
+    int nviews = 0;
+    bool geometry_ready = false;
+
+    // local alignment (host copy)
+    bool has_local = false;
+    std::vector<int> lnx, lny, lxoff, lyoff, lgoff;
+    std::vector<float> lxs, lys, ldx, ldy;
+
+    // device buffers
+    float* raw = nullptr;      // uploaded projections (never modified)
+    float* logd = nullptr;     // --log output (when enabled)
+    float* filt = nullptr;     // filtered + stretched, f32
+    void* store = nullptr;     // f16 / bf16 copy of filt
+    float* scale = nullptr;    // [nz] dequantisation scale
+    unsigned* absmax = nullptr;
+    float* vol = nullptr;      // [out_ny][out_nz][out_nx]
+    float* vol_tmp = nullptr;  // fake-SIRT ping-pong
+    float* sino = nullptr;     // SIRT reprojection [nz][in_ny][in_nx]
+    float4* vgeom = nullptr;
+    float* zc = nullptr;
+    float* means = nullptr;
+    double* partial = nullptr;
+    int partial_n = 0;
+    float* stretch = nullptr;
+    float* weights = nullptr;
+    float2* tw = nullptr;
+    int* err = nullptr;
+    double* resid = nullptr;
+    float* stats = nullptr;
+    int stats_rows = 0;
+    int* d_lnx = nullptr;
+    int* d_lny = nullptr;
+    int* d_lxoff = nullptr;
+    int* d_lyoff = nullptr;
+    int* d_lgoff = nullptr;
+    float* d_lxs = nullptr;
+    float* d_lys = nullptr;
+    float* d_ldx = nullptr;
+    float* d_ldy = nullptr;
+
+    int nz_pad = 0;
+    int tile_z = 64;
+    int lds_cap = 0;
+    bool two_row = false;
+    int radix[kMaxRadix];
+    int nradix = 0;
+    bool uploaded = false;
+    bool ran = false;
+    int sirt_done = 0;
+    int64_t bytes = 0;
+    std::vector<void*> allocs;
+
+    std::vector<TimedRegion> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ms[kNumCls] = {0, 0, 0, 0, 0};
+    int64_t n_bp = 0, n_filter = 0, n_fp = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(tiltgpu_session* s, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TILTGPU_ERR_BUDGET : TILTGPU_ERR_HIP,
+                    "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    s->allocs.push_back(q);
+    s->bytes += static_cast<int64_t>(bytes);
+    *p = static_cast<T*>(q);
+    return TILTGPU_OK;
+}
+
+template <typename T>
+void dev_free(tiltgpu_session* s, T*& p) {
+    if (!p) return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(p));
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+hipEvent_t take_event(tiltgpu_session* s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct Scope {
+    tiltgpu_session* s;
+    TimedRegion r;
+    Scope(tiltgpu_session* s_, int cls) : s(s_) {
+        r.a = take_event(s);
+        r.b = take_event(s);
+        r.cls = cls;
+        (void)hipEventRecord(r.a, s->stream);
+    }
+    ~Scope() {
+        (void)hipEventRecord(r.b, s->stream);
+        s->pending.push_back(r);
+    }
+};
+
+void collect_timing(tiltgpu_session* s) {
+    for (auto& r : s->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) s->ms[r.cls] += ms;
+        s->event_pool.push_back(r.a);
+        s->event_pool.push_back(r.b);
+    }
+    s->pending.clear();
+}
+
+float f32_proj_cos(float deg) {
+    const float rad = deg * static_cast<float>(M_PI) / 180.0f;
+    return cosf(rad);
+}
+
+// 1/cos(theta) for the cosine stretch, 0 when the view is skipped
+// (main.rs:449-459).
+float stretch_factor(float deg, float max_tilt) {
+    if (std::fabs(deg) > max_tilt) return 0.f;
+    const float ct = f32_proj_cos(deg);
+    if (std::fabs(ct) < 1e-6f) return 0.f;
+    return 1.0f / ct;
+}
+
+int pick_pairs_per_wg(int n) { return std::max(1, 2048 / std::max(1, n)); }
+
+// Upper bound on the LDS entries one backprojection tile needs for a view.
+void tile_bounds(const tiltgpu_session* s, int view, const tiltgpu_view_params& g, int tile_z,
+                 int* seg, int* rows) {
+    double ldx_span = 0, ldy_span = 0;
+    if (s->has_local && s->lnx[view] > 0 && s->lny[view] > 0) {
+        const int ng = s->lnx[view] * s->lny[view];
+        const float* gx = s->ldx.data() + s->lgoff[view];
+        const float* gy = s->ldy.data() + s->lgoff[view];
+        float xmn = gx[0], xmx = gx[0], ymn = gy[0], ymx = gy[0];
+        for (int i = 1; i < ng; ++i) {
+            xmn = std::min(xmn, gx[i]);
+            xmx = std::max(xmx, gx[i]);
+            ymn = std::min(ymn, gy[i]);
+            ymx = std::max(ymx, gy[i]);
+        }
+        ldx_span = static_cast<double>(xmx) - xmn;
+        ldy_span = static_cast<double>(ymx) - ymn;
+    }
+    const double sx = std::fabs(static_cast<double>(g.cos_t) * g.cos_xt) * (kBpTileX - 1) +
+                      std::fabs(static_cast<double>(g.sin_t) * g.zfactor) * (tile_z - 1) +
+                      ldx_span + 4.0;
+    const double sy = std::fabs(static_cast<double>(g.sin_xt)) * (kBpTileX - 1) + ldy_span + 3.0;
+    *seg = static_cast<int>(std::min(sx, 1e9));
+    *rows = static_cast<int>(std::min(sy, 1e9));
+}
+
+int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& vp) {
+    const int nv = static_cast<int>(vp.size());
+    s->nviews = nv;
+    s->two_row = s->has_local;
+    for (const auto& g : vp)
+        if (g.sin_xt != 0.0f) s->two_row = true;
+    // tile depth and LDS capacity from the per-view bounds
+    const int max_lds_bytes = 64 * 1024;
+    int chosen = 0;
+    for (int tz : {64, 32}) {
+        int cap = 1;
+        for (int v = 0; v < nv; ++v) {
+            int seg, rows;
+            tile_bounds(s, v, vp[v], tz, &seg, &rows);
+            const int need = s->two_row ? seg * rows : seg;
+            cap = std::max(cap, need);
+        }
+        cap = (cap + 63) / 64 * 64;
+        if (bp_lds_bytes(cap, s->two_row) <= static_cast<size_t>(max_lds_bytes)) {
+            s->tile_z = tz;
+            s->lds_cap = cap;
+            chosen = 1;
+            break;
+        }
+    }
+    if (!chosen)
+        return fail(TILTGPU_ERR_ARG,
+                    "geometry needs more LDS per backprojection tile than supported "
+                    "(local-alignment shifts vary too much within 64 pixels)");
+    const int out_nz = s->d.out_nz;
+    s->nz_pad = (out_nz + s->tile_z - 1) / s->tile_z * s->tile_z;
+    std::vector<float4> g4(std::max(nv, 1));
+    std::vector<float> zc(static_cast<size_t>(std::max(nv, 1)) * s->nz_pad);
+    const float center_z = static_cast<float>(out_nz) / 2.0f;
+    for (int v = 0; v < nv; ++v) {
+        g4[v] = make_float4(vp[v].cos_t, vp[v].sin_t, vp[v].cos_xt, vp[v].sin_xt);
+        for (int oz = 0; oz < s->nz_pad; ++oz) {
+            const float dz = static_cast<float>(oz) - center_z;
+            // main.rs:1015-1016: z_contrib = dz * sin_t * zfactor
+            zc[static_cast<size_t>(v) * s->nz_pad + oz] = dz * vp[v].sin_t * vp[v].zfactor;
+        }
+    }
+    dev_free(s, s->vgeom);
+    dev_free(s, s->zc);
+    int rc;
+    if ((rc = dev_alloc(s, &s->vgeom, g4.size())) || (rc = dev_alloc(s, &s->zc, zc.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(s->vgeom, g4.data(), g4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->zc, zc.data(), zc.size() * sizeof(float), hipMemcpyHostToDevice));
+    s->geometry_ready = true;
+    return TILTGPU_OK;
+}
+
+int upload_local(tiltgpu_session* s) {
+    if (!s->has_local) return TILTGPU_OK;
+    int rc;
+    auto up_i = [&](int** dp, const std::vector<int>& h) -> int {
+        dev_free(s, *dp);
+        int r = dev_alloc(s, dp, h.size());
+        if (r) return r;
+        if (!h.empty()) HIP_TRY(hipMemcpy(*dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        return TILTGPU_OK;
+    };
+    auto up_f = [&](float** dp, const std::vector<float>& h) -> int {
+        dev_free(s, *dp);
+        int r = dev_alloc(s, dp, h.size());
+        if (r) return r;
+        if (!h.empty()) HIP_TRY(hipMemcpy(*dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        return TILTGPU_OK;
+    };
+    if ((rc = up_i(&s->d_lnx, s->lnx)) || (rc = up_i(&s->d_lny, s->lny)) ||
+        (rc = up_i(&s->d_lxoff, s->lxoff)) || (rc = up_i(&s->d_lyoff, s->lyoff)) ||
+        (rc = up_i(&s->d_lgoff, s->lgoff)) || (rc = up_f(&s->d_lxs, s->lxs)) ||
+        (rc = up_f(&s->d_lys, s->lys)) || (rc = up_f(&s->d_ldx, s->ldx)) ||
+        (rc = up_f(&s->d_ldy, s->ldy)))
+        return rc;
+    return TILTGPU_OK;
+}
+
+// Geometry of the full pipeline from the angle / xtilt / zfactor inputs.
+int prepare_pipeline_geometry(tiltgpu_session* s) {
+    if (!s->has_angles) return fail(TILTGPU_ERR_STATE, "tilt angles not set");
+    const int nz = s->d.nz;
+    const int nv = std::min(nz, static_cast<int>(s->angles.size()));
+    std::vector<tiltgpu_view_params> vp(std::max(nv, 1));
+    int rc = tiltgpu_proj_params(s->angles.data(), nv, s->has_xtilt ? s->xtilt.data() : nullptr,
+                                 static_cast<int32_t>(s->xtilt.size()),
+                                 s->has_zfac ? s->zfac.data() : nullptr,
+                                 static_cast<int32_t>(s->zfac.size()), vp.data());
+    if (rc) return rc;
+    vp.resize(nv);
+    if ((rc = setup_geometry(s, vp))) return rc;
+    if ((rc = upload_local(s))) return rc;
+    // stretch factors for every section (main.rs:449 indexes tilt_angles[p], p < nz)
+    std::vector<float> st(std::max(nz, 1), 0.f);
+    if (!s->o.no_cosine_stretch) {
+        if (static_cast<int>(s->angles.size()) < nz)
+            return fail(TILTGPU_ERR_ARG,
+                        "cosine stretch needs one tilt angle per section (%zu angles, %d "
+                        "sections); the reference panics here (main.rs:450)",
+                        s->angles.size(), nz);
+        for (int p = 0; p < nz; ++p) st[p] = stretch_factor(s->angles[p], s->o.max_cosine_tilt);
+    }
+    dev_free(s, s->stretch);
+    if ((rc = dev_alloc(s, &s->stretch, st.size()))) return rc;
+    HIP_TRY(hipMemcpy(s->stretch, st.data(), st.size() * 4, hipMemcpyHostToDevice));
+    return TILTGPU_OK;
+}
+
+int prepare_filter(tiltgpu_session* s) {
+    const int n = s->d.in_nx;
+    if (filter_plan(n, s->radix, &s->nradix) != 0)
+        return fail(TILTGPU_ERR_ARG,
+                    "projection width %d has a prime factor above 31; the LDS FFT "
+                    "supports widths whose prime factors are all <= 31",
+                    n);
+    if (filter_lds_bytes(n, pick_pairs_per_wg(n)) > 160 * 1024)
+        return fail(TILTGPU_ERR_ARG, "projection width %d too large for the LDS FFT", n);
+    std::vector<float> w(n);
+    int rc = tiltgpu_radial_weights(n, s->o.exact_filter_size, s->o.radial_cutoff,
+                                    s->o.radial_falloff, s->o.hamming, w.data());
+    if (rc) return rc;
+    std::vector<float2> tw(n);
+    for (int t = 0; t < n; ++t) {
+        const double a = -2.0 * M_PI * static_cast<double>(t) / static_cast<double>(n);
+        tw[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+    dev_free(s, s->weights);
+    dev_free(s, s->tw);
+    if ((rc = dev_alloc(s, &s->weights, n)) || (rc = dev_alloc(s, &s->tw, n))) return rc;
+    HIP_TRY(hipMemcpy(s->weights, w.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    return TILTGPU_OK;
+}
+
+int check_kernel_error(tiltgpu_session* s) {
+    int e = 0;
+    HIP_TRY(hipMemcpy(&e, s->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) {
+        int z = 0;
+        HIP_TRY(hipMemcpy(s->err, &z, sizeof(int), hipMemcpyHostToDevice));
+        return fail(TILTGPU_ERR_KERNEL,
+                    "a backprojection tile needed more LDS entries than its bound (%d)",
+                    s->lds_cap);
+    }
+    return TILTGPU_OK;
+}
+
+BpParams make_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
+                 int out_mode, float* out) {
+    BpParams p{};
+    p.proj = proj;
+    p.view_scale = scale;
+    p.vgeom = s->vgeom;
+    p.zc_tab = s->zc;
+    p.means = s->means;
+    p.out = out;
+    p.in_nx = s->d.in_nx;
+    p.in_ny = s->d.in_ny;
+    p.nviews = s->nviews;
+    p.out_nx = s->d.out_nx;
+    p.out_nz = s->d.out_nz;
+    p.nz_pad = s->nz_pad;
+    p.y0 = y0;
+    p.y1 = y1;
+    p.ntx = (s->d.out_nx + kBpTileX - 1) / kBpTileX;
+    p.ntz = s->nz_pad / s->tile_z;
+    p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
+    p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
+    p.inv_n = 1.0f / static_cast<float>(s->d.nz);
+    p.fill_mode = s->o.fill_mode;
+    p.out_mode = out_mode;
+    p.dw = s->o.densweight;
+    p.os = s->o.outscale;
+    p.oa = s->o.outadd;
+    p.relax = s->o.sirt_relax;
+    p.has_local = s->has_local ? 1 : 0;
+    p.lnx = s->d_lnx;
+    p.lny = s->d_lny;
+    p.lxoff = s->d_lxoff;
+    p.lyoff = s->d_lyoff;
+    p.lgoff = s->d_lgoff;
+    p.lxs = s->d_lxs;
+    p.lys = s->d_lys;
+    p.ldx = s->d_ldx;
+    p.ldy = s->d_ldy;
+    p.lds_cap = s->lds_cap;
+    p.err = s->err;
+    return p;
+}
+
+int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
+           int out_mode, float* out) {
+    BpParams p = make_bp(s, proj, scale, y0, y1, out_mode, out);
+    Scope sc(s, kClsBp);
+    HIP_TRY(launch_backproject(p, s->seam ? TILTGPU_STORE_F32 : s->o.storage, s->two_row ? 1 : 0,
+                               s->tile_z, s->stream));
+    s->n_bp++;
+    return TILTGPU_OK;
+}
+
+// Filter (and optionally stretch) rows of `src` (minus `sub`) into s->filt and,
+// for FP16/BF16 storage, convert into s->store.
+int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
+               double* sq_out) {
+    const int n = s->d.in_nx;
+    const int nrows = s->d.nz * s->d.in_ny;
+    FilterParams f{};
+    f.src = src;
+    f.src_sub = sub;
+    f.dst = s->filt;
+    f.weights = s->o.no_weight ? nullptr : s->weights;
+    f.tw = s->tw;
+    f.stretch = (stretch && !s->o.no_cosine_stretch) ? s->stretch : nullptr;
+    f.n = n;
+    f.nrows = nrows;
+    f.rows_per_view = s->d.in_ny;
+    f.pairs_per_wg = pick_pairs_per_wg(n);
+    f.cubic = s->o.cosine_cubic;
+    f.nradix = s->nradix;
+    for (int i = 0; i < s->nradix; ++i) f.radix[i] = s->radix[i];
+    const int nblk = ((nrows + 1) / 2 + f.pairs_per_wg - 1) / f.pairs_per_wg;
+    if (sq_out) {
+        if (s->partial_n < nblk) {
+            dev_free(s, s->partial);
+            int rc = dev_alloc(s, &s->partial, nblk);
+            if (rc) return rc;
+            s->partial_n = nblk;
+        }
+        f.sq_partial = s->partial;
+    }
+    {
+        Scope sc(s, kClsFilter);
+        HIP_TRY(launch_filter(f, s->stream));
+        if (s->o.storage != TILTGPU_STORE_F32)
+            HIP_TRY(launch_convert_storage(s->filt, s->store, s->scale, s->absmax,
+                                           static_cast<int64_t>(n) * s->d.in_ny, s->d.nz,
+                                           s->o.storage, s->stream));
+        s->n_filter++;
+    }
+    if (sq_out) {
+        Scope sc(s, kClsOther);
+        HIP_TRY(launch_sum_partials(s->partial, nblk, sq_out, 1, s->stream));
+    }
+    return TILTGPU_OK;
+}
+
+int run_forward(tiltgpu_session* s) {
+    Scope sc(s, kClsFp);
+    const int64_t total = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    if (s->two_row) {
+        HIP_TRY(launch_fill_f32(s->sino, total, 0.f, s->stream));
+        FpScatterParams p{};
+        p.vol = s->vol;
+        p.sino = s->sino;
+        p.vgeom = s->vgeom;
+        p.zc_tab = s->zc;
+        p.in_nx = s->d.in_nx;
+        p.in_ny = s->d.in_ny;
+        p.nviews = s->nviews;
+        p.out_nx = s->d.out_nx;
+        p.out_nz = s->d.out_nz;
+        p.nz_pad = s->nz_pad;
+        p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
+        p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
+        p.has_local = s->has_local ? 1 : 0;
+        p.lnx = s->d_lnx;
+        p.lny = s->d_lny;
+        p.lxoff = s->d_lxoff;
+        p.lyoff = s->d_lyoff;
+        p.lgoff = s->d_lgoff;
+        p.lxs = s->d_lxs;
+        p.lys = s->d_lys;
+        p.ldx = s->d_ldx;
+        p.ldy = s->d_ldy;
+        HIP_TRY(launch_forward_scatter(p, s->stream));
+    } else {
+        // views >= nviews are never projected (main.rs:835 iterates proj_params)
+        if (s->nviews < s->d.nz) {
+            const int64_t per = static_cast<int64_t>(s->d.in_ny) * s->d.in_nx;
+            HIP_TRY(launch_fill_f32(s->sino + s->nviews * per, (s->d.nz - s->nviews) * per, 0.f,
+                                    s->stream));
+        }
+        FpParams p{};
+        p.vol = s->vol;
+        p.sino = s->sino;
+        p.vgeom = s->vgeom;
+        p.zc_tab = s->zc;
+        p.in_nx = s->d.in_nx;
+        p.in_ny = s->d.in_ny;
+        p.nviews = s->nviews;
+        p.out_nx = s->d.out_nx;
+        p.out_nz = s->d.out_nz;
+        p.nz_pad = s->nz_pad;
+        p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
+        p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
+        HIP_TRY(launch_forward_gather(p, s->stream));
+    }
+    s->n_fp++;
+    return TILTGPU_OK;
+}
+
+int validate_dims(const tiltgpu_dims* d) {
+    if (!d) return fail(TILTGPU_ERR_ARG, "dims is NULL");
+    if (d->in_nx < 2 || d->in_ny < 1 || d->nz < 1 || d->out_nx < 1 || d->out_nz < 1)
+        return fail(TILTGPU_ERR_ARG, "invalid dimensions in=%dx%dx%d out_nx=%d out_nz=%d",
+                    d->in_nx, d->in_ny, d->nz, d->out_nx, d->out_nz);
+    return TILTGPU_OK;
+}
+
+int init_device(tiltgpu_session* s) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");
+    HIP_TRY(hipGetDevice(&s->device));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    int rc;
+    if ((rc = dev_alloc(s, &s->err, 1))) return rc;
+    HIP_TRY(hipMemset(s->err, 0, sizeof(int)));
+    return TILTGPU_OK;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+
+extern "C" {
+
+const char* tiltgpu_last_error(void) { return g_last_error.c_str(); }
+
+const char* tiltgpu_version(void) { return "tiltgpu 0.1 (gfx950)"; }
+
+int tiltgpu_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+void tiltgpu_default_options(tiltgpu_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->radial_cutoff = 0.35f;
+    o->radial_falloff = 0.05f;
+    o->max_cosine_tilt = 80.0f;
+    o->fill_mode = TILTGPU_FILL_MEAN;
+    o->densweight = 1.0f;
+    o->outscale = 1.0f;
+    o->outadd = 0.0f;
+    o->sirt_relax = 1.0f;
+    o->storage = TILTGPU_STORE_F32;
+}
+
+int tiltgpu_proj_params(const float* angles, int32_t n, const float* xtilt, int32_t n_xtilt,
+                        const float* zfactor, int32_t n_zfactor, tiltgpu_view_params* out) {
+    if (n < 0 || (n > 0 && (!angles || !out))) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const float pi = static_cast<float>(M_PI);
+    for (int i = 0; i < n; ++i) {
+        // main.rs:688-697, all in f32
+        const float rad = angles[i] * pi / 180.0f;
+        const float xt_deg = (xtilt && i < n_xtilt) ? xtilt[i] : 0.0f;
+        const float xt_rad = xt_deg * pi / 180.0f;
+        const float zf = (zfactor && i < n_zfactor) ? zfactor[i] : 1.0f;
+        out[i].cos_t = cosf(rad);
+        out[i].sin_t = sinf(rad);
+        out[i].cos_xt = cosf(xt_rad);
+        out[i].sin_xt = sinf(xt_rad);
+        out[i].zfactor = zf;
+    }
+    return TILTGPU_OK;
+}
+
+int tiltgpu_radial_weights(int32_t n, int32_t exact_filter_size, float cutoff, float falloff,
+                           int32_t hamming, float* out) {
+    if (n < 1 || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int nxc = exact_filter_size > 0 ? exact_filter_size : n / 2 + 1;
+    const float pi = static_cast<float>(M_PI);
+    for (int i = 0; i < n; ++i) {
+        // main.rs:351-369
+        const int bin = (i <= n / 2) ? i : n - i;
+        const float w = static_cast<float>(bin) / static_cast<float>(nxc);
+        float weight;
+        if (hamming) {
+            float h;
+            if (cutoff > 1e-12f && w <= cutoff)
+                h = 0.54f + 0.46f * cosf(pi * w / cutoff);
+            else if (w > cutoff)
+                h = 0.0f;
+            else
+                h = 1.0f;
+            weight = w * h;
+        } else if (w <= cutoff) {
+            weight = w;
+        } else {
+            const float dd = w - cutoff;
+            weight = w * expf(-dd * dd / (2.0f * falloff * falloff));
+        }
+        out[i] = weight;
+    }
+    return TILTGPU_OK;
+}
+
+int tiltgpu_create(const tiltgpu_dims* dims, const tiltgpu_options* opts,
+                   tiltgpu_session** out) {
+    if (!out) return fail(TILTGPU_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = validate_dims(dims);
+    if (rc) return rc;
+    auto* s = new tiltgpu_session();
+    s->d = *dims;
+    if (opts)
+        s->o = *opts;
+    else
+        tiltgpu_default_options(&s->o);
+    if (s->o.storage < 0 || s->o.storage > 2) {
+        delete s;
+        return fail(TILTGPU_ERR_ARG, "unknown storage %d", s->o.storage);
+    }
+    if (s->o.fill_mode < 0 || s->o.fill_mode > 2) s->o.fill_mode = TILTGPU_FILL_MEAN;
+    if ((rc = init_device(s)) || (rc = prepare_filter(s))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    const int64_t nvol = static_cast<int64_t>(s->d.in_ny) * s->d.out_nz * s->d.out_nx;
+    if ((rc = dev_alloc(s, &s->raw, nproj)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
+        (rc = dev_alloc(s, &s->vol, nvol)) || (rc = dev_alloc(s, &s->means, s->d.nz)) ||
+        (rc = dev_alloc(s, &s->scale, s->d.nz)) || (rc = dev_alloc(s, &s->absmax, s->d.nz))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    if (s->o.log && (rc = dev_alloc(s, &s->logd, nproj))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    if (s->o.storage != TILTGPU_STORE_F32) {
+        unsigned short* st = nullptr;
+        if ((rc = dev_alloc(s, &st, nproj))) {
+            tiltgpu_destroy(s);
+            return rc;
+        }
+        s->store = st;
+    }
+    if (s->o.sirt > 0) {
+        if ((rc = dev_alloc(s, &s->sino, nproj)) || (rc = dev_alloc(s, &s->resid, s->o.sirt))) {
+            tiltgpu_destroy(s);
+            return rc;
+        }
+    }
+    if (s->o.fake_sirt > 0 && (rc = dev_alloc(s, &s->vol_tmp, nvol))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_angles(tiltgpu_session* s, const float* a, int32_t n) {
+    if (!s || n < 0 || (n > 0 && !a)) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    s->angles.assign(a, a + n);
+    s->has_angles = true;
+    s->geometry_ready = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_xtilt(tiltgpu_session* s, const float* a, int32_t n) {
+    if (!s || n < 0 || (n > 0 && !a)) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    s->xtilt.assign(a, a + n);
+    s->has_xtilt = true;
+    s->geometry_ready = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_zfactor(tiltgpu_session* s, const float* a, int32_t n) {
+    if (!s || n < 0 || (n > 0 && !a)) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    s->zfac.assign(a, a + n);
+    s->has_zfac = true;
+    s->geometry_ready = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_local_alignment(tiltgpu_session* s, const int32_t* nx, const int32_t* ny,
+                                const float* xs, const float* ys, const float* dx,
+                                const float* dy) {
+    if (!s || !nx || !ny) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int nz = s->d.nz;
+    s->lnx.assign(nx, nx + nz);
+    s->lny.assign(ny, ny + nz);
+    s->lxoff.assign(nz + 1, 0);
+    s->lyoff.assign(nz + 1, 0);
+    s->lgoff.assign(nz + 1, 0);
+    for (int v = 0; v < nz; ++v) {
+        if (nx[v] < 0 || ny[v] < 0 || (nx[v] == 0) != (ny[v] == 0))
+            return fail(TILTGPU_ERR_ARG, "bad local grid size for view %d", v);
+        s->lxoff[v + 1] = s->lxoff[v] + nx[v];
+        s->lyoff[v + 1] = s->lyoff[v] + ny[v];
+        s->lgoff[v + 1] = s->lgoff[v] + nx[v] * ny[v];
+    }
+    const int tx = s->lxoff[nz], ty = s->lyoff[nz], tg = s->lgoff[nz];
+    if ((tx && !xs) || (ty && !ys) || (tg && (!dx || !dy)))
+        return fail(TILTGPU_ERR_ARG, "local grid arrays are NULL");
+    s->lxs.assign(xs, xs + tx);
+    s->lys.assign(ys, ys + ty);
+    s->ldx.assign(dx, dx + tg);
+    s->ldy.assign(dy, dy + tg);
+    s->has_local = true;
+    s->geometry_ready = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t count) {
+    if (!s || !host) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    if (count != nproj)
+        return fail(TILTGPU_ERR_ARG, "expected %lld projection values, got %lld",
+                    static_cast<long long>(nproj), static_cast<long long>(count));
+    HIP_TRY(hipMemcpyAsync(s->raw, host, nproj * sizeof(float), hipMemcpyHostToDevice,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->uploaded = true;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_run(tiltgpu_session* s) {
+    if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    const int nz = s->d.nz;
+    const int64_t nxy = static_cast<int64_t>(s->d.in_nx) * s->d.in_ny;
+    const float* O = s->o.log ? s->logd : s->raw;
+    // log + means
+    {
+        const int nblk = static_cast<int>(std::min<int64_t>(64, (nxy + 4095) / 4096));
+        const int need = nblk * nz;
+        if (s->partial_n < need) {
+            dev_free(s, s->partial);
+            if ((rc = dev_alloc(s, &s->partial, need))) return rc;
+            s->partial_n = need;
+        }
+        Scope sc(s, kClsPrep);
+        HIP_TRY(launch_log_means(s->raw, s->logd, nxy, nz, s->o.log, s->partial, nblk, s->means,
+                                 s->stream));
+    }
+    // R-weighting + cosine stretch (+ storage conversion)
+    if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
+    const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
+                                                         : s->store;
+    const int out_ny = s->d.in_ny;
+    const bool fused_transform = s->o.sirt == 0 && s->o.fake_sirt == 0;
+    if ((rc = run_bp(s, proj, s->scale, 0, out_ny, fused_transform ? 1 : 0, s->vol))) return rc;
+    // SIRT (main.rs:919-953)
+    s->sirt_done = 0;
+    for (int it = 0; it < s->o.sirt; ++it) {
+        if ((rc = run_forward(s))) return rc;
+        if ((rc = run_filter(s, O, s->sino, false, s->resid + it))) return rc;
+        if ((rc = run_bp(s, proj, s->scale, 0, out_ny, 2, s->vol))) return rc;
+        s->sirt_done++;
+    }
+    // fake-SIRT low-pass (main.rs:955-962, 976-985)
+    const int onz = s->d.out_nz, onx = s->d.out_nx;
+    for (int k = 0; k < s->o.fake_sirt; ++k) {
+        Scope sc(s, kClsOther);
+        HIP_TRY(launch_lowpass_x(s->vol, s->vol_tmp, out_ny, onz, onx, s->stream));
+        HIP_TRY(launch_lowpass_z(s->vol_tmp, s->vol, out_ny, onz, onx, s->stream));
+        HIP_TRY(launch_lowpass_y(s->vol, s->vol_tmp, out_ny, onz, onx, s->stream));
+        std::swap(s->vol, s->vol_tmp);
+    }
+    if (!fused_transform) {
+        Scope sc(s, kClsOther);
+        HIP_TRY(launch_transform(s->vol, static_cast<int64_t>(out_ny) * onz * onx,
+                                 s->o.densweight, s->o.outscale, s->o.outadd, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    if ((rc = check_kernel_error(s))) return rc;
+    s->ran = true;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_read_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out, float* stats) {
+    if (!s || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    if (!s->ran) return fail(TILTGPU_ERR_STATE, "nothing reconstructed yet");
+    if (y0 < 0 || y1 > s->d.in_ny || y0 > y1) return fail(TILTGPU_ERR_ARG, "bad row range");
+    const int64_t per = static_cast<int64_t>(s->d.out_nz) * s->d.out_nx;
+    const int rows = y1 - y0;
+    if (rows == 0) return TILTGPU_OK;
+    if (stats) {
+        if (s->stats_rows < rows) {
+            dev_free(s, s->stats);
+            int rc = dev_alloc(s, &s->stats, 3 * static_cast<size_t>(rows));
+            if (rc) return rc;
+            s->stats_rows = rows;
+        }
+        Scope sc(s, kClsOther);
+        HIP_TRY(launch_slice_stats(s->vol + y0 * per, rows, per, s->stats, s->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(out, s->vol + y0 * per, rows * per * sizeof(float),
+                           hipMemcpyDeviceToHost, s->stream));
+    if (stats)
+        HIP_TRY(hipMemcpyAsync(stats, s->stats, 3 * rows * sizeof(float), hipMemcpyDeviceToHost,
+                               s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    return TILTGPU_OK;
+}
+
+int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n) {
+    if (!s || !out || n < 0) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int k = std::min<int>(n, s->sirt_done);
+    if (k > 0) HIP_TRY(hipMemcpy(out, s->resid, k * sizeof(double), hipMemcpyDeviceToHost));
+    return k;
+}
+
+int tiltgpu_read_filtered(tiltgpu_session* s, float* out) {
+    if (!s || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->o.storage == TILTGPU_STORE_F32 || s->seam) {
+        HIP_TRY(hipMemcpy(out, s->filt, nproj * 4, hipMemcpyDeviceToHost));
+        return TILTGPU_OK;
+    }
+    std::vector<unsigned short> h(nproj);
+    std::vector<float> sc(s->d.nz);
+    HIP_TRY(hipMemcpy(h.data(), s->store, nproj * 2, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(sc.data(), s->scale, s->d.nz * 4, hipMemcpyDeviceToHost));
+    const int64_t nxy = static_cast<int64_t>(s->d.in_ny) * s->d.in_nx;
+    for (int64_t i = 0; i < nproj; ++i) {
+        float v;
+        if (s->o.storage == TILTGPU_STORE_BF16) {
+            const uint32_t u = static_cast<uint32_t>(h[i]) << 16;
+            std::memcpy(&v, &u, 4);
+        } else {
+            _Float16 hv;
+            std::memcpy(&hv, &h[i], 2);
+            v = static_cast<float>(hv);
+        }
+        out[i] = v * sc[i / nxy];
+    }
+    return TILTGPU_OK;
+}
+
+int tiltgpu_read_means(tiltgpu_session* s, float* out) {
+    if (!s || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->means, s->d.nz * 4, hipMemcpyDeviceToHost));
+    return TILTGPU_OK;
+}
+
+int tiltgpu_synchronize(tiltgpu_session* s) {
+    if (!s) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    return TILTGPU_OK;
+}
+
+int tiltgpu_get_timing(tiltgpu_session* s, tiltgpu_timing* t) {
+    if (!s || !t) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    t->prep_ms = s->ms[kClsPrep];
+    t->filter_ms = s->ms[kClsFilter];
+    t->bp_ms = s->ms[kClsBp];
+    t->fp_ms = s->ms[kClsFp];
+    t->other_ms = s->ms[kClsOther];
+    t->bp_launches = s->n_bp;
+    t->filter_launches = s->n_filter;
+    t->fp_launches = s->n_fp;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_reset_timing(tiltgpu_session* s) {
+    if (!s) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    for (double& m : s->ms) m = 0;
+    s->n_bp = s->n_filter = s->n_fp = 0;
+    return TILTGPU_OK;
+}
+
+int64_t tiltgpu_device_bytes(const tiltgpu_session* s) { return s ? s->bytes : 0; }
+
+void tiltgpu_destroy(tiltgpu_session* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& r : s->pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+// ----------------------------------------------------- GpuSession-shaped seam
+
+int tiltgpu_bp_session_new(const float* projections, const tiltgpu_view_params* params,
+                           const float* proj_means, int32_t n_views, int32_t in_nx,
+                           int32_t in_ny, int32_t nz, int32_t out_nx, int32_t out_nz,
+                           int32_t fill_mode, tiltgpu_session** out) {
+    if (!out) return fail(TILTGPU_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    tiltgpu_dims d{in_nx, in_ny, nz, out_nx, out_nz};
+    int rc = validate_dims(&d);
+    if (rc) return rc;
+    if (!projections || !proj_means || n_views < 0 || n_views > nz || (n_views && !params))
+        return fail(TILTGPU_ERR_ARG, "bad arguments");
+    auto* s = new tiltgpu_session();
+    s->seam = true;
+    s->d = d;
+    tiltgpu_default_options(&s->o);
+    s->o.fill_mode = (fill_mode >= 0 && fill_mode <= 2) ? fill_mode : TILTGPU_FILL_MEAN;
+    const int64_t nproj = static_cast<int64_t>(nz) * in_ny * in_nx;
+    std::vector<tiltgpu_view_params> vp(params, params + n_views);
+    if ((rc = init_device(s)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
+        (rc = dev_alloc(s, &s->means, nz)) || (rc = setup_geometry(s, vp))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(s->filt, projections, nproj * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->means, proj_means, nz * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        tiltgpu_destroy(s);
+        return fail(TILTGPU_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+    }
+    s->uploaded = true;
+    *out = s;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out) {
+    if (!s || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    if (!s->seam) return fail(TILTGPU_ERR_STATE, "not a backprojection seam session");
+    if (y0 < 0 || y1 > s->d.in_ny || y0 >= y1) return fail(TILTGPU_ERR_ARG, "bad row range");
+    const int rows = y1 - y0;
+    const int64_t per = static_cast<int64_t>(s->d.out_nz) * s->d.out_nx;
+    if (!s->vol || s->stats_rows < rows) {
+        // the seam keeps a slab buffer sized by the largest request so far
+        dev_free(s, s->vol);
+        int rc = dev_alloc(s, &s->vol, rows * per);
+        if (rc) return rc;
+        s->stats_rows = rows;
+    }
+    int rc = run_bp(s, s->filt, nullptr, y0, y1, 0, s->vol);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, s->vol, rows * per * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    return check_kernel_error(s);
+}
+
+int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out) {
+    return tiltgpu_reconstruct_rows(s, iy, iy + 1, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Internal declarations shared by the libtiltgpu translation units: kernel
+// parameter blocks and the host-side launch functions.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tiltgpu {
+
+constexpr int kWave = 64;        // CDNA wavefront width
+constexpr int kBpThreads = 256;  // 4 waves per backprojection workgroup
+constexpr int kBpTileX = 64;     // one wave spans 64 output x positions
+constexpr int kMaxRadix = 20;
+
+// Backprojection of one series (one launch covers Y rows [y0, y1)).
+//
+// Layout in HBM:
+//   proj    [nz][in_ny][in_nx]  storage type (f32, f16 or bf16)
+//   out     [y1-y0][out_nz][out_nx] f32 (the MRC section order)
+//   zc_tab  [nviews][nz_pad] f32: (dz * sin_t) * zfactor, dz = oz - out_nz/2
+//   vgeom   [nviews] float4 (cos_t, sin_t, cos_xt, sin_xt)
+struct BpParams {
+    const void* proj;
+    const float* view_scale;  // [nz] dequantisation scale (f16/bf16), else null
+    const float4* vgeom;
+    const float* zc_tab;
+    const float* means;       // [nz] per-view means (Mean fill)
+    float* out;
+    int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
+    int y0, y1;
+    int ntx, ntz;             // tiles along x (64 wide) and z (tile_z deep)
+    float cx, ocx, inv_n;
+    int fill_mode;
+    int out_mode;             // 0: acc/nz, 1: output transform, 2: out += relax*acc/nz
+    float dw, os, oa, relax;
+    // local alignment grids (has_local != 0)
+    int has_local;
+    const int* lnx;
+    const int* lny;
+    const int* lxoff;
+    const int* lyoff;
+    const int* lgoff;
+    const float* lxs;
+    const float* lys;
+    const float* ldx;
+    const float* ldy;
+    int lds_cap;              // capacity of one LDS buffer in entries
+    int* err;                 // set to 1 when a tile needs more than lds_cap entries
+};
+
+// R-weighting filter + cosine stretch of rows of length n.
+struct FilterParams {
+    const float* src;       // [nrows][n]
+    const float* src_sub;   // optional: rows are src - src_sub (SIRT residual)
+    float* dst;             // [nrows][n]
+    const float* weights;   // [n] radial weights W(k)
+    const float2* tw;       // [n] exp(-2*pi*i*t/n)
+    const float* stretch;   // [nviews] 1/cos(theta) or 0 (no stretch); null = none
+    double* sq_partial;     // optional: per-workgroup sum of squares of the input rows
+    int n, nrows, rows_per_view;
+    int pairs_per_wg;
+    int cubic;
+    int nradix;
+    int radix[kMaxRadix];
+};
+
+// Gather-form forward projection (no x-axis tilt, no local alignment).
+struct FpParams {
+    const float* vol;     // [ny][out_nz][out_nx]
+    float* sino;          // [nz][ny][in_nx]
+    const float4* vgeom;
+    const float* zc_tab;
+    int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
+    float cx, ocx;
+};
+
+// Scatter-form forward projection (x-axis tilt and/or local alignment).
+struct FpScatterParams {
+    const float* vol;
+    float* sino;
+    const float4* vgeom;
+    const float* zc_tab;
+    int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
+    float cx, ocx;
+    int has_local;
+    const int* lnx;
+    const int* lny;
+    const int* lxoff;
+    const int* lyoff;
+    const int* lgoff;
+    const float* lxs;
+    const float* lys;
+    const float* ldx;
+    const float* ldy;
+};
+
+// ---- launchers (kernels_*.hip); all asynchronous on `st` ----
+hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z,
+                              hipStream_t st);
+size_t bp_lds_bytes(int lds_cap, int two_row);
+
+hipError_t launch_filter(const FilterParams& p, hipStream_t st);
+size_t filter_lds_bytes(int n, int pairs_per_wg);
+int filter_plan(int n, int* radix, int* nradix);  // 0 ok, -1 unsupported n
+
+hipError_t launch_log_means(const float* proj, float* logd, int64_t nxy, int nz, int do_log,
+                            double* partial, int nblk, float* means, hipStream_t st);
+hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
+                                  unsigned* absmax_bits, int64_t nxy, int nz, int storage,
+                                  hipStream_t st);
+hipError_t launch_forward_gather(const FpParams& p, hipStream_t st);
+hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st);
+hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
+                               hipStream_t st);
+hipError_t launch_transform(float* v, int64_t n, float dw, float os, float oa,
+                            hipStream_t st);
+hipError_t launch_lowpass_x(const float* src, float* dst, int ny, int nz, int nx,
+                            hipStream_t st);
+hipError_t launch_lowpass_z(const float* src, float* dst, int ny, int nz, int nx,
+                            hipStream_t st);
+hipError_t launch_lowpass_y(const float* src, float* dst, int ny, int nz, int nx,
+                            hipStream_t st);
+hipError_t launch_slice_stats(const float* v, int rows, int64_t per_row, float* stats,
+                              hipStream_t st);
+hipError_t launch_fill_f32(float* p, int64_t n, float value, hipStream_t st);
+
+}  // namespace tiltgpu

@@ -1,0 +1,404 @@
+"""Python front end of libtiltgpu (ctypes over include/tiltgpu.h).
+
+Mirrors the interfaces of the `tilt` reconstruction core of imod-rs:
+
+* ``TiltOptions`` -- the option surface of apps/tilt/src/main.rs:18-120, same
+  names and defaults;
+* ``reconstruct`` -- the pipeline of main.rs:601-1093 (after file reading) on
+  the GPU: returns the output volume [ny][out_nz][out_nx] and the header
+  statistics (amin, amax, amean);
+* ``GpuSession`` -- the GpuSession seam of apps/tilt/src/gpu.rs:26-269
+  (``new`` returns None when no GPU can be used, ``reconstruct_row``), fed with
+  what the CPU path also needs so its rows equal main.rs:1011-1073.
+
+There is no CPU fallback: when libtiltgpu.so is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libtiltgpu.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NODEV, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_KERNEL = 0, -1, -2, -3, -4, -5, -6, -7
+FILL = {"mean": 0, "zero": 1, "edge": 2}
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+
+
+class TiltGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tiltgpu error {code}: {msg}")
+        self.code = code
+
+
+class Dims(C.Structure):
+    _fields_ = [("in_nx", C.c_int32), ("in_ny", C.c_int32), ("nz", C.c_int32),
+                ("out_nx", C.c_int32), ("out_nz", C.c_int32)]
+
+
+class Options(C.Structure):
+    _fields_ = [("radial_cutoff", C.c_float), ("radial_falloff", C.c_float),
+                ("no_weight", C.c_int32), ("exact_filter_size", C.c_int32),
+                ("hamming", C.c_int32), ("no_cosine_stretch", C.c_int32),
+                ("cosine_cubic", C.c_int32), ("max_cosine_tilt", C.c_float),
+                ("log", C.c_int32), ("fill_mode", C.c_int32),
+                ("densweight", C.c_float), ("outscale", C.c_float), ("outadd", C.c_float),
+                ("sirt", C.c_int32), ("sirt_relax", C.c_float), ("fake_sirt", C.c_int32),
+                ("storage", C.c_int32)]
+
+
+class ViewParams(C.Structure):
+    _fields_ = [("cos_t", C.c_float), ("sin_t", C.c_float), ("cos_xt", C.c_float),
+                ("sin_xt", C.c_float), ("zfactor", C.c_float)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("prep_ms", C.c_double), ("filter_ms", C.c_double), ("bp_ms", C.c_double),
+                ("fp_ms", C.c_double), ("other_ms", C.c_double),
+                ("bp_launches", C.c_int64), ("filter_launches", C.c_int64),
+                ("fp_launches", C.c_int64)]
+
+
+class Phantom(C.Structure):
+    _fields_ = [("vol_nx", C.c_int32), ("vol_ny", C.c_int32), ("vol_nz", C.c_int32),
+                ("n_ellipsoids", C.c_int32), ("n_beads", C.c_int32),
+                ("axis_min", C.c_float), ("axis_max", C.c_float),
+                ("bead_radius", C.c_float), ("bead_density", C.c_float),
+                ("noise_rel", C.c_float), ("seed", C.c_uint64)]
+
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+# (name, restype, argtypes) of every symbol include/tiltgpu.h declares.
+SIGNATURES = [
+    ("tiltgpu_last_error", C.c_char_p, []),
+    ("tiltgpu_version", C.c_char_p, []),
+    ("tiltgpu_device_count", C.c_int, []),
+    ("tiltgpu_default_options", None, [C.POINTER(Options)]),
+    ("tiltgpu_proj_params", C.c_int, [_fp, C.c_int32, _fp, C.c_int32, _fp, C.c_int32,
+                                      C.POINTER(ViewParams)]),
+    ("tiltgpu_radial_weights", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_float,
+                                         C.c_int32, _fp]),
+    ("tiltgpu_create", C.c_int, [C.POINTER(Dims), C.POINTER(Options), C.POINTER(_vp)]),
+    ("tiltgpu_set_angles", C.c_int, [_vp, _fp, C.c_int32]),
+    ("tiltgpu_set_xtilt", C.c_int, [_vp, _fp, C.c_int32]),
+    ("tiltgpu_set_zfactor", C.c_int, [_vp, _fp, C.c_int32]),
+    ("tiltgpu_set_local_alignment", C.c_int, [_vp, _ip, _ip, _fp, _fp, _fp, _fp]),
+    ("tiltgpu_upload_projections", C.c_int, [_vp, _fp, C.c_int64]),
+    ("tiltgpu_run", C.c_int, [_vp]),
+    ("tiltgpu_read_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
+    ("tiltgpu_sirt_residuals", C.c_int, [_vp, _dp, C.c_int32]),
+    ("tiltgpu_read_filtered", C.c_int, [_vp, _fp]),
+    ("tiltgpu_read_means", C.c_int, [_vp, _fp]),
+    ("tiltgpu_synchronize", C.c_int, [_vp]),
+    ("tiltgpu_get_timing", C.c_int, [_vp, C.POINTER(Timing)]),
+    ("tiltgpu_reset_timing", C.c_int, [_vp]),
+    ("tiltgpu_device_bytes", C.c_int64, [_vp]),
+    ("tiltgpu_destroy", None, [_vp]),
+    ("tiltgpu_bp_session_new", C.c_int, [_fp, C.POINTER(ViewParams), _fp, C.c_int32,
+                                         C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    ("tiltgpu_reconstruct_row", C.c_int, [_vp, C.c_int32, _fp]),
+    ("tiltgpu_reconstruct_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
+    ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
+    ("tiltgpu_synth_series", C.c_int, [C.POINTER(Phantom), C.c_int32, C.c_int32, C.c_int32,
+                                       _fp, _fp, _fp, _fp, C.c_int32, _fp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libtiltgpu.so (built by imod-rs_amd/build.py); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TiltGpuError(ERR_STATE, f"{LIB_PATH} is missing: run imod-rs_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise TiltGpuError(rc, lib().tiltgpu_last_error().decode(errors="replace"))
+    return rc
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def device_count() -> int:
+    return lib().tiltgpu_device_count()
+
+
+@dataclass
+class TiltOptions:
+    """apps/tilt/src/main.rs:18-120 (file names aside) plus ``storage``."""
+    thickness: int = 0
+    width: int | None = None
+    radial: tuple = (0.35, 0.05)
+    no_weight: bool = False
+    sirt: int = 0
+    sirt_relax: float = 1.0
+    no_cosine_stretch: bool = False
+    log: bool = False
+    densweight: float = 1.0
+    outscale: float = 1.0
+    outadd: float = 0.0
+    exact_filter_size: int | None = None
+    fill_mode: str = "mean"
+    cosine_interp: str = "linear"
+    max_cosine_tilt: float = 80.0
+    hamming: bool = False
+    fake_sirt: int = 0
+    storage: str = "f32"
+
+    def to_c(self) -> Options:
+        o = Options()
+        lib().tiltgpu_default_options(C.byref(o))
+        o.radial_cutoff, o.radial_falloff = float(self.radial[0]), float(self.radial[1])
+        o.no_weight = int(self.no_weight)
+        o.exact_filter_size = int(self.exact_filter_size or 0)
+        o.hamming = int(self.hamming)
+        o.no_cosine_stretch = int(self.no_cosine_stretch)
+        o.cosine_cubic = int(self.cosine_interp.lower() == "cubic")
+        o.max_cosine_tilt = float(self.max_cosine_tilt)
+        o.log = int(self.log)
+        o.fill_mode = FILL.get(self.fill_mode.lower(), 0)  # unknown -> mean (main.rs:593)
+        o.densweight, o.outscale, o.outadd = self.densweight, self.outscale, self.outadd
+        o.sirt, o.sirt_relax, o.fake_sirt = int(self.sirt), float(self.sirt_relax), int(self.fake_sirt)
+        o.storage = STORAGE[self.storage]
+        return o
+
+
+def proj_params(angles, xtilt=None, zfactor=None):
+    ang = _f32(angles)
+    xt, zf = _f32(xtilt), _f32(zfactor)
+    out = (ViewParams * max(1, ang.size))()
+    _check(lib().tiltgpu_proj_params(_f(ang), ang.size, _f(xt), 0 if xt is None else xt.size,
+                                     _f(zf), 0 if zf is None else zf.size, out))
+    return out
+
+
+def radial_weights(n, cutoff=0.35, falloff=0.05, exact_filter_size=0, hamming=False):
+    out = np.zeros(n, np.float32)
+    _check(lib().tiltgpu_radial_weights(n, exact_filter_size or 0, cutoff, falloff,
+                                        int(hamming), _f(out)))
+    return out
+
+
+class Session:
+    """One reconstruction resident on the GPU (tiltgpu_create ... tiltgpu_destroy)."""
+
+    def __init__(self, in_nx, in_ny, nz, opts: TiltOptions | None = None):
+        self.opts = opts or TiltOptions()
+        self.out_nx = self.opts.width or in_nx
+        self.out_nz = self.opts.thickness if self.opts.thickness > 0 else in_nx
+        self.dims = Dims(in_nx, in_ny, nz, self.out_nx, self.out_nz)
+        self._copts = self.opts.to_c()
+        h = _vp()
+        _check(lib().tiltgpu_create(C.byref(self.dims), C.byref(self._copts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tiltgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_angles(self, a):
+        a = _f32(a)
+        _check(lib().tiltgpu_set_angles(self.h, _f(a), a.size))
+
+    def set_xtilt(self, a):
+        a = _f32(a)
+        _check(lib().tiltgpu_set_xtilt(self.h, _f(a), a.size))
+
+    def set_zfactor(self, a):
+        a = _f32(a)
+        _check(lib().tiltgpu_set_zfactor(self.h, _f(a), a.size))
+
+    def set_local(self, nx, ny, xs, ys, dx, dy):
+        nx = np.ascontiguousarray(nx, np.int32)
+        ny = np.ascontiguousarray(ny, np.int32)
+        self._local = [_f32(v) if np.size(v) else np.zeros(1, np.float32) for v in (xs, ys, dx, dy)]
+        _check(lib().tiltgpu_set_local_alignment(
+            self.h, nx.ctypes.data_as(_ip), ny.ctypes.data_as(_ip), *[_f(v) for v in self._local]))
+
+    def upload(self, P):
+        P = _f32(P)
+        _check(lib().tiltgpu_upload_projections(self.h, _f(P), P.size))
+
+    def run(self):
+        _check(lib().tiltgpu_run(self.h))
+
+    def read_rows(self, y0=0, y1=None, stats=True):
+        y1 = self.dims.in_ny if y1 is None else y1
+        out = np.empty((y1 - y0, self.out_nz, self.out_nx), np.float32)
+        st = np.empty((y1 - y0, 3), np.float32) if stats else None
+        _check(lib().tiltgpu_read_rows(self.h, y0, y1, _f(out), _f(st)))
+        return out, st
+
+    def residuals(self):
+        out = np.zeros(max(1, self.opts.sirt), np.float64)
+        k = _check(lib().tiltgpu_sirt_residuals(self.h, out.ctypes.data_as(_dp), out.size))
+        return out[:k]
+
+    def read_filtered(self):
+        out = np.empty((self.dims.nz, self.dims.in_ny, self.dims.in_nx), np.float32)
+        _check(lib().tiltgpu_read_filtered(self.h, _f(out)))
+        return out
+
+    def read_means(self):
+        out = np.empty(self.dims.nz, np.float32)
+        _check(lib().tiltgpu_read_means(self.h, _f(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib().tiltgpu_synchronize(self.h))
+
+    def timing(self):
+        t = Timing()
+        _check(lib().tiltgpu_get_timing(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def reset_timing(self):
+        _check(lib().tiltgpu_reset_timing(self.h))
+
+    def device_bytes(self):
+        return lib().tiltgpu_device_bytes(self.h)
+
+
+def header_stats(row_stats, per_row):
+    """Header amin/amax/amean from per-section (min, max, mean) as main.rs:1080-1092
+    combines them: f64 running sum of smean * (out_nx*out_nz)."""
+    gmin = float(np.min(row_stats[:, 0])) if len(row_stats) else float(np.finfo(np.float32).max)
+    gmax = float(np.max(row_stats[:, 1])) if len(row_stats) else float(np.finfo(np.float32).min)
+    gsum = 0.0
+    for m in row_stats[:, 2]:
+        gsum += float(m) * float(per_row)
+    gmean = float(np.float32(gsum / (per_row * len(row_stats)))) if len(row_stats) else 0.0
+    return gmin, gmax, gmean
+
+
+def reconstruct(P, angles, opts: TiltOptions | None = None, xtilt=None, zfactor=None,
+                local=None, return_session=False):
+    """The whole reconstruction on the GPU.  P: raw projections [nz][ny][nx];
+    local: (nx, ny, xs, ys, dx, dy) flat per-view grids or an object with .flat()."""
+    P = _f32(P)
+    nz, ny, nx = P.shape
+    s = Session(nx, ny, nz, opts)
+    s.set_angles(angles)
+    if xtilt is not None:
+        s.set_xtilt(xtilt)
+    if zfactor is not None:
+        s.set_zfactor(zfactor)
+    if local is not None:
+        s.set_local(*(local.flat() if hasattr(local, "flat") else local))
+    s.upload(P)
+    s.run()
+    vol, st = s.read_rows()
+    hs = header_stats(st, s.out_nx * s.out_nz)
+    if return_session:
+        return vol, hs, s
+    s.close()
+    return vol, hs
+
+
+class GpuSession:
+    """Shape of GpuSession (apps/tilt/src/gpu.rs:26-269) over tiltgpu_bp_session_new."""
+
+    def __init__(self, h, out_nx, out_nz):
+        self.h, self.out_nx, self.out_nz = h, out_nx, out_nz
+
+    @classmethod
+    def new(cls, projections, params, proj_means, in_nx, in_ny, n_projs, out_nx, out_nz,
+            fill_mode="mean"):
+        """Returns None when no session can be created (gpu.rs:43-74 -> Option)."""
+        P = _f32(projections)
+        m = _f32(proj_means)
+        if not isinstance(params, C.Array):
+            params = proj_params(*params)
+        h = _vp()
+        rc = lib().tiltgpu_bp_session_new(_f(P), params, _f(m), len(params), in_nx, in_ny,
+                                          n_projs, out_nx, out_nz, FILL[fill_mode], C.byref(h))
+        if rc != OK:
+            return None
+        return cls(h, out_nx, out_nz)
+
+    def reconstruct_row(self, iy):
+        out = np.empty(self.out_nx * self.out_nz, np.float32)
+        _check(lib().tiltgpu_reconstruct_row(self.h, iy, _f(out)))
+        return out
+
+    def reconstruct_rows(self, y0, y1):
+        out = np.empty((y1 - y0, self.out_nz, self.out_nx), np.float32)
+        _check(lib().tiltgpu_reconstruct_rows(self.h, y0, y1, _f(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().tiltgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_tilt_file(path, cap=100000):
+    out = np.zeros(cap, np.float32)
+    n = _check(lib().tiltgpu_read_tilt_file(path.encode(), _f(out), cap))
+    return out[:min(n, cap)]
+
+
+def read_per_view_file(path, cap=100000):
+    out = np.zeros(cap, np.float32)
+    n = _check(lib().tiltgpu_read_per_view_file(path.encode(), _f(out), cap))
+    return out[:min(n, cap)]
+
+
+def synth_series(nx, ny, angles, vol_nz, n_ellipsoids=200, n_beads=30, seed=0, noise_rel=0.1,
+                 xtilt=None, shift_x=None, shift_y=None, vol_nx=None, axis=(5.0, 40.0),
+                 bead_radius=5.0, bead_density=10.0, nthreads=0):
+    """Seeded ellipsoid + bead phantom projected analytically with the tilt
+    geometry; returns [n_views][ny][nx] float32."""
+    ang = _f32(angles)
+    ph = Phantom(vol_nx or nx, ny, vol_nz, n_ellipsoids, n_beads, axis[0], axis[1],
+                 bead_radius, bead_density, noise_rel, seed)
+    out = np.empty((ang.size, ny, nx), np.float32)
+    xt, sx, sy = _f32(xtilt), _f32(shift_x), _f32(shift_y)
+    _check(lib().tiltgpu_synth_series(C.byref(ph), nx, ny, ang.size, _f(ang), _f(xt), _f(sx),
+                                      _f(sy), nthreads, _f(out)))
+    return out

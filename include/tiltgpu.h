@@ -1,0 +1,212 @@
+/*
+ * tiltgpu.h -- C ABI of libtiltgpu, an MI355X (gfx950) implementation of the
+ * reconstruction core of the `tilt` app of imod-rs: R-weighted backprojection
+ * (row FFT radial filter, cosine stretch, per-slice linear-interpolation
+ * backprojection over views) and the SIRT reproject/backproject iteration.
+ *
+ * Plain C types only (pointers, sizes, fixed-width integers).  Every call
+ * returns an int status: 0 = OK, negative = error; tiltgpu_last_error() gives
+ * the message of the calling thread's last failure.  No exception or abort
+ * crosses this boundary.  The caller owns every host buffer; a session owns its
+ * device memory and its HIP stream.  One session is used by one host thread at
+ * a time; the server API (tiltgpu_server_*) is thread-safe.
+ *
+ * Reference interfaces this library replaces (paths relative to imod-rs):
+ *   apps/tilt/src/gpu.rs:26-269      GpuSession (the in-app GPU seam)
+ *   apps/tilt/src/main.rs:601-1093   the CPU reconstruction pipeline
+ *   apps/tilt/src/main.rs:18-120     the option surface it is driven by
+ */
+#ifndef TILTGPU_H
+#define TILTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define TILTGPU_OK 0
+#define TILTGPU_ERR_ARG (-1)     /* invalid argument or unsupported shape      */
+#define TILTGPU_ERR_HIP (-2)     /* a HIP runtime call failed                  */
+#define TILTGPU_ERR_NODEV (-3)   /* no usable GPU                              */
+#define TILTGPU_ERR_BUDGET (-4)  /* request exceeds the device-memory budget   */
+#define TILTGPU_ERR_STATE (-5)   /* call out of order (e.g. run before upload) */
+#define TILTGPU_ERR_IO (-6)      /* file could not be read or written          */
+#define TILTGPU_ERR_KERNEL (-7)  /* a kernel reported a violated precondition  */
+
+/* Fill value for out-of-range projection samples (main.rs:122-128,589-594). */
+#define TILTGPU_FILL_MEAN 0
+#define TILTGPU_FILL_ZERO 1
+#define TILTGPU_FILL_EDGE 2
+
+/* Storage precision of the filtered projections held in HBM.  Accumulation is
+ * always FP32; F16/BF16 carry one FP32 scale per view. */
+#define TILTGPU_STORE_F32 0
+#define TILTGPU_STORE_F16 1
+#define TILTGPU_STORE_BF16 2
+
+/* Sizes of one reconstruction.  in_* come from the MRC header of the aligned
+ * stack (main.rs:560-571): nz = number of sections; out_ny = in_ny. */
+typedef struct {
+    int32_t in_nx, in_ny, nz;
+    int32_t out_nx;  /* --width, default in_nx                    */
+    int32_t out_nz;  /* --thickness, default in_nx (when <= 0)    */
+} tiltgpu_dims;
+
+/* The `tilt` option surface (main.rs:18-120).  tiltgpu_default_options() fills
+ * in the reference defaults. */
+typedef struct {
+    float radial_cutoff, radial_falloff; /* --radial "0.35,0.05"              */
+    int32_t no_weight;                   /* --no-weight                        */
+    int32_t exact_filter_size;           /* --exact-filter-size, 0 = n/2+1     */
+    int32_t hamming;                     /* --hamming                          */
+    int32_t no_cosine_stretch;           /* --no-cosine-stretch                */
+    int32_t cosine_cubic;                /* --cosine-interp cubic              */
+    float max_cosine_tilt;               /* --max-cosine-tilt 80               */
+    int32_t log;                         /* --log                              */
+    int32_t fill_mode;                   /* --fill-mode (TILTGPU_FILL_*)       */
+    float densweight, outscale, outadd;  /* output transform                   */
+    int32_t sirt;                        /* --sirt N                           */
+    float sirt_relax;                    /* --sirt-relax                       */
+    int32_t fake_sirt;                   /* --fake-sirt N                      */
+    int32_t storage;                     /* TILTGPU_STORE_* (build addition)   */
+} tiltgpu_options;
+
+/* Per-view geometry, the ProjParams of main.rs:130-140,682-700, in FP32 as the
+ * reference computes it (the host fills it with tiltgpu_proj_params). */
+typedef struct {
+    float cos_t, sin_t, cos_xt, sin_xt, zfactor;
+} tiltgpu_view_params;
+
+/* Accumulated device time per kernel class since the last reset (ms), measured
+ * with HIP events on the session's stream. */
+typedef struct {
+    double prep_ms;      /* log + per-view means                       */
+    double filter_ms;    /* R-weighting + cosine stretch (+ storage)   */
+    double bp_ms;        /* backprojection                             */
+    double fp_ms;        /* forward projection (SIRT)                  */
+    double other_ms;     /* SIRT update, lowpass, statistics           */
+    int64_t bp_launches;
+    int64_t filter_launches;
+    int64_t fp_launches;
+} tiltgpu_timing;
+
+typedef struct tiltgpu_session tiltgpu_session;
+
+/* ---------------------------------------------------------------- utility */
+const char* tiltgpu_last_error(void);
+const char* tiltgpu_version(void);
+/* Number of visible GPUs (0 when none); never fails. */
+int tiltgpu_device_count(void);
+void tiltgpu_default_options(tiltgpu_options* opts);
+
+/* ProjParams table for the first n views (main.rs:682-700): rad = deg*PI/180 in
+ * FP32, cos/sin in FP32; xtilt / zfactor may be NULL (main.rs:689-691 defaults). */
+int tiltgpu_proj_params(const float* angles_deg, int32_t n, const float* xtilt_deg,
+                        int32_t n_xtilt, const float* zfactor, int32_t n_zfactor,
+                        tiltgpu_view_params* out);
+
+/* Radial filter weights W[0..n) (main.rs:348-369). */
+int tiltgpu_radial_weights(int32_t n, int32_t exact_filter_size, float cutoff,
+                           float falloff, int32_t hamming, float* out);
+
+/* ----------------------------------------------- full pipeline (drop-in) */
+
+/* Create a session for one reconstruction; allocates the device buffers.
+ * Replaces GpuSession::new (apps/tilt/src/gpu.rs:43-51), but carries the whole
+ * option surface so the device path matches the CPU path of main.rs. */
+int tiltgpu_create(const tiltgpu_dims* dims, const tiltgpu_options* opts,
+                   tiltgpu_session** out);
+
+/* Tilt angles in degrees as read by read_tilt_file
+ * (crates/imod-transforms/src/tilt.rs:6-23).  Views iterated = min(nz, n);
+ * cosine stretch needs n >= nz (main.rs:449-450 indexes every section). */
+int tiltgpu_set_angles(tiltgpu_session* s, const float* angles_deg, int32_t n);
+/* --xtilt-file and --zfactor-file contents (main.rs:664-680,689-691). */
+int tiltgpu_set_xtilt(tiltgpu_session* s, const float* xtilt_deg, int32_t n);
+int tiltgpu_set_zfactor(tiltgpu_session* s, const float* zfactor, int32_t n);
+/* --local-file grids (main.rs:232-307), one per section v < nz: an nx[v] by
+ * ny[v] grid with sorted positions xs/ys and row-major dx/dy [iy*nx+ix];
+ * arrays are the per-view pieces concatenated in view order. */
+int tiltgpu_set_local_alignment(tiltgpu_session* s, const int32_t* nx, const int32_t* ny,
+                                const float* xs, const float* ys, const float* dx,
+                                const float* dy);
+
+/* Raw projections [nz][in_ny][in_nx] (main.rs:601-605), before --log. */
+int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t count);
+
+/* Run the reconstruction on the device: log, per-view means, R-weighting,
+ * cosine stretch, then WBP, SIRT or fake-SIRT, and the output transform.  The
+ * volume stays resident in HBM.  Replaces main.rs:607-1090 minus file I/O. */
+int tiltgpu_run(tiltgpu_session* s);
+
+/* Copy output Y rows [y0, y1) to host as [y][out_nz][out_nx] (the MRC section
+ * order of main.rs:1077-1085) with per-row statistics min, max, mean
+ * (min_max_mean of each section; stats may be NULL). */
+int tiltgpu_read_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out,
+                      float* stats);
+
+/* SIRT residual norms ||O - FP(V)||_2 per iteration, reduced in FP64 on the
+ * device (a build addition: the reference computes none, main.rs:928-953). */
+int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n);
+
+/* Copy the filtered (and stretched) projections back, dequantised to FP32,
+ * [nz][in_ny][in_nx]; for parity tests of the filter stage. */
+int tiltgpu_read_filtered(tiltgpu_session* s, float* out);
+/* Per-view means (main.rs:615-623) as computed on the device. */
+int tiltgpu_read_means(tiltgpu_session* s, float* out);
+
+int tiltgpu_synchronize(tiltgpu_session* s);
+int tiltgpu_get_timing(tiltgpu_session* s, tiltgpu_timing* t);
+int tiltgpu_reset_timing(tiltgpu_session* s);
+/* Device bytes held by the session. */
+int64_t tiltgpu_device_bytes(const tiltgpu_session* s);
+void tiltgpu_destroy(tiltgpu_session* s);
+
+/* ------------------------------------------- GpuSession-shaped seam entry */
+
+/* Backprojection of already filtered and stretched projections, the shape of
+ * GpuSession::new (apps/tilt/src/gpu.rs:43-51) with what the CPU path also
+ * needs (per-view params incl. xtilt/zfactor, proj means, fill mode), so the
+ * result equals main.rs:1011-1073 rather than the WGSL shader's. */
+int tiltgpu_bp_session_new(const float* projections, const tiltgpu_view_params* params,
+                           const float* proj_means, int32_t n_views, int32_t in_nx,
+                           int32_t in_ny, int32_t nz, int32_t out_nx, int32_t out_nz,
+                           int32_t fill_mode, tiltgpu_session** out);
+/* GpuSession::reconstruct_row (gpu.rs:224): one [out_nz][out_nx] slice, already
+ * multiplied by 1/nz, no output transform. */
+int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out);
+/* The same for rows [y0, y1) in one launch. */
+int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out);
+
+/* -------------------------------------------------------- host utilities */
+
+/* read_tilt_file (crates/imod-transforms/src/tilt.rs:6-23).  Writes at most cap
+ * angles; returns the count (>= 0) or an error. */
+int tiltgpu_read_tilt_file(const char* path, float* out, int32_t cap);
+/* load_per_view_file (main.rs:310-326); a parse error is TILTGPU_ERR_IO. */
+int tiltgpu_read_per_view_file(const char* path, float* out, int32_t cap);
+
+/* Synthetic tilt series: the seeded ellipsoid + bead phantom projected
+ * analytically with the tilt geometry (per-view angle, x-axis tilt, in-plane
+ * shift), plus Gaussian noise of sigma = noise_rel * std of the noiseless
+ * stack.  out = [n_views][ny][nx].  Runs on host threads. */
+typedef struct {
+    int32_t vol_nx, vol_ny, vol_nz;  /* phantom box (x, y, thickness)      */
+    int32_t n_ellipsoids, n_beads;
+    float axis_min, axis_max;        /* ellipsoid semi-axes (px)           */
+    float bead_radius, bead_density;
+    float noise_rel;
+    uint64_t seed;
+} tiltgpu_phantom;
+int tiltgpu_synth_series(const tiltgpu_phantom* ph, int32_t nx, int32_t ny,
+                         int32_t n_views, const float* angles_deg, const float* xtilt_deg,
+                         const float* shift_x, const float* shift_y, int32_t nthreads,
+                         float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TILTGPU_H */

@@ -1,0 +1,380 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import
+this module.  It restates the `tilt` reconstruction core of imod-rs
+(apps/tilt/src/main.rs) on the CPU, in two forms:
+
+* ``f32`` -- the reference's FP32 operation order (C, -ffp-contract=off);
+* ``f64`` -- the same algorithm in double precision (the FP64 oracle).
+
+It also restates the text parsers the path uses: ``read_tilt_file``
+(crates/imod-transforms/src/tilt.rs:6-23), ``load_per_view_file``
+(apps/tilt/src/main.rs:310-326) and ``load_local_alignment`` (main.rs:232-307).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+FILL = {"mean": 0, "zero": 1, "edge": 2}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrOpts(C.Structure):
+    _fields_ = [
+        ("in_nx", C.c_int), ("in_ny", C.c_int), ("nz", C.c_int),
+        ("out_nx", C.c_int), ("out_nz", C.c_int),
+        ("cutoff", C.c_float), ("falloff", C.c_float),
+        ("no_weight", C.c_int), ("exact_filter_size", C.c_int), ("hamming", C.c_int),
+        ("no_cosine_stretch", C.c_int), ("cosine_cubic", C.c_int),
+        ("max_cosine_tilt", C.c_float),
+        ("log_", C.c_int), ("fill_mode", C.c_int),
+        ("densweight", C.c_float), ("outscale", C.c_float), ("outadd", C.c_float),
+        ("sirt", C.c_int), ("sirt_relax", C.c_float), ("fake_sirt", C.c_int),
+        ("nthreads", C.c_int),
+    ]
+
+
+def _declare(L):
+    L.or_min_max_mean.argtypes = [_fp, C.c_long, _fp]
+    for suf in ("f32", "f64"):
+        getattr(L, f"or_tilt_run_{suf}").argtypes = [
+            C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
+            _ip, _ip, _fp, _fp, _fp, _fp, _fp, _fp]
+        getattr(L, f"or_backproject_simple_{suf}").argtypes = [
+            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+            _fp, _fp, _fp, C.c_int, C.c_int, _fp, C.c_int]
+        getattr(L, f"or_forward_project_simple_{suf}").argtypes = [
+            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]
+        getattr(L, f"or_rweight_float_{suf}").argtypes = [
+            _fp, C.c_long, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
+        getattr(L, f"or_cosine_stretch_float_{suf}").argtypes = [
+            _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_float]
+        getattr(L, f"or_dft_float_{suf}").argtypes = [_fp, _fp, C.c_int, C.c_int]
+        getattr(L, f"or_local_interp_float_{suf}").argtypes = [
+            _fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float, C.c_float, _fp]
+    L.or_radial_weights_f32.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _fp]
+    L.or_radial_weights_f64.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _dp]
+    L.or_proj_params_f32.argtypes = [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
+                                     _fp, _fp, _fp, _fp, _fp]
+    L.or_proj_params_f64.argtypes = [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
+                                     _dp, _dp, _dp, _dp, _dp]
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+def _f32c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ------------------------------------------------------------------ parsers --
+
+def read_tilt_file(path):
+    """read_tilt_file, crates/imod-transforms/src/tilt.rs:6-23: one f32 per line,
+    blank lines skipped, unparsable lines silently skipped."""
+    out = []
+    with open(path, "r") as fh:
+        for line in fh:
+            t = line.strip()
+            if not t:
+                continue
+            v = _rust_parse_f32(t)
+            if v is not None:
+                out.append(v)
+    return np.array(out, dtype=np.float32)
+
+
+def load_per_view_file(path):
+    """load_per_view_file, main.rs:310-326: '#' and blank lines skipped, a parse
+    error raises (the reference exits with status 1)."""
+    out = []
+    with open(path, "r") as fh:
+        for line in fh:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            v = _rust_parse_f32(t)
+            if v is None:
+                raise ValueError(f"parse error: {t!r}")
+            out.append(v)
+    return np.array(out, dtype=np.float32)
+
+
+def _rust_parse_f32(t):
+    """str::parse::<f32>: decimal / exponent / inf / nan, no hex, no spaces."""
+    s = t
+    if not s or s != s.strip() or "_" in s:
+        return None
+    low = s.lower().lstrip("+-")
+    if low in ("inf", "infinity", "nan"):
+        return float(np.float32(float(s)))
+    if low.startswith("0x"):
+        return None
+    try:
+        return float(np.float32(float(s)))
+    except ValueError:
+        return None
+
+
+def _rust_parse_usize(t):
+    if not t or not t.lstrip("+").isdigit():
+        return None
+    return int(t)
+
+
+@dataclass
+class LocalGrids:
+    """Per-view patch grids (main.rs:153-168): xs, ys sorted unique, dx/dy
+    row-major [iy*nx + ix]."""
+    nz: int
+    xs: list = field(default_factory=list)
+    ys: list = field(default_factory=list)
+    dx: list = field(default_factory=list)
+    dy: list = field(default_factory=list)
+
+    def flat(self):
+        nx = np.array([len(x) for x in self.xs], dtype=np.int32)
+        ny = np.array([len(y) for y in self.ys], dtype=np.int32)
+        cat = lambda L: (np.concatenate([np.asarray(a, np.float32) for a in L])
+                         if sum(len(a) for a in L) else np.zeros(1, np.float32))
+        return nx, ny, cat(self.xs), cat(self.ys), cat(self.dx), cat(self.dy)
+
+
+def _dedup_sorted(vals):
+    """Vec::dedup_by(|a, b| (a - b).abs() < 0.01) on a sorted list: an element is
+    dropped when it is within 0.01 of the last element kept."""
+    out = []
+    for v in vals:
+        if out and abs(np.float32(v) - np.float32(out[-1])) < np.float32(0.01):
+            continue
+        out.append(v)
+    return out
+
+
+def build_local_grids(patches, nz):
+    """load_local_alignment grid building, main.rs:255-306.  patches = list of
+    (view, x, y, dx, dy)."""
+    g = LocalGrids(nz, [[] for _ in range(nz)], [[] for _ in range(nz)],
+                   [[] for _ in range(nz)], [[] for _ in range(nz)])
+    by_view = [[] for _ in range(nz)]
+    for p in patches:
+        if p[0] < nz:
+            by_view[p[0]].append(p)
+    for vi, vp in enumerate(by_view):
+        if not vp:
+            continue
+        xs = _dedup_sorted(sorted(np.float32(p[1]) for p in vp))
+        ys = _dedup_sorted(sorted(np.float32(p[2]) for p in vp))
+        nx, ny = len(xs), len(ys)
+        gdx = np.zeros(nx * ny, np.float32)
+        gdy = np.zeros(nx * ny, np.float32)
+        for p in vp:
+            ix = next((i for i, xv in enumerate(xs)
+                       if abs(np.float32(xv) - np.float32(p[1])) < np.float32(0.01)), 0)
+            iy = next((i for i, yv in enumerate(ys)
+                       if abs(np.float32(yv) - np.float32(p[2])) < np.float32(0.01)), 0)
+            gdx[iy * nx + ix] = p[3]
+            gdy[iy * nx + ix] = p[4]
+        g.xs[vi], g.ys[vi] = np.array(xs, np.float32), np.array(ys, np.float32)
+        g.dx[vi], g.dy[vi] = gdx, gdy
+    return g
+
+
+def parse_local_file(path):
+    """Line parsing of load_local_alignment, main.rs:237-253: '#'/blank skipped,
+    fewer than 5 fields skipped, each field unwrap_or(0)."""
+    patches = []
+    with open(path) as fh:
+        for line in fh:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            parts = t.split()
+            if len(parts) < 5:
+                continue
+            view = _rust_parse_usize(parts[0])
+            vals = [_rust_parse_f32(x) for x in parts[1:5]]
+            patches.append((view if view is not None else 0,
+                            *[v if v is not None else 0.0 for v in vals]))
+    return patches
+
+
+# ----------------------------------------------------------------- numerics --
+
+def min_max_mean(a):
+    a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    out = np.zeros(3, np.float32)
+    lib().or_min_max_mean(_f(a), a.size, _f(out))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def radial_weights(n, cutoff=0.35, falloff=0.05, exact_filter_size=0, hamming=False,
+                   prec="f32"):
+    dt = np.float32 if prec == "f32" else np.float64
+    out = np.zeros(n, dt)
+    ptr = out.ctypes.data_as(_fp if prec == "f32" else _dp)
+    getattr(lib(), f"or_radial_weights_{prec}")(n, exact_filter_size, cutoff, falloff,
+                                                 int(hamming), ptr)
+    return out
+
+
+def rweight(rows2d, cutoff=0.35, falloff=0.05, exact_filter_size=0, hamming=False,
+            prec="f32", nthreads=8):
+    a = np.array(rows2d, dtype=np.float32, copy=True)
+    n = a.shape[-1]
+    getattr(lib(), f"or_rweight_float_{prec}")(_f(a), a.size // n, n, cutoff, falloff,
+                                                exact_filter_size, int(hamming), nthreads)
+    return a
+
+
+def dft(x, sign=-1, prec="f64"):
+    x = np.asarray(x, np.complex128)
+    re = np.ascontiguousarray(x.real, np.float32)
+    im = np.ascontiguousarray(x.imag, np.float32)
+    getattr(lib(), f"or_dft_float_{prec}")(_f(re), _f(im), x.size, sign)
+    return re.astype(np.float64) + 1j * im.astype(np.float64)
+
+
+def cosine_stretch(P, angles, cubic=False, max_tilt=80.0, prec="f32"):
+    a = np.array(P, dtype=np.float32, copy=True)
+    nz, ny, nx = a.shape
+    ang = _f32c(angles)
+    assert ang.size >= nz
+    getattr(lib(), f"or_cosine_stretch_float_{prec}")(_f(a), nx, ny, nz, _f(ang),
+                                                       int(cubic), max_tilt)
+    return a
+
+
+def proj_params(angles, xtilt=None, zfactor=None, prec="f32"):
+    ang = _f32c(angles)
+    n = ang.size
+    dt = np.float32 if prec == "f32" else np.float64
+    tab = np.zeros((5, n), dt)
+    xt = _f32c(xtilt)
+    zf = _f32c(zfactor)
+    ptr = (lambda k: tab[k].ctypes.data_as(_fp if prec == "f32" else _dp))
+    getattr(lib(), f"or_proj_params_{prec}")(
+        _f(ang), n, _f(xt), 0 if xt is None else xt.size, _f(zf),
+        0 if zf is None else zf.size, ptr(0), ptr(1), ptr(2), ptr(3), ptr(4))
+    return tab
+
+
+def backproject(P, ptab, means, out_nx, out_nz, fill="mean", y0=0, y1=None,
+                prec="f32", nthreads=8):
+    """Plain BP (no local alignment): out[(iy-y0)][oz][ox], times 1/nz."""
+    P = _f32c(P)
+    nz, ny, nx = P.shape
+    ptab = np.ascontiguousarray(ptab, np.float32)
+    nviews = ptab.shape[1]
+    y1 = ny if y1 is None else y1
+    out = np.zeros((y1 - y0, out_nz, out_nx), np.float32)
+    getattr(lib(), f"or_backproject_simple_{prec}")(
+        nx, ny, nz, nviews, out_nx, out_nz, FILL[fill] if isinstance(fill, str) else fill,
+        _f(ptab), _f(_f32c(means)), _f(P), y0, y1, _f(out), nthreads)
+    return out
+
+
+def forward_project(V, ptab, in_nx, nz, prec="f32"):
+    V = _f32c(V)
+    ny, out_nz, out_nx = V.shape
+    ptab = np.ascontiguousarray(ptab, np.float32)
+    nviews = ptab.shape[1]
+    R = np.zeros((nz, ny, in_nx), np.float32)
+    getattr(lib(), f"or_forward_project_simple_{prec}")(
+        in_nx, ny, nz, nviews, out_nx, out_nz, _f(ptab), _f(V), _f(R))
+    return R
+
+
+def local_interp(grids, view, x, y, prec="f32"):
+    out = np.zeros(2, np.float32)
+    xs = _f32c(grids.xs[view]) if len(grids.xs[view]) else np.zeros(1, np.float32)
+    ys = _f32c(grids.ys[view]) if len(grids.ys[view]) else np.zeros(1, np.float32)
+    gdx = _f32c(grids.dx[view]) if len(grids.dx[view]) else np.zeros(1, np.float32)
+    gdy = _f32c(grids.dy[view]) if len(grids.dy[view]) else np.zeros(1, np.float32)
+    getattr(lib(), f"or_local_interp_float_{prec}")(
+        _f(xs), len(grids.xs[view]), _f(ys), len(grids.ys[view]), _f(gdx), _f(gdy),
+        x, y, _f(out))
+    return float(out[0]), float(out[1])
+
+
+@dataclass
+class TiltOptions:
+    """The `tilt` option surface (apps/tilt/src/main.rs:18-120) with its defaults."""
+    thickness: int = 0
+    width: int | None = None
+    radial: tuple = (0.35, 0.05)
+    no_weight: bool = False
+    sirt: int = 0
+    sirt_relax: float = 1.0
+    no_cosine_stretch: bool = False
+    log: bool = False
+    densweight: float = 1.0
+    outscale: float = 1.0
+    outadd: float = 0.0
+    exact_filter_size: int = 0
+    fill_mode: str = "mean"
+    cosine_interp: str = "linear"
+    max_cosine_tilt: float = 80.0
+    hamming: bool = False
+    fake_sirt: int = 0
+
+
+def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
+             prec="f32", nthreads=8):
+    """The whole reconstruction (main.rs:601-1093 after reading).  P: [nz][ny][nx]
+    raw projections.  Returns (volume [ny][out_nz][out_nx] float32, (amin, amax, amean))."""
+    P = np.array(P, dtype=np.float32, copy=True)
+    nz, ny, nx = P.shape
+    ang = _f32c(angles)
+    out_nx = opts.width if opts.width else nx
+    out_nz = opts.thickness if opts.thickness > 0 else nx
+    if not opts.no_cosine_stretch and ang.size < nz:
+        raise IndexError("fewer tilt angles than sections (main.rs:450 would panic)")
+    o = OrOpts(nx, ny, nz, out_nx, out_nz, opts.radial[0], opts.radial[1],
+               int(opts.no_weight), int(opts.exact_filter_size or 0), int(opts.hamming),
+               int(opts.no_cosine_stretch), int(opts.cosine_interp.lower() == "cubic"),
+               opts.max_cosine_tilt, int(opts.log), FILL.get(opts.fill_mode.lower(), 0),
+               opts.densweight, opts.outscale, opts.outadd, int(opts.sirt),
+               opts.sirt_relax, int(opts.fake_sirt), nthreads)
+    vol = np.zeros((ny, out_nz, out_nx), np.float32)
+    stats = np.zeros(3, np.float32)
+    xt = _f32c(xtilt)
+    zf = _f32c(zfactor)
+    if local is not None:
+        lnx, lny, lxs, lys, ldx, ldy = local.flat()
+        lp = (lnx.ctypes.data_as(_ip), lny.ctypes.data_as(_ip), _f(lxs), _f(lys),
+              _f(ldx), _f(ldy))
+    else:
+        lp = (None,) * 6
+    getattr(lib(), f"or_tilt_run_{prec}")(
+        C.byref(o), _f(P), _f(ang), ang.size, _f(xt), 0 if xt is None else xt.size,
+        _f(zf), 0 if zf is None else zf.size, *lp, _f(vol), _f(stats))
+    return vol, (float(stats[0]), float(stats[1]), float(stats[2]))

@@ -1,0 +1,767 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the oracle.  Nothing in the product library,
+ * the drop-in host or the timed GPU path may include, link or call this code.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it,
+ * and only as the checker / the CPU baseline.
+ *
+ * A CPU restatement of the `tilt` reconstruction core of imod-rs
+ * (apps/tilt/src/main.rs).  This header is included twice by tilt_oracle.c:
+ *   REAL=float,  SUF=f32  -> the "faithful" FP32 form: the reference's operation
+ *                            order, compiled with -ffp-contract=off so a*b+c is
+ *                            never fused (Rust never contracts either);
+ *   REAL=double, SUF=f64  -> the FP64 oracle: the same algorithm in double, with
+ *                            geometry and filter tables also in double.
+ *
+ * The FFT inside the R-weighting is not rustfft (a third-party crate absent from
+ * the reference tree, apps/tilt/Cargo.toml:17 "rustfft = 6", no lockfile): it is
+ * a textbook mixed-radix decimation-in-time Cooley-Tukey DFT whose twiddles are
+ * computed in double.  Its exact FP32 rounding is therefore "parity unpinned"
+ * against rustfft; it is pinned against numpy.fft and closed forms in tests/.
+ */
+
+#define CAT_(a, b) a##_##b
+#define CAT(a, b) CAT_(a, b)
+#define F(name) CAT(name, SUF)
+
+typedef struct {
+    REAL re, im;
+} F(cpx);
+
+/* ------------------------------------------------------------------ FFT -- */
+
+/* Smallest prime factor of n (n >= 2). */
+static int F(spf)(int n) {
+    if (n % 2 == 0) return 2;
+    for (int p = 3; (long)p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/*
+ * Recursive DIT DFT: out[k] = sum_j in[j*stride] * exp(sign*2*pi*i*j*k/n).
+ * tw[t] = exp(sign*2*pi*i*t/N) for the top-level length N; level length n
+ * divides N, so w_n^e = tw[(e*(N/n)) % N].
+ */
+static void F(dft_rec)(const F(cpx) * in, int stride, int n, F(cpx) * out,
+                       F(cpx) * scratch, const F(cpx) * tw, int N) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int p = F(spf)(n);
+    int m = n / p;
+    /* sub-DFTs of the p decimated sequences, stored contiguously in out */
+    for (int r = 0; r < p; r++)
+        F(dft_rec)(in + (long)r * stride, stride * p, m, out + (long)r * m,
+                   scratch, tw, N);
+    /* butterflies: X[k + q*m] = sum_r Y_r[k] * w_n^{r*(k+q*m)} */
+    long step = N / n;
+    for (int k = 0; k < m; k++) {
+        for (int q = 0; q < p; q++) {
+            int kk = k + q * m;
+            REAL sr = 0, si = 0;
+            for (int r = 0; r < p; r++) {
+                long e = ((long)r * kk) % n;
+                F(cpx) w = tw[(e * step) % N];
+                F(cpx) y = out[(long)r * m + k];
+                sr += y.re * w.re - y.im * w.im;
+                si += y.re * w.im + y.im * w.re;
+            }
+            scratch[kk].re = sr;
+            scratch[kk].im = si;
+        }
+    }
+    for (int k = 0; k < n; k++) out[k] = scratch[k];
+}
+
+/* Unnormalised DFT of length n with sign -1 (forward) or +1 (inverse). */
+static void F(dft)(F(cpx) * data, int n, int sign, F(cpx) * work3n) {
+    F(cpx)* tw = work3n;           /* n */
+    F(cpx)* res = work3n + n;      /* n */
+    F(cpx)* scr = work3n + 2 * n;  /* n */
+    for (int t = 0; t < n; t++) {
+        double a = sign * 2.0 * M_PI * (double)t / (double)n;
+        tw[t].re = (REAL)cos(a);
+        tw[t].im = (REAL)sin(a);
+    }
+    F(dft_rec)(data, 1, n, res, scr, tw, n);
+    for (int t = 0; t < n; t++) data[t] = res[t];
+}
+
+/* ------------------------------------------------------ radial weighting -- */
+
+/*
+ * Radial filter weight of one frequency bin.  Restates apps/tilt/src/main.rs:349-369:
+ * freq_bin = i <= n/2 ? i : n-i;  w = freq_bin / filter_nxc;
+ * Gaussian roll-off (default) or Hamming window.
+ */
+static REAL F(radial_weight)(int i, int n, int filter_nxc, REAL cutoff, REAL falloff,
+                             int hamming) {
+    int freq_bin = (i <= n / 2) ? i : n - i;
+    REAL w = (REAL)freq_bin / (REAL)filter_nxc;
+    if (hamming) {
+        REAL h;
+        if (cutoff > (REAL)1e-12 && w <= cutoff)
+#if REAL_IS_DOUBLE
+            h = 0.54 + 0.46 * cos(M_PI * w / cutoff);
+#else
+            h = 0.54f + 0.46f * cosf((float)M_PI * w / cutoff);
+#endif
+        else if (w > cutoff)
+            h = 0;
+        else
+            h = 1;
+        return w * h;
+    }
+    if (w <= cutoff) return w;
+    REAL d = w - cutoff;
+#if REAL_IS_DOUBLE
+    return w * exp(-d * d / (2.0 * falloff * falloff));
+#else
+    return w * expf(-d * d / (2.0f * falloff * falloff));
+#endif
+}
+
+void F(or_radial_weights)(int n, int exact_filter_size, float cutoff, float falloff,
+                          int hamming, REAL* out) {
+    int nxc = exact_filter_size > 0 ? exact_filter_size : n / 2 + 1;
+    for (int i = 0; i < n; i++)
+        out[i] = F(radial_weight)(i, n, nxc, (REAL)cutoff, (REAL)falloff, hamming);
+}
+
+typedef struct {
+    REAL* data;
+    long rows;
+    int n;
+    const REAL* wt;
+} F(rw_ctx);
+
+static void F(rw_rows)(void* vctx, long r0, long r1) {
+    F(rw_ctx)* c = (F(rw_ctx)*)vctx;
+    int n = c->n;
+    F(cpx)* buf = (F(cpx)*)malloc(sizeof(F(cpx)) * 4 * (size_t)n);
+    F(cpx)* work = buf + n;
+    for (long r = r0; r < r1; r++) {
+        REAL* row = c->data + r * (long)n;
+        /* main.rs:344 -- real row into a complex buffer */
+        for (int i = 0; i < n; i++) {
+            buf[i].re = row[i];
+            buf[i].im = 0;
+        }
+        F(dft)(buf, n, -1, work); /* main.rs:346 forward */
+        for (int i = 0; i < n; i++) { /* main.rs:370-371 */
+            buf[i].re *= c->wt[i];
+            buf[i].im *= c->wt[i];
+        }
+        F(dft)(buf, n, +1, work); /* main.rs:374 inverse (unnormalised) */
+        REAL inv_n = (REAL)1 / (REAL)n; /* main.rs:376-379 */
+        for (int i = 0; i < n; i++) row[i] = buf[i].re * inv_n;
+    }
+    free(buf);
+}
+
+/* apply_rweighting, apps/tilt/src/main.rs:333-381.  data = rows x n, in place. */
+void F(or_rweight)(REAL* data, long rows, int n, float cutoff, float falloff,
+                   int exact_filter_size, int hamming, int nthreads) {
+    REAL* wt = (REAL*)malloc(sizeof(REAL) * (size_t)n);
+    F(or_radial_weights)(n, exact_filter_size, cutoff, falloff, hamming, wt);
+    F(rw_ctx) c = {data, rows, n, wt};
+    or_parallel_for(rows, 1, nthreads, F(rw_rows), &c);
+    free(wt);
+}
+
+/* -------------------------------------------------------- cosine stretch -- */
+
+/* cubic_interp_1d, main.rs:411-420 (Catmull-Rom) */
+static REAL F(cubic_interp_1d)(REAL p0, REAL p1, REAL p2, REAL p3, REAL t) {
+    REAL t2 = t * t;
+    REAL t3 = t2 * t;
+    REAL a = (REAL)-0.5 * p0 + (REAL)1.5 * p1 - (REAL)1.5 * p2 + (REAL)0.5 * p3;
+    REAL b = p0 - (REAL)2.5 * p1 + (REAL)2.0 * p2 - (REAL)0.5 * p3;
+    REAL c = (REAL)-0.5 * p0 + (REAL)0.5 * p2;
+    REAL d = p1;
+    return a * t3 + b * t2 + c * t + d;
+}
+
+/* sample_row_cubic, main.rs:425-441 */
+static REAL F(sample_row_cubic)(const REAL* row, REAL src_x, int len) {
+    long sx0 = (long)floor(src_x);
+    REAL frac = src_x - (REAL)sx0;
+    if (sx0 >= 1 && sx0 + 2 < len) {
+        long i = sx0;
+        return F(cubic_interp_1d)(row[i - 1], row[i], row[i + 1], row[i + 2], frac);
+    } else if (sx0 >= 0 && sx0 + 1 < len) {
+        return row[sx0] * ((REAL)1 - frac) + row[sx0 + 1] * frac;
+    } else if (sx0 >= 0 && sx0 < len) {
+        return row[sx0];
+    }
+    return 0;
+}
+
+/* Per-view stretch factor 1/cos(theta) or 0 when the view is not stretched
+ * (main.rs:450-459). */
+static REAL F(stretch_factor)(float deg, float max_tilt) {
+    if (fabsf(deg) > max_tilt) return 0;
+#if REAL_IS_DOUBLE
+    double rad = (double)deg * M_PI / 180.0;
+    double ct = cos(rad);
+    if (fabs(ct) < 1e-6) return 0;
+    return 1.0 / ct;
+#else
+    float rad = deg * (float)M_PI / 180.0f;
+    float ct = cosf(rad);
+    if (fabsf(ct) < 1e-6f) return 0;
+    return 1.0f / ct;
+#endif
+}
+
+/* apply_cosine_stretch, apps/tilt/src/main.rs:446-488.  `angles` must hold at
+ * least nz entries (the reference indexes tilt_angles[p] for every p < nz). */
+void F(or_cosine_stretch)(REAL* data, int in_nx, int in_ny, int nz, const float* angles,
+                          int use_cubic, float max_tilt) {
+    REAL center = (REAL)in_nx / (REAL)2;
+    REAL* orig = (REAL*)malloc(sizeof(REAL) * (size_t)in_nx);
+    for (int p = 0; p < nz; p++) {
+        REAL stretch = F(stretch_factor)(angles[p], max_tilt);
+        if (stretch == 0) continue;
+        for (int iy = 0; iy < in_ny; iy++) {
+            REAL* out_row = data + ((long)p * in_ny + iy) * in_nx;
+            memcpy(orig, out_row, sizeof(REAL) * (size_t)in_nx);
+            for (int ox = 0; ox < in_nx; ox++) {
+                REAL dx = (REAL)ox - center;
+                REAL src_x = dx * stretch + center;
+                if (use_cubic) {
+                    out_row[ox] = F(sample_row_cubic)(orig, src_x, in_nx);
+                } else {
+                    long sx0 = (long)floor(src_x);
+                    if (sx0 >= 0 && sx0 + 1 < in_nx) {
+                        REAL frac = src_x - (REAL)sx0;
+                        out_row[ox] = orig[sx0] * ((REAL)1 - frac) + orig[sx0 + 1] * frac;
+                    } else if (sx0 >= 0 && sx0 < in_nx) {
+                        out_row[ox] = orig[sx0];
+                    } else {
+                        out_row[ox] = 0;
+                    }
+                }
+            }
+        }
+    }
+    free(orig);
+}
+
+/* -------------------------------------------------------------- geometry -- */
+
+/*
+ * ProjParams table, main.rs:683-700, for the first n = min(nz, n_angles) views.
+ * xtilt / zfactor may be NULL (absent file); missing entries default to 0 deg / 1.0
+ * (main.rs:689-691).  Output: 5 arrays of n REALs.
+ */
+void F(or_proj_params)(const float* angles, int n, const float* xtilt, int n_xtilt,
+                       const float* zfac, int n_zfac, REAL* cos_t, REAL* sin_t,
+                       REAL* cos_xt, REAL* sin_xt, REAL* zf) {
+    for (int i = 0; i < n; i++) {
+        float deg = angles[i];
+        float xt_deg = (xtilt && i < n_xtilt) ? xtilt[i] : 0.0f;
+        float z = (zfac && i < n_zfac) ? zfac[i] : 1.0f;
+#if REAL_IS_DOUBLE
+        double rad = (double)deg * M_PI / 180.0;
+        double xrad = (double)xt_deg * M_PI / 180.0;
+        cos_t[i] = cos(rad);
+        sin_t[i] = sin(rad);
+        cos_xt[i] = cos(xrad);
+        sin_xt[i] = sin(xrad);
+#else
+        float rad = deg * (float)M_PI / 180.0f;
+        float xrad = xt_deg * (float)M_PI / 180.0f;
+        cos_t[i] = cosf(rad);
+        sin_t[i] = sinf(rad);
+        cos_xt[i] = cosf(xrad);
+        sin_xt[i] = sinf(xrad);
+#endif
+        zf[i] = (REAL)z;
+    }
+}
+
+/* ------------------------------------------------------- local alignment -- */
+
+/* ViewPatches::find_bracket, main.rs:200-228 */
+static void F(find_bracket)(const float* pos, int len, REAL val, int* idx, REAL* frac) {
+    if (len == 1) {
+        *idx = 0;
+        *frac = 0;
+        return;
+    }
+    int lo = 0, hi = len - 1;
+    if (val <= (REAL)pos[lo]) {
+        *idx = 0;
+        *frac = 0;
+        return;
+    }
+    if (val >= (REAL)pos[hi]) {
+        *idx = hi;
+        *frac = 0;
+        return;
+    }
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if ((REAL)pos[mid] <= val)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    REAL span = (REAL)pos[hi] - (REAL)pos[lo];
+    *idx = lo;
+    *frac = span > (REAL)1e-12 ? (val - (REAL)pos[lo]) / span : 0;
+}
+
+/* ViewPatches::interpolate, main.rs:172-198.  nx, ny = grid size of the view. */
+static void F(local_interp)(const float* xs, int nx, const float* ys, int ny,
+                            const float* gdx, const float* gdy, REAL x, REAL y, REAL* odx,
+                            REAL* ody) {
+    if (nx == 0 || ny == 0) {
+        *odx = 0;
+        *ody = 0;
+        return;
+    }
+    int ix0, iy0;
+    REAL fx, fy;
+    F(find_bracket)(xs, nx, x, &ix0, &fx);
+    F(find_bracket)(ys, ny, y, &iy0, &fy);
+    int ix1 = ix0 + 1 < nx - 1 ? ix0 + 1 : nx - 1;
+    int iy1 = iy0 + 1 < ny - 1 ? iy0 + 1 : ny - 1;
+    REAL d00x = gdx[iy0 * nx + ix0], d00y = gdy[iy0 * nx + ix0];
+    REAL d10x = gdx[iy0 * nx + ix1], d10y = gdy[iy0 * nx + ix1];
+    REAL d01x = gdx[iy1 * nx + ix0], d01y = gdy[iy1 * nx + ix0];
+    REAL d11x = gdx[iy1 * nx + ix1], d11y = gdy[iy1 * nx + ix1];
+#define LERP(a, b, t) ((a) + (t) * ((b) - (a)))
+    *odx = LERP(LERP(d00x, d10x, fx), LERP(d01x, d11x, fx), fy);
+    *ody = LERP(LERP(d00y, d10y, fx), LERP(d01y, d11y, fx), fy);
+#undef LERP
+}
+
+/* ------------------------------------------------------------ projection -- */
+
+/* The full geometry of one reconstruction, shared by BP and FP. */
+typedef struct {
+    int in_nx, in_ny, nz;   /* projection width, height, sections (views in file) */
+    int nviews;             /* views iterated = min(nz, n_angles), main.rs:683-686 */
+    int out_nx, out_nz;     /* output width, thickness */
+    int fill_mode;          /* 0 mean, 1 zero, 2 edge */
+    const REAL *cos_t, *sin_t, *cos_xt, *sin_xt, *zf; /* [nviews] */
+    const REAL* means;      /* [nz] per-view means (fill value for Mean) */
+    /* local alignment: per view v, grid nxs[v] x nys[v] at offsets */
+    int has_local;
+    const int *lnx, *lny;
+    const long *lxoff, *lyoff, *lgoff;
+    const float *lxs, *lys, *ldx, *ldy;
+} F(or_geom);
+
+/* sample_row_with_fill, main.rs:386-406 */
+static inline REAL F(sample_fill)(const REAL* row, REAL proj_x, int row_len, int fill,
+                                  REAL mean) {
+    long px0 = (long)floor(proj_x);
+    if (px0 >= 0 && px0 + 1 < row_len) {
+        REAL frac = proj_x - (REAL)px0;
+        return row[px0] * ((REAL)1 - frac) + row[px0 + 1] * frac;
+    }
+    if (fill == 1) return 0;
+    if (fill == 0) return mean;
+    if (px0 < 0) return row[0];
+    return row_len > 0 ? row[row_len - 1] : 0;
+}
+
+/* Geometry of one voxel-view pair: proj_x, sample_y and the two-row flag.
+ * main.rs:1015-1051 (BP) and :837-876 (FP) compute it identically. */
+static inline void F(voxel_coords)(const F(or_geom) * g, int pi, int iy, int oz, int ox,
+                                   REAL* proj_x, REAL* sample_y, int* two) {
+    REAL center_x = (REAL)g->in_nx / (REAL)2;
+    REAL center_z = (REAL)g->out_nz / (REAL)2;
+    REAL out_center_x = (REAL)g->out_nx / (REAL)2;
+    REAL dz = (REAL)oz - center_z;
+    REAL z_contrib = dz * g->sin_t[pi] * g->zf[pi];
+    REAL dx = (REAL)ox - out_center_x;
+    REAL px = dx * g->cos_t[pi] * g->cos_xt[pi] + z_contrib + center_x;
+    REAL y_off = dx * g->sin_xt[pi];
+    REAL sy = (REAL)iy + y_off;
+    if (g->has_local) {
+        int nx = g->lnx[pi], ny = g->lny[pi];
+        if (nx > 0) { /* "!vp.xs.is_empty()" */
+            REAL ldx, ldy;
+            F(local_interp)(g->lxs + g->lxoff[pi], nx, g->lys + g->lyoff[pi], ny,
+                            g->ldx + g->lgoff[pi], g->ldy + g->lgoff[pi], (REAL)ox,
+                            (REAL)iy, &ldx, &ldy);
+            px += ldx;
+            sy = sy + ldy;
+        }
+    }
+    *proj_x = px;
+    *sample_y = sy;
+    *two = (fabs(y_off) > (REAL)1e-6) || g->has_local;
+}
+
+/* One output Y slice: the hot loop of main.rs:1011-1073 (== backproject_all,
+ * main.rs:759-826).  slice = [out_nz][out_nx], multiplied by inv_n = 1/nz. */
+static void F(bp_slice)(const F(or_geom) * g, const REAL* P, int iy, REAL* slice) {
+    int in_nx = g->in_nx, in_ny = g->in_ny;
+    long nxy = (long)in_nx * in_ny;
+    memset(slice, 0, sizeof(REAL) * (size_t)g->out_nx * g->out_nz);
+    for (int pi = 0; pi < g->nviews; pi++) {
+        REAL pm = g->means[pi];
+        for (int oz = 0; oz < g->out_nz; oz++) {
+            REAL* srow = slice + (long)oz * g->out_nx;
+            for (int ox = 0; ox < g->out_nx; ox++) {
+                REAL px, sy;
+                int two;
+                F(voxel_coords)(g, pi, iy, oz, ox, &px, &sy, &two);
+                long sy0 = (long)floor(sy);
+                REAL fy = sy - (REAL)sy0;
+                if (sy0 >= 0 && sy0 + 1 < in_ny && two) {
+                    const REAL* r0 = P + pi * nxy + sy0 * in_nx;
+                    const REAL* r1 = r0 + in_nx;
+                    REAL v0 = F(sample_fill)(r0, px, in_nx, g->fill_mode, pm);
+                    REAL v1 = F(sample_fill)(r1, px, in_nx, g->fill_mode, pm);
+                    srow[ox] += v0 * ((REAL)1 - fy) + v1 * fy;
+                } else if (sy0 >= 0 && sy0 < in_ny) {
+                    const REAL* r0 = P + pi * nxy + sy0 * in_nx;
+                    srow[ox] += F(sample_fill)(r0, px, in_nx, g->fill_mode, pm);
+                }
+            }
+        }
+    }
+    REAL inv_n = (REAL)1 / (REAL)g->nz; /* main.rs:705 -- 1/nz even with fewer angles */
+    long nv = (long)g->out_nx * g->out_nz;
+    for (long i = 0; i < nv; i++) slice[i] *= inv_n;
+}
+
+typedef struct {
+    const F(or_geom) * g;
+    const REAL* P;
+    int y0;
+    REAL* out;
+} F(bp_ctx);
+
+static void F(bp_rows)(void* vctx, long r0, long r1) {
+    F(bp_ctx)* c = (F(bp_ctx)*)vctx;
+    long nv = (long)c->g->out_nx * c->g->out_nz;
+    for (long r = r0; r < r1; r++) F(bp_slice)(c->g, c->P, c->y0 + (int)r, c->out + r * nv);
+}
+
+/* Backproject Y rows [y0, y1) into out[(iy-y0)][oz][ox]; rows are processed in
+ * chunks of max(32, out_ny/20) and each chunk is split across threads, as in
+ * main.rs:1004-1010 (rayon par_iter over the rows of a chunk). */
+void F(or_backproject)(const F(or_geom) * g, const REAL* P, int y0, int y1, REAL* out,
+                       int nthreads) {
+    int out_ny = g->in_ny;
+    int chunk = out_ny / 20 > 32 ? out_ny / 20 : 32;
+    long nv = (long)g->out_nx * g->out_nz;
+    for (int c0 = y0; c0 < y1; c0 += chunk) {
+        int c1 = c0 + chunk < y1 ? c0 + chunk : y1;
+        F(bp_ctx) c = {g, P, c0, out + (long)(c0 - y0) * nv};
+        or_parallel_for(c1 - c0, 1, nthreads, F(bp_rows), &c);
+    }
+}
+
+/* forward_project, main.rs:831-896: the adjoint scatter of the BP interpolation,
+ * serial, loop order iy -> view -> oz -> ox.  V = [in_ny][out_nz][out_nx],
+ * R = [nz][in_ny][in_nx] (zeroed here). */
+void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R) {
+    int in_nx = g->in_nx, in_ny = g->in_ny;
+    long nxy = (long)in_nx * in_ny;
+    memset(R, 0, sizeof(REAL) * (size_t)g->nz * nxy);
+    for (int iy = 0; iy < in_ny; iy++) {
+        const REAL* slice = V + (long)iy * g->out_nz * g->out_nx;
+        for (int pi = 0; pi < g->nviews; pi++) {
+            for (int oz = 0; oz < g->out_nz; oz++) {
+                for (int ox = 0; ox < g->out_nx; ox++) {
+                    REAL px, sy;
+                    int two;
+                    F(voxel_coords)(g, pi, iy, oz, ox, &px, &sy, &two);
+                    long px0 = (long)floor(px);
+                    if (!(px0 >= 0 && px0 + 1 < in_nx)) continue;
+                    REAL frac = px - (REAL)px0;
+                    REAL val = slice[(long)oz * g->out_nx + ox];
+                    long sy0 = (long)floor(sy);
+                    REAL fy = sy - (REAL)sy0;
+                    if (sy0 >= 0 && sy0 + 1 < in_ny && two) {
+                        REAL* r0 = R + pi * nxy + sy0 * in_nx;
+                        REAL* r1 = r0 + in_nx;
+                        REAL w0 = (REAL)1 - fy, w1 = fy;
+                        r0[px0] += val * ((REAL)1 - frac) * w0;
+                        r0[px0 + 1] += val * frac * w0;
+                        r1[px0] += val * ((REAL)1 - frac) * w1;
+                        r1[px0 + 1] += val * frac * w1;
+                    } else if (sy0 >= 0 && sy0 < in_ny) {
+                        REAL* r0 = R + pi * nxy + sy0 * in_nx;
+                        r0[px0] += val * ((REAL)1 - frac);
+                        r0[px0 + 1] += val * frac;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* lowpass_3d_volume, main.rs:493-538: one separable 3x3x3 box pass (X, Z, Y)
+ * with clamped edges.  vol = [ny][nz][nx], in place. */
+void F(or_lowpass_3d)(REAL* vol, int ny, int nz, int nx) {
+    if (ny == 0) return;
+    long ns = (long)nz * nx;
+    REAL* tmp = (REAL*)malloc(sizeof(REAL) * (size_t)ns);
+    for (int iy = 0; iy < ny; iy++) {
+        REAL* s = vol + iy * ns;
+        for (int oz = 0; oz < nz; oz++)
+            for (int ox = 0; ox < nx; ox++) {
+                int x0 = ox > 0 ? ox - 1 : 0, x2 = ox + 1 < nx ? ox + 1 : nx - 1;
+                long o = (long)oz * nx;
+                tmp[o + ox] = (s[o + x0] + s[o + ox] + s[o + x2]) / (REAL)3;
+            }
+        memcpy(s, tmp, sizeof(REAL) * (size_t)ns);
+    }
+    for (int iy = 0; iy < ny; iy++) {
+        REAL* s = vol + iy * ns;
+        for (int oz = 0; oz < nz; oz++) {
+            int z0 = oz > 0 ? oz - 1 : 0, z2 = oz + 1 < nz ? oz + 1 : nz - 1;
+            for (int ox = 0; ox < nx; ox++)
+                tmp[(long)oz * nx + ox] =
+                    (s[(long)z0 * nx + ox] + s[(long)oz * nx + ox] + s[(long)z2 * nx + ox]) /
+                    (REAL)3;
+        }
+        memcpy(s, tmp, sizeof(REAL) * (size_t)ns);
+    }
+    free(tmp);
+    REAL* prev = (REAL*)malloc(sizeof(REAL) * (size_t)ny * ns);
+    memcpy(prev, vol, sizeof(REAL) * (size_t)ny * ns);
+    for (int iy = 0; iy < ny; iy++) {
+        int y0 = iy > 0 ? iy - 1 : 0, y2 = iy + 1 < ny ? iy + 1 : ny - 1;
+        for (long i = 0; i < ns; i++)
+            vol[iy * ns + i] = (prev[y0 * ns + i] + prev[iy * ns + i] + prev[y2 * ns + i]) / (REAL)3;
+    }
+    free(prev);
+}
+
+/* ------------------------------------------------------------- pipeline -- */
+
+/*
+ * The whole reconstruction of main.rs:601-1093 after file reading:
+ *   log (:607-613), per-view means (:615-623), SIRT original copy (:625-630),
+ *   R-weighting (:638-646), cosine stretch (:648-653), ProjParams (:682-700),
+ *   then WBP (:1002-1090), SIRT (:919-975) or fake-SIRT (:976-1001),
+ *   output transform and statistics (:1077-1093).
+ * P is [nz][in_ny][in_nx] float (raw, modified in place).  vol receives
+ * [in_ny][out_nz][out_nx] floats after the output transform.  stats[3] =
+ * header amin, amax, amean.
+ */
+
+void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_angles,
+                    const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
+                    const int* lnx, const int* lny, const float* lxs, const float* lys,
+                    const float* ldx, const float* ldy, float* vol, float* stats) {
+    long nxy = (long)o->in_nx * o->in_ny;
+    long total = nxy * o->nz;
+    if (o->log_)
+        for (long i = 0; i < total; i++) {
+            float v = P_in[i] > 0.001f ? P_in[i] : 0.001f;
+            P_in[i] = log10f(v);
+        }
+    REAL* means = (REAL*)malloc(sizeof(REAL) * (size_t)(o->nz > 0 ? o->nz : 1));
+    for (int p = 0; p < o->nz; p++) {
+        float mmm[3];
+        or_min_max_mean(P_in + p * nxy, nxy, mmm);
+#if REAL_IS_DOUBLE
+        double s = 0;
+        for (long i = 0; i < nxy; i++) s += P_in[p * nxy + i];
+        means[p] = nxy ? s / (double)nxy : 0;
+#else
+        means[p] = mmm[2];
+#endif
+    }
+    REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)total);
+    for (long i = 0; i < total; i++) P[i] = (REAL)P_in[i];
+    REAL* O = NULL;
+    if (o->sirt > 0) {
+        O = (REAL*)malloc(sizeof(REAL) * (size_t)total);
+        memcpy(O, P, sizeof(REAL) * (size_t)total);
+    }
+    if (!o->no_weight)
+        F(or_rweight)(P, (long)o->nz * o->in_ny, o->in_nx, o->cutoff, o->falloff,
+                      o->exact_filter_size, o->hamming, o->nthreads);
+    if (!o->no_cosine_stretch)
+        F(or_cosine_stretch)(P, o->in_nx, o->in_ny, o->nz, angles, o->cosine_cubic,
+                             o->max_cosine_tilt);
+    int nviews = n_angles < o->nz ? n_angles : o->nz;
+    REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews > 0 ? nviews : 1));
+    F(or_proj_params)(angles, nviews, xtilt, n_xtilt, zfac, n_zfac, tab, tab + nviews,
+                      tab + 2 * nviews, tab + 3 * nviews, tab + 4 * nviews);
+    F(or_geom) g;
+    memset(&g, 0, sizeof(g));
+    g.in_nx = o->in_nx;
+    g.in_ny = o->in_ny;
+    g.nz = o->nz;
+    g.nviews = nviews;
+    g.out_nx = o->out_nx;
+    g.out_nz = o->out_nz;
+    g.fill_mode = o->fill_mode;
+    g.cos_t = tab;
+    g.sin_t = tab + nviews;
+    g.cos_xt = tab + 2 * nviews;
+    g.sin_xt = tab + 3 * nviews;
+    g.zf = tab + 4 * nviews;
+    g.means = means;
+    long *lxoff = NULL, *lyoff = NULL, *lgoff = NULL;
+    if (lnx) {
+        lxoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
+        lyoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
+        lgoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
+        for (int v = 0; v < o->nz; v++) {
+            lxoff[v + 1] = lxoff[v] + lnx[v];
+            lyoff[v + 1] = lyoff[v] + lny[v];
+            lgoff[v + 1] = lgoff[v] + (long)lnx[v] * lny[v];
+        }
+        g.has_local = 1;
+        g.lnx = lnx;
+        g.lny = lny;
+        g.lxoff = lxoff;
+        g.lyoff = lyoff;
+        g.lgoff = lgoff;
+        g.lxs = lxs;
+        g.lys = lys;
+        g.ldx = ldx;
+        g.ldy = ldy;
+    }
+    int out_ny = o->in_ny;
+    long nv = (long)o->out_nx * o->out_nz;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(out_ny * nv > 0 ? out_ny * nv : 1));
+    F(or_backproject)(&g, P, 0, out_ny, V, o->nthreads);
+    if (o->sirt > 0) {
+        REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)total);
+        REAL* D = (REAL*)malloc(sizeof(REAL) * (size_t)(out_ny * nv > 0 ? out_ny * nv : 1));
+        for (int it = 0; it < o->sirt; it++) {
+            F(or_forward_project)(&g, V, R);
+            for (long i = 0; i < total; i++) R[i] = O[i] - R[i];
+            if (!o->no_weight)
+                F(or_rweight)(R, (long)o->nz * o->in_ny, o->in_nx, o->cutoff, o->falloff,
+                              o->exact_filter_size, o->hamming, o->nthreads);
+            F(or_backproject)(&g, R, 0, out_ny, D, o->nthreads);
+            REAL relax = (REAL)o->sirt_relax;
+            for (long i = 0; i < out_ny * nv; i++) V[i] += relax * D[i];
+        }
+        free(R);
+        free(D);
+    }
+    for (int k = 0; k < o->fake_sirt; k++) F(or_lowpass_3d)(V, out_ny, o->out_nz, o->out_nx);
+    /* output transform + statistics, main.rs:1077-1093 */
+    float gmin = FLT_MAX, gmax = -FLT_MAX;
+    double gsum = 0;
+    for (int iy = 0; iy < out_ny; iy++) {
+        float* dst = vol + (long)iy * nv;
+        for (long i = 0; i < nv; i++) {
+            REAL v = (V[iy * nv + i] * (REAL)o->densweight) * (REAL)o->outscale + (REAL)o->outadd;
+            dst[i] = (float)v;
+        }
+        float mmm[3];
+        or_min_max_mean(dst, nv, mmm);
+        if (mmm[0] < gmin) gmin = mmm[0];
+        if (mmm[1] > gmax) gmax = mmm[1];
+        gsum += (double)mmm[2] * (double)nv;
+    }
+    stats[0] = gmin;
+    stats[1] = gmax;
+    stats[2] = (float)(gsum / (double)(nv * out_ny));
+    free(V);
+    free(tab);
+    free(means);
+    free(P);
+    free(O);
+    free(lxoff);
+    free(lyoff);
+    free(lgoff);
+}
+
+/* Thin wrappers so tests can drive BP / FP on caller-given float arrays. */
+void F(or_backproject_simple)(int in_nx, int in_ny, int nz, int nviews, int out_nx,
+                              int out_nz, int fill_mode, const float* ptab /* 5*nviews */,
+                              const float* means, const float* P_in, int y0, int y1,
+                              float* out, int nthreads) {
+    long total = (long)in_nx * in_ny * nz;
+    REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)total);
+    for (long i = 0; i < total; i++) P[i] = (REAL)P_in[i];
+    REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews ? nviews : 1));
+    for (int i = 0; i < 5 * nviews; i++) tab[i] = (REAL)ptab[i];
+    REAL* m = (REAL*)malloc(sizeof(REAL) * (size_t)(nz ? nz : 1));
+    for (int i = 0; i < nz; i++) m[i] = (REAL)means[i];
+    F(or_geom) g;
+    memset(&g, 0, sizeof(g));
+    g.in_nx = in_nx; g.in_ny = in_ny; g.nz = nz; g.nviews = nviews;
+    g.out_nx = out_nx; g.out_nz = out_nz; g.fill_mode = fill_mode;
+    g.cos_t = tab; g.sin_t = tab + nviews; g.cos_xt = tab + 2 * nviews;
+    g.sin_xt = tab + 3 * nviews; g.zf = tab + 4 * nviews; g.means = m;
+    long nv = (long)out_nx * out_nz;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)((y1 - y0) * nv > 0 ? (y1 - y0) * nv : 1));
+    F(or_backproject)(&g, P, y0, y1, V, nthreads);
+    for (long i = 0; i < (y1 - y0) * nv; i++) out[i] = (float)V[i];
+    free(V); free(m); free(tab); free(P);
+}
+
+void F(or_forward_project_simple)(int in_nx, int in_ny, int nz, int nviews, int out_nx,
+                                  int out_nz, const float* ptab, const float* V_in,
+                                  float* R_out) {
+    long nvol = (long)in_ny * out_nx * out_nz;
+    long total = (long)in_nx * in_ny * nz;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(nvol ? nvol : 1));
+    for (long i = 0; i < nvol; i++) V[i] = (REAL)V_in[i];
+    REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)(total ? total : 1));
+    REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews ? nviews : 1));
+    for (int i = 0; i < 5 * nviews; i++) tab[i] = (REAL)ptab[i];
+    REAL zero = 0;
+    F(or_geom) g;
+    memset(&g, 0, sizeof(g));
+    g.in_nx = in_nx; g.in_ny = in_ny; g.nz = nz; g.nviews = nviews;
+    g.out_nx = out_nx; g.out_nz = out_nz; g.fill_mode = 1;
+    g.cos_t = tab; g.sin_t = tab + nviews; g.cos_xt = tab + 2 * nviews;
+    g.sin_xt = tab + 3 * nviews; g.zf = tab + 4 * nviews; g.means = &zero;
+    F(or_forward_project)(&g, V, R);
+    for (long i = 0; i < total; i++) R_out[i] = (float)R[i];
+    free(tab); free(R); free(V);
+}
+
+/* R-weighting / stretch on float arrays for tests (computed in REAL). */
+void F(or_rweight_float)(float* data, long rows, int n, float cutoff, float falloff,
+                         int exact_filter_size, int hamming, int nthreads) {
+    long total = rows * (long)n;
+    REAL* d = (REAL*)malloc(sizeof(REAL) * (size_t)(total ? total : 1));
+    for (long i = 0; i < total; i++) d[i] = (REAL)data[i];
+    F(or_rweight)(d, rows, n, cutoff, falloff, exact_filter_size, hamming, nthreads);
+    for (long i = 0; i < total; i++) data[i] = (float)d[i];
+    free(d);
+}
+
+void F(or_cosine_stretch_float)(float* data, int in_nx, int in_ny, int nz,
+                                const float* angles, int use_cubic, float max_tilt) {
+    long total = (long)in_nx * in_ny * nz;
+    REAL* d = (REAL*)malloc(sizeof(REAL) * (size_t)(total ? total : 1));
+    for (long i = 0; i < total; i++) d[i] = (REAL)data[i];
+    F(or_cosine_stretch)(d, in_nx, in_ny, nz, angles, use_cubic, max_tilt);
+    for (long i = 0; i < total; i++) data[i] = (float)d[i];
+    free(d);
+}
+
+void F(or_dft_float)(float* re, float* im, int n, int sign) {
+    F(cpx)* b = (F(cpx)*)malloc(sizeof(F(cpx)) * 4 * (size_t)n);
+    for (int i = 0; i < n; i++) { b[i].re = (REAL)re[i]; b[i].im = (REAL)im[i]; }
+    F(dft)(b, n, sign, b + n);
+    for (int i = 0; i < n; i++) { re[i] = (float)b[i].re; im[i] = (float)b[i].im; }
+    free(b);
+}
+
+void F(or_local_interp_float)(const float* xs, int nx, const float* ys, int ny,
+                              const float* gdx, const float* gdy, float x, float y,
+                              float* out2) {
+    REAL a, b;
+    F(local_interp)(xs, nx, ys, ny, gdx, gdy, (REAL)x, (REAL)y, &a, &b);
+    out2[0] = (float)a;
+    out2[1] = (float)b;
+}
+
+#undef F
+#undef CAT
+#undef CAT_

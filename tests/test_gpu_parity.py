@@ -1,0 +1,217 @@
+"""GPU parity: libtiltgpu (through its C ABI) against the FP32 oracle, which
+restates apps/tilt/src/main.rs in the reference's operation order.
+
+Bar (north star): max |gpu - oracle| <= 1e-4 * RMS(oracle volume) for every FP32
+result.  FP16/BF16 storage is compared with the FP64 oracle at the tolerance
+stated in its test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def case(tg, nx=128, ny=20, nviews=31, thick=48, seed=0, span=60.0, xtilt=None,
+         shifts=None, n_ell=30, n_beads=5):
+    angles = np.linspace(-span, span, nviews).astype(np.float32)
+    sx = sy = None
+    if shifts is not None:
+        sx, sy = shifts
+    P = tg.synth_series(nx, ny, angles, vol_nz=thick, n_ellipsoids=n_ell, n_beads=n_beads,
+                        seed=seed, xtilt=xtilt, shift_x=sx, shift_y=sy,
+                        axis=(2.0, 10.0), bead_radius=2.0)
+    return P, angles
+
+
+def both(tg, P, angles, gopts, xtilt=None, zfactor=None, local=None, prec="f32"):
+    print(f"case: shape={P.shape} opts={gopts}", flush=True)
+    vol_g, st_g = tg.reconstruct(P, angles, gopts, xtilt=xtilt, zfactor=zfactor, local=local)
+    oopts = O.TiltOptions(**{k: v for k, v in vars(gopts).items() if k != "storage"})
+    if oopts.exact_filter_size is None:
+        oopts.exact_filter_size = 0
+    vol_o, st_o = O.tilt_run(P, angles, oopts, xtilt=xtilt, zfactor=zfactor, local=local,
+                             prec=prec)
+    return vol_g, st_g, vol_o, st_o
+
+
+def test_wbp_defaults(gpu, rel):
+    P, ang = case(gpu)
+    g, sg, o, so = both(gpu, P, ang, gpu.TiltOptions(thickness=48))
+    assert g.shape == o.shape == (20, 48, 128)
+    assert rel(g, o) <= TOL
+    assert np.allclose(sg, so, rtol=1e-4, atol=1e-4 * np.sqrt(np.mean(o.astype(np.float64) ** 2)))
+
+
+@pytest.mark.parametrize("opts", [
+    dict(no_weight=True, no_cosine_stretch=True),
+    dict(no_weight=True),
+    dict(no_cosine_stretch=True),
+    dict(fill_mode="zero"),
+    dict(fill_mode="edge"),
+    dict(hamming=True, radial=(0.3, 0.05)),
+    dict(exact_filter_size=40, radial=(0.4, 0.1)),
+    dict(cosine_interp="cubic"),
+    dict(max_cosine_tilt=30.0),
+    dict(densweight=2.0, outscale=0.5, outadd=3.0),
+    dict(width=96),
+    dict(width=200, thickness=100),
+    dict(log=True),
+])
+def test_wbp_options(gpu, rel, opts):
+    P, ang = case(gpu, seed=1)
+    if opts.get("log"):
+        P = np.abs(P) + 0.5
+    kw = dict(thickness=48)
+    kw.update(opts)
+    g, sg, o, so = both(gpu, P, ang, gpu.TiltOptions(**kw))
+    assert g.shape == o.shape
+    assert rel(g, o) <= TOL, opts
+
+
+@pytest.mark.parametrize("nx", [96, 120, 160, 250, 768])
+def test_wbp_mixed_radix_widths(gpu, rel, nx):
+    P, ang = case(gpu, nx=nx, ny=6, nviews=21, thick=32, seed=nx)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=32))
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_odd_rows_and_fewer_angles(gpu, rel):
+    # in_ny odd: FFT row pairs straddle views; 25 angles for 27 sections
+    P, ang = case(gpu, ny=7, nviews=27, thick=40, seed=3)
+    g, _, o, _ = both(gpu, P, ang[:25], gpu.TiltOptions(thickness=40, no_cosine_stretch=True))
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_xtilt(gpu, rel):
+    P, ang = case(gpu, seed=4, xtilt=np.full(31, 0.5, np.float32))
+    xt = np.full(31, 0.5, np.float32)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=48), xtilt=xt)
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_xtilt_large_and_zfactor(gpu, rel):
+    P, ang = case(gpu, seed=5, ny=40)
+    xt = np.linspace(-3, 3, 31).astype(np.float32)
+    zf = np.linspace(0.9, 1.1, 31).astype(np.float32)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=48), xtilt=xt, zfactor=zf)
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_local_constant_shifts(gpu, rel):
+    rng = np.random.default_rng(6)
+    sx = rng.normal(0, 2, 31).astype(np.float32)
+    sy = rng.normal(0, 2, 31).astype(np.float32)
+    P, ang = case(gpu, seed=6, shifts=(sx, sy), xtilt=np.full(31, 0.5, np.float32))
+    patches = [(v, 64.0, 10.0, float(sx[v]), float(sy[v])) for v in range(31)]
+    grids = O.build_local_grids(patches, 31)
+    xt = np.full(31, 0.5, np.float32)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=48), xtilt=xt, local=grids)
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_local_grid(gpu, rel):
+    rng = np.random.default_rng(7)
+    P, ang = case(gpu, seed=7, ny=32)
+    patches = []
+    for v in range(31):
+        if v == 5:
+            continue  # a view without patches: no shift for it
+        for x in (20.0, 64.0, 110.0):
+            for y in (4.0, 16.0, 28.0):
+                patches.append((v, x, y, float(rng.normal(0, 1.5)), float(rng.normal(0, 1.5))))
+    grids = O.build_local_grids(patches, 31)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=48), local=grids)
+    assert rel(g, o) <= TOL
+
+
+def test_filter_stage_and_means(gpu, rel):
+    P, ang = case(gpu, seed=8, ny=9)
+    opts = gpu.TiltOptions(thickness=48)
+    vol, hs, s = gpu.reconstruct(P, ang, opts, return_session=True)
+    filt = s.read_filtered()
+    means = s.read_means()
+    s.close()
+    ref_means = np.array([O.min_max_mean(P[v])[2] for v in range(P.shape[0])], np.float32)
+    assert np.array_equal(means, ref_means)
+    rw = O.rweight(P.reshape(-1, P.shape[2]), prec="f32").reshape(P.shape)
+    st = O.cosine_stretch(rw, ang, prec="f32")
+    assert rel(filt, st) <= 1e-5
+
+
+def test_sirt_matches_oracle(gpu, rel):
+    P, ang = case(gpu, nx=96, ny=6, nviews=21, thick=32, seed=9)
+    opts = gpu.TiltOptions(thickness=32, sirt=4, sirt_relax=0.8)
+    vol, hs, s = gpu.reconstruct(P, ang, opts, return_session=True)
+    res = s.residuals()
+    s.close()
+    o, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=32, sirt=4, sirt_relax=0.8))
+    assert rel(vol, o) <= TOL
+    assert res.shape == (4,) and np.all(np.isfinite(res)) and np.all(res > 0)
+
+
+def test_sirt_xtilt_scatter(gpu, rel):
+    P, ang = case(gpu, nx=96, ny=10, nviews=21, thick=32, seed=10)
+    xt = np.full(21, 1.0, np.float32)
+    opts = gpu.TiltOptions(thickness=32, sirt=2)
+    g, _, o, _ = both(gpu, P, ang, opts, xtilt=xt)
+    assert rel(g, o) <= TOL
+
+
+def test_fake_sirt(gpu, rel):
+    P, ang = case(gpu, seed=11, ny=8)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=48, fake_sirt=2,
+                                                   densweight=1.5))
+    assert rel(g, o) <= TOL
+
+
+def test_seam_session_rows(gpu, rel):
+    P, ang = case(gpu, seed=12, ny=10)
+    rw = O.rweight(P.reshape(-1, P.shape[2]), prec="f32").reshape(P.shape)
+    st = O.cosine_stretch(rw, ang, prec="f32")
+    means = np.array([O.min_max_mean(P[v])[2] for v in range(P.shape[0])], np.float32)
+    tab = O.proj_params(ang, prec="f32")
+    s = gpu.GpuSession.new(st, (ang,), means, 128, 10, 31, 128, 48)
+    assert s is not None
+    row3 = s.reconstruct_row(3).reshape(48, 128)
+    rows = s.reconstruct_rows(0, 10)
+    s.close()
+    ref = O.backproject(st, tab, means, 128, 48, fill="mean", prec="f32")
+    assert rel(row3, ref[3]) <= TOL
+    assert rel(rows, ref) <= TOL
+
+
+def test_deterministic(gpu):
+    P, ang = case(gpu, seed=13, ny=8, xtilt=None)
+    a, _ = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=48))
+    b, _ = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=48))
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 3e-3), ("bf16", 2e-2)])
+def test_half_storage_vs_fp64(gpu, rel, storage, tol):
+    P, ang = case(gpu, seed=14, ny=8)
+    g, _ = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=48, storage=storage))
+    o64, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=48), prec="f64")
+    err = rel(g, o64)
+    print(f"{storage}: max|d|/rms vs FP64 = {err:.3e}")
+    assert err <= tol
+
+
+def test_c0_config(gpu, rel):
+    """Config 0: 512x512 projections, 61 views -60..60 step 2, thickness 128,
+    radial 0.35,0.05, phantom of 200 ellipsoids + 30 beads, noise 0.1."""
+    ang = np.arange(-60, 61, 2, dtype=np.float32)
+    P = gpu.synth_series(512, 512, ang, vol_nz=128, n_ellipsoids=200, n_beads=30, seed=0,
+                         noise_rel=0.1)
+    opts = gpu.TiltOptions(thickness=128, radial=(0.35, 0.05))
+    print("case: C0 512x512x61 -> z128", flush=True)
+    g, sg = gpu.reconstruct(P, ang, opts)
+    o, so = O.tilt_run(P, ang, O.TiltOptions(thickness=128), nthreads=16)
+    err = rel(g, o)
+    print(f"C0 max|d|/rms = {err:.3e}")
+    assert err <= TOL
