@@ -67,6 +67,9 @@ def _declare(L):
         getattr(L, f"or_tilt_run_{suf}").argtypes = [
             C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
             _ip, _ip, _fp, _fp, _fp, _fp, _fp, _fp]
+        getattr(L, f"or_tilt_run_rows_{suf}").argtypes = [
+            C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
+            _ip, _ip, _fp, _fp, _fp, _fp, C.c_int, C.c_int, _fp, _fp]
         getattr(L, f"or_backproject_simple_{suf}").argtypes = [
             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
             _fp, _fp, _fp, C.c_int, C.c_int, _fp, C.c_int]
@@ -348,9 +351,10 @@ class TiltOptions:
 
 
 def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
-             prec="f32", nthreads=8):
+             prec="f32", nthreads=8, rows=None):
     """The whole reconstruction (main.rs:601-1093 after reading).  P: [nz][ny][nx]
-    raw projections.  Returns (volume [ny][out_nz][out_nx] float32, (amin, amax, amean))."""
+    raw projections.  Returns (volume [ny][out_nz][out_nx] float32, (amin, amax, amean)).
+    rows=(y0, y1) reconstructs (and returns) only those output rows."""
     P = np.array(P, dtype=np.float32, copy=True)
     nz, ny, nx = P.shape
     ang = _f32c(angles)
@@ -364,7 +368,8 @@ def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
                opts.max_cosine_tilt, int(opts.log), FILL.get(opts.fill_mode.lower(), 0),
                opts.densweight, opts.outscale, opts.outadd, int(opts.sirt),
                opts.sirt_relax, int(opts.fake_sirt), nthreads)
-    vol = np.zeros((ny, out_nz, out_nx), np.float32)
+    y0, y1 = (0, ny) if rows is None else rows
+    vol = np.zeros((y1 - y0, out_nz, out_nx), np.float32)
     stats = np.zeros(3, np.float32)
     xt = _f32c(xtilt)
     zf = _f32c(zfactor)
@@ -374,7 +379,7 @@ def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
               _f(ldx), _f(ldy))
     else:
         lp = (None,) * 6
-    getattr(lib(), f"or_tilt_run_{prec}")(
+    getattr(lib(), f"or_tilt_run_rows_{prec}")(
         C.byref(o), _f(P), _f(ang), ang.size, _f(xt), 0 if xt is None else xt.size,
-        _f(zf), 0 if zf is None else zf.size, *lp, _f(vol), _f(stats))
+        _f(zf), 0 if zf is None else zf.size, *lp, y0, y1, _f(vol), _f(stats))
     return vol, (float(stats[0]), float(stats[1]), float(stats[2]))

@@ -556,6 +556,17 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
                     const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
                     const int* lnx, const int* lny, const float* lxs, const float* lys,
                     const float* ldx, const float* ldy, float* vol, float* stats) {
+    F(or_tilt_run_rows)(o, P_in, angles, n_angles, xtilt, n_xtilt, zfac, n_zfac, lnx, lny,
+                        lxs, lys, ldx, ldy, 0, o->in_ny, vol, stats);
+}
+
+/* The same for output rows [y0, y1) only (WBP: rows are independent); SIRT and
+ * fake-SIRT always reconstruct every row first.  vol = [y1-y0][out_nz][out_nx]. */
+void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int n_angles,
+                         const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
+                         const int* lnx, const int* lny, const float* lxs, const float* lys,
+                         const float* ldx, const float* ldy, int y0, int y1, float* vol,
+                         float* stats) {
     long nxy = (long)o->in_nx * o->in_ny;
     long total = nxy * o->nz;
     if (o->log_)
@@ -630,8 +641,10 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
     }
     int out_ny = o->in_ny;
     long nv = (long)o->out_nx * o->out_nz;
-    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(out_ny * nv > 0 ? out_ny * nv : 1));
-    F(or_backproject)(&g, P, 0, out_ny, V, o->nthreads);
+    int whole = o->sirt > 0 || o->fake_sirt > 0;
+    int ry0 = whole ? 0 : y0, ry1 = whole ? out_ny : y1;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)((ry1 - ry0) * nv > 0 ? (ry1 - ry0) * nv : 1));
+    F(or_backproject)(&g, P, ry0, ry1, V, o->nthreads);
     if (o->sirt > 0) {
         REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)total);
         REAL* D = (REAL*)malloc(sizeof(REAL) * (size_t)(out_ny * nv > 0 ? out_ny * nv : 1));
@@ -652,10 +665,11 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
     /* output transform + statistics, main.rs:1077-1093 */
     float gmin = FLT_MAX, gmax = -FLT_MAX;
     double gsum = 0;
-    for (int iy = 0; iy < out_ny; iy++) {
-        float* dst = vol + (long)iy * nv;
+    for (int iy = y0; iy < y1; iy++) {
+        float* dst = vol + (long)(iy - y0) * nv;
+        const REAL* src = V + (long)(iy - ry0) * nv;
         for (long i = 0; i < nv; i++) {
-            REAL v = (V[iy * nv + i] * (REAL)o->densweight) * (REAL)o->outscale + (REAL)o->outadd;
+            REAL v = (src[i] * (REAL)o->densweight) * (REAL)o->outscale + (REAL)o->outadd;
             dst[i] = (float)v;
         }
         float mmm[3];
@@ -666,7 +680,7 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
     }
     stats[0] = gmin;
     stats[1] = gmax;
-    stats[2] = (float)(gsum / (double)(nv * out_ny));
+    stats[2] = (y1 > y0) ? (float)(gsum / (double)(nv * (y1 - y0))) : 0.0f;
     free(V);
     free(tab);
     free(means);
