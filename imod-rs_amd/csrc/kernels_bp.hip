@@ -4,16 +4,20 @@
 // Work decomposition.  A workgroup (4 waves) owns one output Y row iy and a
 // 64 (x) by 4*ZPT (z) tile of that row's XZ slice.  Lane l of every wave owns
 // ox = tile_x*64 + l; wave w owns ZPT consecutive oz values, so the per-(oz,view)
-// term (dz*sin_t)*zfactor is wave-uniform and comes from a table through
-// scalar loads.  FP32 accumulators live in registers for all views.
+// term (dz*sin_t)*zfactor is wave-uniform and arrives through scalar loads.
+// FP32 accumulators live in registers for all views.
 //
-// LDS staging.  Per view, the workgroup stages the projection x-segment the
-// tile can touch as (v0, v1-v0) pairs -- or, for the two-row (x-axis tilt /
-// local alignment) path, (v0_r, d_r, v0_r+1, d_r+1) quads -- with one padded
-// entry on each side that holds (fill, 0).  Clamping floor(proj_x) into
-// [-1, in_nx-1] then reproduces sample_row_with_fill (main.rs:386-406)
-// without a per-sample select: v = v0 + frac*d is the lerp inside the row and
-// exactly the fill value outside it.
+// LDS staging.  Per view the workgroup stages one entry for every x in
+// [floor(min proj_x), floor(max proj_x)] of the tile:
+//   one-row geometry:  (v0, d0)            d0 = v[x+1]-v[x]
+//   two-row geometry:  (v0, d0, dv, dd)    rows r and r+1, dv = v0'-v0, dd = d0'-d0
+// for 0 <= x <= in_nx-2, and (fill, 0[, fill'-fill, 0]) for x outside that range
+// (left fill for x < 0, right fill for x >= in_nx-1).  Indexing the entries with
+// floor(proj_x) then reproduces sample_row_with_fill (main.rs:386-406) with no
+// clamp and no per-sample select: v = v0 + frac*d is the lerp inside the row and
+// exactly the fill value outside it.  The next view's entries are loaded into
+// registers before the current view is accumulated, so the global-load latency
+// hides behind the interpolation work; one barrier per view.
 //
 // Exactness.  The file is compiled with -ffp-contract=off, so the coordinate
 // chain proj_x = ((dx*cos_t)*cos_xt + (dz*sin_t)*zf) + cx [+ ldx] and
@@ -27,7 +31,14 @@
 
 #include "tiltgpu_internal.h"
 
+#define TILTGPU_CONSTANT __attribute__((address_space(4)))
+
 namespace tiltgpu {
+
+template <typename T>
+__device__ __forceinline__ const TILTGPU_CONSTANT T* cptr(const T* p) {
+    return (const TILTGPU_CONSTANT T*)(p);
+}
 
 // Workgroup id -> logical tile.  Workgroups are dispatched round-robin over the
 // 8 XCDs; giving each XCD a contiguous run of logical tiles keeps the tiles of
@@ -39,17 +50,24 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
 }
 
+// Raw projection sample as stored (f32 bits or 16-bit code); decoded after the
+// compute of the previous view so the loads stay in flight meanwhile.
 template <int ST>
-__device__ __forceinline__ float load_proj(const void* base, long i, float scale) {
-    if constexpr (ST == 0) {
-        return static_cast<const float*>(base)[i];
-    } else if constexpr (ST == 1) {
-        const unsigned short u = static_cast<const unsigned short*>(base)[i];
-        return __half2float(__ushort_as_half(u)) * scale;
-    } else {
-        const unsigned short u = static_cast<const unsigned short*>(base)[i];
-        return __uint_as_float(static_cast<unsigned>(u) << 16) * scale;
-    }
+__device__ __forceinline__ unsigned load_raw(const void* base, long i) {
+    if constexpr (ST == 0)
+        return __float_as_uint(static_cast<const float*>(base)[i]);
+    else
+        return static_cast<unsigned>(static_cast<const unsigned short*>(base)[i]);
+}
+
+template <int ST>
+__device__ __forceinline__ float decode(unsigned u, float scale) {
+    if constexpr (ST == 0)
+        return __uint_as_float(u);
+    else if constexpr (ST == 1)
+        return __half2float(__ushort_as_half(static_cast<unsigned short>(u))) * scale;
+    else
+        return __uint_as_float(u << 16) * scale;
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -71,6 +89,16 @@ __device__ __forceinline__ int wave_max_i(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return v;
+}
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// floor(x) as an int in one instruction (x is already clamped to a small range).
+__device__ __forceinline__ int cvt_floor(float x) {
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
 }
 
 // ViewPatches::find_bracket, main.rs:200-228.
@@ -97,16 +125,11 @@ __device__ __forceinline__ void find_bracket(const float* pos, int len, float va
     frac = span > 1e-12f ? (val - pos[lo]) / span : 0.f;
 }
 
-// ViewPatches::interpolate, main.rs:172-198 (lerp a + t*(b-a), unfused).
-// Returns false when the view has no patches (no shift is applied then).
-template <typename PP>
-__device__ __forceinline__ bool local_shift(const PP& P, int view, float x, float y,
-                                            float& odx, float& ody) {
+// ViewPatches::interpolate, main.rs:172-198 (lerp a + t*(b-a), unfused), for a
+// view with a patch grid.
+__device__ __forceinline__ void grid_shift(const BpParams& P, int view, float x, float y,
+                                           float& odx, float& ody) {
     const int nx = P.lnx[view], ny = P.lny[view];
-    odx = 0.f;
-    ody = 0.f;
-    if (nx == 0) return false;
-    if (ny == 0) return true;
     const float* xs = P.lxs + P.lxoff[view];
     const float* ys = P.lys + P.lyoff[view];
     const float* gx = P.ldx + P.lgoff[view];
@@ -125,27 +148,214 @@ __device__ __forceinline__ bool local_shift(const PP& P, int view, float x, floa
     const float ay = d00y + fx * (d10y - d00y), by = d01y + fx * (d11y - d01y);
     odx = ax + fy * (bx - ax);
     ody = ay + fy * (by - ay);
-    return true;
 }
 
-// One staged entry of row `row` (pointer to its first element) at x:
-// (v[x], v[x+1]-v[x]) inside, (fill, 0) on the padded sides.
-template <int ST>
-__device__ __forceinline__ float2 stage_entry(const void* proj, long row_off, int x, int n,
-                                              float scale, int fill_mode, float mean) {
-    if (x >= 0 && x <= n - 2) {
-        const float u0 = load_proj<ST>(proj, row_off + x, scale);
-        const float u1 = load_proj<ST>(proj, row_off + x + 1, scale);
-        return make_float2(u0, u1 - u0);
+constexpr int kSlots = 2;  // staged entries prefetched per thread
+
+// Per-view state of one lane and of its wave (the wave-level values are equal in
+// all four waves: they hold the same 64 ox positions).
+struct Vs {
+    float a;      // (dx*cos_t)*cos_xt
+    float ldx;    // x shift (0 when none)
+    float fy;     // weight of row r+1 (0: one row)
+    int row;      // row pair start r, or -1 (no contribution)
+    int seg_lo, L, r_lo, NR, total, shift;
+};
+
+template <bool TWO, bool LOCAL>
+__device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int ox, int iy,
+                                         int zt0, int TZ) {
+    Vs s;
+    const TILTGPU_CONSTANT BpView* V = cptr(P.views) + p;
+    const float cos_t = V->cos_t, cos_xt = V->cos_xt, sin_xt = V->sin_xt;
+    const int mode = LOCAL ? V->shift_mode : 0;
+    s.shift = mode;
+    s.a = (dx * cos_t) * cos_xt;
+    float ldx = 0.f, ldy = 0.f;
+    if (LOCAL) {
+        if (mode == 1) {
+            ldx = V->sdx;
+            ldy = V->sdy;
+        } else if (mode == 2) {
+            grid_shift(P, p, static_cast<float>(ox), static_cast<float>(iy), ldx, ldy);
+        }
     }
-    float f;
-    if (fill_mode == 0)
-        f = mean;
-    else if (fill_mode == 1)
-        f = 0.f;
+    s.ldx = ldx;
+    const TILTGPU_CONSTANT float* zc = cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad;
+    const float z_first = zc[zt0], z_last = zc[zt0 + TZ - 1];
+    float pa = (s.a + z_first) + P.cx;
+    float pb = (s.a + z_last) + P.cx;
+    if (mode) {
+        pa = pa + ldx;
+        pb = pb + ldx;
+    }
+    float lo = fminf(pa, pb), hi = fmaxf(pa, pb);
+    if (mode == 2) {
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+    } else {
+        // proj_x is monotonic in ox (a constant shift keeps it so): extremes at lanes 0, 63
+        lo = fminf(lane_f(lo, 0), lane_f(lo, 63));
+        hi = fmaxf(lane_f(hi, 0), lane_f(hi, 63));
+    }
+    s.seg_lo = cvt_floor(lo);
+    s.L = cvt_floor(hi) - s.seg_lo + 1;
+    s.row = iy;
+    s.fy = 0.f;
+    s.r_lo = iy;
+    int NR = 1;
+    if constexpr (TWO) {
+        const float yo = dx * sin_xt;
+        float sy = static_cast<float>(iy) + yo;
+        if (mode) sy = sy + ldy;
+        const float fl = floorf(sy);
+        const float fny = static_cast<float>(P.in_ny);
+        const bool two = LOCAL || fabsf(yo) > 1e-6f;
+        s.row = -1;
+        if (two && fl >= 0.f && fl + 1.f < fny) {
+            s.row = static_cast<int>(fl);
+            s.fy = sy - fl;
+        } else if (fl >= 0.f && fl < fny) {
+            s.row = static_cast<int>(fl);
+        }
+        int rmin, rmax;
+        if (mode == 2) {
+            rmin = wave_min_i(s.row >= 0 ? s.row : INT_MAX);
+            rmax = wave_max_i(s.row);
+        } else {
+            // floor(sy) is monotonic in ox; clamp the extremes into the valid rows
+            const float f0 = lane_f(fl, 0), f1 = lane_f(fl, 63);
+            const float lo_r = fmaxf(fminf(f0, f1), 0.f);
+            const float hi_r = fminf(fmaxf(f0, f1), fny - 1.f);
+            rmin = lo_r <= hi_r ? static_cast<int>(lo_r) : INT_MAX;
+            rmax = lo_r <= hi_r ? static_cast<int>(hi_r) : -1;
+        }
+        if (rmax < 0) {
+            NR = 0;
+            s.r_lo = 0;
+        } else {
+            s.r_lo = rmin;
+            NR = rmax - rmin + 1;
+        }
+    }
+    s.NR = NR;
+    s.total = NR * s.L;
+    return s;
+}
+
+// Global loads for staged entry e of view p: v[x], v[x+1] of row r (and r+1);
+// a padded entry loads the edge value instead (used by the Edge fill).
+template <bool TWO, int ST>
+__device__ __forceinline__ void load_entry(const BpParams& P, const Vs& s, int p, int iy,
+                                           int ri, int xi, unsigned (&r)[4]) {
+    const int n = P.in_nx;
+    const long nxy = static_cast<long>(n) * P.in_ny;
+    const int x = s.seg_lo + xi;
+    const int row = TWO ? s.r_lo + ri : iy;
+    const long off = static_cast<long>(p) * nxy + static_cast<long>(row) * n;
+    const int i0 = x < 0 ? 0 : (x > n - 2 ? n - 1 : x);
+    const int i1 = (x >= 0 && x <= n - 2) ? x + 1 : i0;
+    r[0] = load_raw<ST>(P.proj, off + i0);
+    r[1] = load_raw<ST>(P.proj, off + i1);
+    r[2] = r[3] = 0u;
+    if constexpr (TWO) {
+        if (row + 1 < P.in_ny) {  // a row past the end is only used with weight 0
+            r[2] = load_raw<ST>(P.proj, off + n + i0);
+            r[3] = load_raw<ST>(P.proj, off + n + i1);
+        }
+    }
+}
+
+template <bool TWO, int ST>
+__device__ __forceinline__ float4 make_entry(const BpParams& P, const Vs& s, int ri, int xi,
+                                             const unsigned (&r)[4], float scale, float mean) {
+    const int x = s.seg_lo + xi;
+    const bool inside = x >= 0 && x <= P.in_nx - 2;
+    const float a0 = decode<ST>(r[0], scale), a1 = decode<ST>(r[1], scale);
+    float v0, d0;
+    if (inside) {
+        v0 = a0;
+        d0 = a1 - a0;
+    } else {
+        v0 = P.fill_mode == 0 ? mean : (P.fill_mode == 1 ? 0.f : a0);
+        d0 = 0.f;
+    }
+    if constexpr (!TWO) {
+        return make_float4(v0, d0, 0.f, 0.f);
+    } else {
+        float v1 = 0.f, d1 = 0.f;
+        if (s.r_lo + ri + 1 < P.in_ny) {
+            const float b0 = decode<ST>(r[2], scale), b1 = decode<ST>(r[3], scale);
+            if (inside) {
+                v1 = b0;
+                d1 = b1 - b0;
+            } else {
+                v1 = P.fill_mode == 0 ? mean : (P.fill_mode == 1 ? 0.f : b0);
+            }
+        }
+        return make_float4(v0, d0, v1 - v0, d1 - d0);
+    }
+}
+
+template <bool TWO, int ST>
+__device__ __forceinline__ void put_entry(void* lds, int idx, float4 q) {
+    if constexpr (TWO)
+        static_cast<float4*>(lds)[idx] = q;
     else
-        f = load_proj<ST>(proj, row_off + (x < 0 ? 0 : n - 1), scale);
-    return make_float2(f, 0.f);
+        static_cast<float2*>(lds)[idx] = make_float2(q.x, q.y);
+}
+
+// Staging work split: slot k of a thread is entry x = seg_lo + tid of row pair k
+// (two-row geometry) or entry x = seg_lo + tid + 256*k (one-row geometry).
+// Entries beyond the slots (segments wider than 256, more than kSlots row
+// pairs) are loaded and stored directly in stage_store.
+template <bool TWO>
+__device__ __forceinline__ bool slot_entry(const Vs& s, int k, int& ri, int& xi) {
+    if constexpr (TWO) {
+        ri = k;
+        xi = threadIdx.x;
+        return ri < s.NR && xi < s.L;
+    } else {
+        ri = 0;
+        xi = threadIdx.x + k * kBpThreads;
+        return xi < s.L;
+    }
+}
+
+template <bool TWO, int ST>
+__device__ __forceinline__ void stage_load(const BpParams& P, const Vs& s, int p, int iy,
+                                           unsigned (&raw)[kSlots][4]) {
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+        int ri, xi;
+        if (slot_entry<TWO>(s, k, ri, xi)) load_entry<TWO, ST>(P, s, p, iy, ri, xi, raw[k]);
+    }
+}
+
+template <bool TWO, int ST>
+__device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int p, int iy,
+                                            const unsigned (&raw)[kSlots][4], void* lds,
+                                            int buf) {
+    const float scale = (ST == 0 || P.view_scale == nullptr) ? 1.f : cptr(P.view_scale)[p];
+    const float mean = cptr(P.means)[p];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+        int ri, xi;
+        if (slot_entry<TWO>(s, k, ri, xi))
+            put_entry<TWO, ST>(lds, buf + ri * s.L + xi,
+                               make_entry<TWO, ST>(P, s, ri, xi, raw[k], scale, mean));
+    }
+    // entries beyond the prefetch slots: load and store directly
+    for (int ri = 0; ri < s.NR; ++ri) {
+        int xi0 = threadIdx.x;
+        if (TWO ? ri < kSlots : true) xi0 += TWO ? kBpThreads : kSlots * kBpThreads;
+        for (int xi = xi0; xi < s.L; xi += kBpThreads) {
+            unsigned one[4];
+            load_entry<TWO, ST>(P, s, p, iy, ri, xi, one);
+            put_entry<TWO, ST>(lds, buf + ri * s.L + xi,
+                               make_entry<TWO, ST>(P, s, ri, xi, one, scale, mean));
+        }
+    }
 }
 
 template <int ZPT, bool TWO, bool LOCAL, int ST>
@@ -163,141 +373,73 @@ __global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P) {
     const int ox = tx * kBpTileX + lane;
     const int zt0 = tz * TZ;
     const int zw0 = zt0 + wave * ZPT;
-    const int n = P.in_nx;
-    const float nm1 = static_cast<float>(n - 1);
     const float dx = static_cast<float>(ox) - P.ocx;
-    const long nxy = static_cast<long>(n) * P.in_ny;
 
     float acc[ZPT];
 #pragma unroll
     for (int k = 0; k < ZPT; ++k) acc[k] = 0.f;
-    int staged = 0;  // views staged so far (workgroup-uniform): selects the LDS buffer
 
-    for (int p = 0; p < P.nviews; ++p) {
-        const float4 g = P.vgeom[p];
-        const float a = (dx * g.x) * g.z;
-        float ldx = 0.f, ldy = 0.f;
-        bool shifted = false;
-        if constexpr (LOCAL) shifted = local_shift(P, p, static_cast<float>(ox),
-                                                   static_cast<float>(iy), ldx, ldy);
-        const float* zc = P.zc_tab + static_cast<long>(p) * P.nz_pad;
-        const float z_first = zc[zt0], z_last = zc[zt0 + TZ - 1];
-        float pa = (a + z_first) + P.cx;
-        float pb = (a + z_last) + P.cx;
-        if constexpr (LOCAL) {
-            if (shifted) {
-                pa = pa + ldx;
-                pb = pb + ldx;
+    if (P.nviews > 0) {
+        unsigned raw[kSlots][4];
+        Vs cur = view_setup<TWO, LOCAL>(P, 0, dx, ox, iy, zt0, TZ);
+        stage_load<TWO, ST>(P, cur, 0, iy, raw);
+        for (int p = 0; p < P.nviews; ++p) {
+            if (cur.total > P.lds_cap) {
+                // host bound violated: report it (the host fails the call) and skip
+                // the view rather than overrun LDS
+                if (tid == 0) atomicOr(P.err, 1);
+                cur.total = 0;
+                cur.NR = 0;
+                cur.L = 0;
+                cur.row = -1;
             }
-        }
-        float lo = fminf(pa, pb), hi = fmaxf(pa, pb);
-        if constexpr (LOCAL) {
-            lo = wave_min(lo);
-            hi = wave_max(hi);
-        } else {
-            // proj_x is monotonic in ox for a fixed oz: the extremes are at lanes 0, 63.
-            lo = fminf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(lo), 0)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lo), 63)));
-            hi = fmaxf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(hi), 0)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hi), 63)));
-        }
-        const int seg_lo = static_cast<int>(fminf(fmaxf(floorf(lo), -1.f), nm1));
-        const int seg_hi = static_cast<int>(fminf(fmaxf(floorf(hi), -1.f), nm1));
-        const int L = seg_hi - seg_lo + 1;
-
-        int my_row = -1;  // projection row pair start for this lane, -1 = no contribution
-        float fy_eff = 0.f;
-        int r_lo = 0, NR = 1;
-        if constexpr (TWO) {
-            const float yo = dx * g.w;
-            float sy = static_cast<float>(iy) + yo;
-            if constexpr (LOCAL) {
-                if (shifted) sy = sy + ldy;
+            // buffer (p & 1) was last read by the compute of view p-2; every thread
+            // has passed the barrier of view p-1 since, so it is free.
+            const int buf = (p & 1) * P.lds_cap;
+            stage_store<TWO, ST>(P, cur, p, iy, raw, lds4, buf);
+            __syncthreads();
+            Vs nxt = cur;
+            if (p + 1 < P.nviews) {
+                nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
+                stage_load<TWO, ST>(P, nxt, p + 1, iy, raw);
             }
-            const float fl = floorf(sy);
-            const float fy = sy - fl;
-            const bool two = LOCAL || fabsf(yo) > 1e-6f;
-            const float fny = static_cast<float>(P.in_ny);
-            if (two && fl >= 0.f && fl + 1.f < fny) {
-                my_row = static_cast<int>(fl);
-                fy_eff = fy;
-            } else if (fl >= 0.f && fl < fny) {
-                my_row = static_cast<int>(fl);
-                fy_eff = 0.f;
-            }
-            const int rmin = wave_min_i(my_row >= 0 ? my_row : INT_MAX);
-            const int rmax = wave_max_i(my_row);
-            if (rmax < 0) continue;  // no lane of this tile samples this view
-            r_lo = rmin;
-            NR = rmax - rmin + 1;
-        }
-        const int total = NR * L;
-        if (total > P.lds_cap) {
-            if (tid == 0) atomicOr(P.err, 1);
-            return;
-        }
-        // buffer offset in entries (float4 entries for two rows, float2 for one)
-        const int buf = (staged & 1) * P.lds_cap;
-        ++staged;
-        const float scale = (ST == 0 || P.view_scale == nullptr) ? 1.f : P.view_scale[p];
-        const float mean = P.means[p];
-        // -- stage the view into LDS buffer (staged & 1).  That buffer was last read
-        //    by the compute of the view staged two steps ago, and every thread has
-        //    passed the barrier of the view staged one step ago, so no thread still
-        //    reads it.
-        if constexpr (TWO) {
-            for (int e = tid; e < total; e += kBpThreads) {
-                const int ri = e / L;
-                const int x = seg_lo + (e - ri * L);
-                const int r = r_lo + ri;
-                const long off0 = static_cast<long>(p) * nxy + static_cast<long>(r) * n;
-                const float2 e0 = stage_entry<ST>(P.proj, off0, x, n, scale, P.fill_mode, mean);
-                float2 e1 = make_float2(0.f, 0.f);
-                if (r + 1 < P.in_ny)
-                    e1 = stage_entry<ST>(P.proj, off0 + n, x, n, scale, P.fill_mode, mean);
-                lds4[buf + e] = make_float4(e0.x, e0.y, e1.x, e1.y);
-            }
-        } else {
-            float2* lds2 = reinterpret_cast<float2*>(lds4);
-            const long off0 = static_cast<long>(p) * nxy + static_cast<long>(iy) * n;
-            for (int e = tid; e < total; e += kBpThreads)
-                lds2[buf + e] =
-                    stage_entry<ST>(P.proj, off0, seg_lo + e, n, scale, P.fill_mode, mean);
-        }
-        __syncthreads();
-
-        // -- accumulate this view into the ZPT voxels of the lane
-        if constexpr (TWO) {
-            if (my_row >= 0) {
-                const float4* E = lds4 + buf + (my_row - r_lo) * L - seg_lo;
+            const TILTGPU_CONSTANT float* zc =
+                cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
+            if constexpr (TWO) {
+                if (cur.row >= 0) {
+                    const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
+                    const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
+                    if (LOCAL && cur.shift) {
+#pragma unroll
+                        for (int k = 0; k < ZPT; ++k) {
+                            const float px = ((a + zc[k]) + P.cx) + ldx;
+                            const float4 q = E[cvt_floor(px)];
+                            const float f = __builtin_amdgcn_fractf(px);
+                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < ZPT; ++k) {
+                            const float px = (a + zc[k]) + P.cx;
+                            const float4 q = E[cvt_floor(px)];
+                            const float f = __builtin_amdgcn_fractf(px);
+                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                        }
+                    }
+                }
+            } else if (cur.total > 0) {
+                const float2* E = reinterpret_cast<const float2*>(lds4) + buf - cur.seg_lo;
+                const float a = cur.a;
 #pragma unroll
                 for (int k = 0; k < ZPT; ++k) {
-                    float px = (a + zc[zw0 + k]) + P.cx;
-                    if constexpr (LOCAL) {
-                        if (shifted) px = px + ldx;
-                    }
-                    const float fl = floorf(px);
-                    const int i = static_cast<int>(fminf(fmaxf(fl, -1.f), nm1));
-                    const float f = px - fl;
-                    const float4 q = E[i];
-                    const float s0 = fmaf(f, q.y, q.x);
-                    const float s1 = fmaf(f, q.w, q.z);
-                    acc[k] += fmaf(fy_eff, s1 - s0, s0);
+                    const float px = (a + zc[k]) + P.cx;
+                    const float2 q = E[cvt_floor(px)];
+                    const float f = __builtin_amdgcn_fractf(px);
+                    acc[k] += fmaf(f, q.y, q.x);
                 }
             }
-        } else {
-            const float2* E = reinterpret_cast<const float2*>(lds4) + buf - seg_lo;
-#pragma unroll
-            for (int k = 0; k < ZPT; ++k) {
-                const float px = (a + zc[zw0 + k]) + P.cx;
-                const float fl = floorf(px);
-                const int i = static_cast<int>(fminf(fmaxf(fl, -1.f), nm1));
-                const float f = px - fl;
-                const float2 q = E[i];
-                acc[k] += fmaf(f, q.y, q.x);
-            }
+            cur = nxt;
         }
-        // No second barrier: the next staged view goes to the other buffer.
     }
 
     if (ox >= P.out_nx) return;
@@ -358,6 +500,7 @@ hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int t
     switch (tile_z) {
         case 32: return launch_bp_z<8>(p, storage, two_row, st);
         case 64: return launch_bp_z<16>(p, storage, two_row, st);
+        case 128: return launch_bp_z<32>(p, storage, two_row, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct tiltgpu_session {
     float* vol_tmp = nullptr;  // fake-SIRT ping-pong
     float* sino = nullptr;     // SIRT reprojection [nz][in_ny][in_nx]
     float4* vgeom = nullptr;
+    BpView* views = nullptr;
     float* zc = nullptr;
     float* means = nullptr;
     double* partial = nullptr;
@@ -102,6 +104,7 @@ struct tiltgpu_session {
 
     int nz_pad = 0;
     int tile_z = 64;
+    int tile_z_override = 0;  // TILTGPU_TILE_Z (tuning): force the backprojection tile depth
     int lds_cap = 0;
     bool two_row = false;
     int radix[kMaxRadix];
@@ -229,10 +232,19 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
     s->two_row = s->has_local;
     for (const auto& g : vp)
         if (g.sin_xt != 0.0f) s->two_row = true;
-    // tile depth and LDS capacity from the per-view bounds
+    // tile depth: minimise padded depth x (1 + per-view overhead / voxels per lane);
+    // LDS capacity from the per-view bounds
     const int max_lds_bytes = 64 * 1024;
     int chosen = 0;
-    for (int tz : {64, 32}) {
+    const int out_nz0 = s->d.out_nz;
+    std::vector<int> order = {128, 64, 32};
+    auto cost = [&](int tz) {
+        const double pad = static_cast<double>((out_nz0 + tz - 1) / tz * tz);
+        return pad * (1.0 + 16.0 / tz);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
+    if (s->tile_z_override > 0) order = {s->tile_z_override};
+    for (int tz : order) {
         int cap = 1;
         for (int v = 0; v < nv; ++v) {
             int seg, rows;
@@ -255,10 +267,28 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
     const int out_nz = s->d.out_nz;
     s->nz_pad = (out_nz + s->tile_z - 1) / s->tile_z * s->tile_z;
     std::vector<float4> g4(std::max(nv, 1));
+    std::vector<BpView> bv(std::max(nv, 1));
     std::vector<float> zc(static_cast<size_t>(std::max(nv, 1)) * s->nz_pad);
     const float center_z = static_cast<float>(out_nz) / 2.0f;
     for (int v = 0; v < nv; ++v) {
         g4[v] = make_float4(vp[v].cos_t, vp[v].sin_t, vp[v].cos_xt, vp[v].sin_xt);
+        BpView& b = bv[v];
+        std::memset(&b, 0, sizeof b);
+        b.cos_t = vp[v].cos_t;
+        b.sin_t = vp[v].sin_t;
+        b.cos_xt = vp[v].cos_xt;
+        b.sin_xt = vp[v].sin_xt;
+        b.shift_mode = 0;
+        if (s->has_local && s->lnx[v] > 0) {
+            if (s->lnx[v] == 1 && s->lny[v] == 1) {
+                // ViewPatches::interpolate of a one-patch grid returns that patch
+                b.shift_mode = 1;
+                b.sdx = s->ldx[s->lgoff[v]];
+                b.sdy = s->ldy[s->lgoff[v]];
+            } else {
+                b.shift_mode = 2;
+            }
+        }
         for (int oz = 0; oz < s->nz_pad; ++oz) {
             const float dz = static_cast<float>(oz) - center_z;
             // main.rs:1015-1016: z_contrib = dz * sin_t * zfactor
@@ -266,11 +296,14 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
         }
     }
     dev_free(s, s->vgeom);
+    dev_free(s, s->views);
     dev_free(s, s->zc);
     int rc;
-    if ((rc = dev_alloc(s, &s->vgeom, g4.size())) || (rc = dev_alloc(s, &s->zc, zc.size())))
+    if ((rc = dev_alloc(s, &s->vgeom, g4.size())) || (rc = dev_alloc(s, &s->views, bv.size())) ||
+        (rc = dev_alloc(s, &s->zc, zc.size())))
         return rc;
     HIP_TRY(hipMemcpy(s->vgeom, g4.data(), g4.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->views, bv.data(), bv.size() * sizeof(BpView), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->zc, zc.data(), zc.size() * sizeof(float), hipMemcpyHostToDevice));
     s->geometry_ready = true;
     return TILTGPU_OK;
@@ -376,7 +409,7 @@ BpParams make_bp(tiltgpu_session* s, const void* proj, const float* scale, int y
     BpParams p{};
     p.proj = proj;
     p.view_scale = scale;
-    p.vgeom = s->vgeom;
+    p.views = s->views;
     p.zc_tab = s->zc;
     p.means = s->means;
     p.out = out;
@@ -534,6 +567,10 @@ int validate_dims(const tiltgpu_dims* d) {
 }
 
 int init_device(tiltgpu_session* s) {
+    if (const char* tz = std::getenv("TILTGPU_TILE_Z")) {
+        const int v = std::atoi(tz);
+        if (v == 32 || v == 64 || v == 128) s->tile_z_override = v;
+    }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
         return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");

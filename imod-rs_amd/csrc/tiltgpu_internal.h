@@ -12,17 +12,28 @@ constexpr int kBpThreads = 256;  // 4 waves per backprojection workgroup
 constexpr int kBpTileX = 64;     // one wave spans 64 output x positions
 constexpr int kMaxRadix = 20;
 
+// Per-view constants of the backprojection, read through the scalar cache.
+// shift_mode: 0 = no local shift, 1 = constant shift (sdx, sdy) (a one-patch
+// grid: ViewPatches::interpolate returns its single value), 2 = patch grid
+// interpolated per (ox, iy).
+struct alignas(32) BpView {
+    float cos_t, sin_t, cos_xt, sin_xt;
+    float sdx, sdy;
+    int shift_mode;
+    int pad;
+};
+
 // Backprojection of one series (one launch covers Y rows [y0, y1)).
 //
 // Layout in HBM:
 //   proj    [nz][in_ny][in_nx]  storage type (f32, f16 or bf16)
 //   out     [y1-y0][out_nz][out_nx] f32 (the MRC section order)
 //   zc_tab  [nviews][nz_pad] f32: (dz * sin_t) * zfactor, dz = oz - out_nz/2
-//   vgeom   [nviews] float4 (cos_t, sin_t, cos_xt, sin_xt)
+//   views   [nviews] BpView
 struct BpParams {
     const void* proj;
     const float* view_scale;  // [nz] dequantisation scale (f16/bf16), else null
-    const float4* vgeom;
+    const BpView* views;
     const float* zc_tab;
     const float* means;       // [nz] per-view means (Mean fill)
     float* out;

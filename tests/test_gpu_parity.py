@@ -227,3 +227,22 @@ def test_c0_config(gpu, rel):
     err = rel(g, o)
     print(f"C0 max|d|/rms = {err:.3e}")
     assert err <= TOL
+
+
+@pytest.mark.parametrize("tile_z", [32, 64, 128])
+@pytest.mark.parametrize("geom", ["plain", "xtilt", "local"])
+def test_tile_depths(gpu, rel, monkeypatch, tile_z, geom):
+    """Every backprojection tile depth gives the oracle's result (one-row,
+    two-row and local-shift kernels)."""
+    monkeypatch.setenv("TILTGPU_TILE_Z", str(tile_z))
+    rng = np.random.default_rng(20 + tile_z)
+    sx = rng.normal(0, 2, 31).astype(np.float32)
+    sy = rng.normal(0, 2, 31).astype(np.float32)
+    xt = np.full(31, 0.5, np.float32) if geom != "plain" else None
+    P, ang = case(gpu, seed=21, ny=12, thick=150, xtilt=xt,
+                  shifts=(sx, sy) if geom == "local" else None)
+    local = None
+    if geom == "local":
+        local = O.build_local_grids([(v, 64.0, 6.0, float(sx[v]), float(sy[v])) for v in range(31)], 31)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=150), xtilt=xt, local=local)
+    assert rel(g, o) <= TOL
