@@ -1,0 +1,339 @@
+"""Pins the oracle (CPU restatement of apps/tilt/src/main.rs) before it is
+trusted as the GPU path's checker: the reference's own known-answer tests,
+closed forms, invariants and brute force on tiny inputs.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------- reference KATs --
+
+def test_min_max_mean_known_answer():
+    # crates/imod-math/src/tests.rs:17-23
+    g = json.load(open(os.path.join(GOLDEN, "min_max_mean.json")))
+    mn, mx, mean = O.min_max_mean(np.array(g["input"], np.float32))
+    assert mn == g["min"] and mx == g["max"]
+    assert abs(mean - g["mean"]) < g["mean_tol"]
+
+
+def test_min_max_mean_f64_accumulation():
+    # the sum is kept in f64 (stats.rs:851-858): an f32 sum would lose the 1
+    a = np.array([1e8, 1.0, -1e8], np.float32)
+    assert O.min_max_mean(a)[2] == pytest.approx(1.0 / 3.0, rel=1e-6)
+    assert O.min_max_mean(np.zeros(0, np.float32)) == (0.0, 0.0, 0.0)
+
+
+def test_tilt_file_roundtrip(tmp_path):
+    # crates/imod-transforms/src/tests.rs:118-131: -60..60 step 3, tolerance 0.1
+    ang = O.read_tilt_file(os.path.join(GOLDEN, "tilt_roundtrip.tlt"))
+    ref = np.arange(-60, 61, 3, dtype=np.float32)
+    assert ang.shape == ref.shape
+    assert np.all(np.abs(ang - ref) < 0.1)
+
+
+def test_tilt_file_skips_blank_and_garbage(tmp_path):
+    p = tmp_path / "x.tlt"
+    p.write_text("  -60.0 \n\nabc\n1e1\n0x10\n 3,5\n+2.5\ninf\n")
+    ang = O.read_tilt_file(str(p))
+    assert list(ang[:3]) == [-60.0, 10.0, 2.5] and np.isinf(ang[3]) and len(ang) == 4
+
+
+def test_generate_tilt_angles():
+    # crates/imod-transforms/src/tests.rs:133-140
+    g = json.load(open(os.path.join(GOLDEN, "generate_tilt_angles.json")))
+    a = np.array([np.float32(g["start"]) + np.float32(i) * np.float32(g["increment"])
+                  for i in range(g["count"])], np.float32)
+    assert len(a) == g["count"]
+    for i, v, tol in g["checks"]:
+        assert abs(a[i] - v) < tol
+
+
+def test_per_view_file_parse_error(tmp_path):
+    p = tmp_path / "xt.txt"
+    p.write_text("# x tilt\n0.5\n\n0.25\n")
+    assert list(O.load_per_view_file(str(p))) == [0.5, 0.25]
+    p.write_text("0.5\nnope\n")
+    with pytest.raises(ValueError):
+        O.load_per_view_file(str(p))
+
+
+# ------------------------------------------------------------------ the DFT --
+
+@pytest.mark.parametrize("n", [2, 7, 12, 30, 97, 256, 768, 1000])
+def test_dft_matches_numpy(n):
+    x = np.random.default_rng(n).normal(size=n) + 1j * np.random.default_rng(n + 1).normal(size=n)
+    x32 = x.astype(np.complex64)
+    ref = np.fft.fft(x32.astype(np.complex128))
+    scale = np.max(np.abs(ref))
+    # the wrapper returns float32, so the f64 form is checked to f32 rounding
+    assert np.max(np.abs(O.dft(x32, -1, "f64") - ref)) < 1.2e-7 * scale
+    assert np.max(np.abs(O.dft(x32, -1, "f32") - ref)) < 2e-6 * scale
+    inv = np.fft.ifft(x32.astype(np.complex128)) * n
+    assert np.max(np.abs(O.dft(x32, +1, "f64") - inv)) < 1.2e-7 * scale
+
+
+# -------------------------------------------------------- radial weighting --
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_radial_weight_shape(prec):
+    n, c, f = 256, 0.35, 0.05
+    W = O.radial_weights(n, c, f, prec=prec)
+    nxc = n // 2 + 1
+    k = np.arange(n)
+    fb = np.where(k <= n // 2, k, n - k)
+    ramp = fb / nxc
+    below = ramp <= c
+    # a pure ramp up to the cutoff (the Ram-Lak filter), DC removed
+    assert W[0] == 0
+    assert np.allclose(W[below], ramp[below], rtol=1e-7, atol=0)
+    # symmetric: W(k) = W(n-k)
+    assert np.array_equal(W[1:], W[1:][::-1])
+    # Gaussian roll-off: at one falloff past the cutoff the weight is w*exp(-1/2)
+    w1 = (c + f)
+    Wc = O.radial_weights(n, c, f, exact_filter_size=100, prec=prec)  # w = k/100
+    k1 = int(round(w1 * 100))
+    assert Wc[k1] == pytest.approx(w1 * np.exp(-0.5), rel=1e-5)
+    assert np.all(np.diff(W[: n // 2 + 1][~below[: n // 2 + 1]]) < 0)
+
+
+def test_radial_weight_hamming():
+    n, c = 200, 0.3
+    W = O.radial_weights(n, c, 0.05, exact_filter_size=100, hamming=True, prec="f64")
+    w = np.arange(n) / 100.0
+    w = np.minimum(w, (n - np.arange(n)) / 100.0)
+    # window 0.54 + 0.46 cos(pi w / c): 1 at w=0, 0.08 at w=c, 0 beyond
+    c32 = float(np.float32(c))  # the cutoff arrives as an f32 (main.rs:582)
+    assert W[10] == pytest.approx(0.1 * (0.54 + 0.46 * np.cos(np.pi * 0.1 / c32)), rel=1e-12)
+    assert W[30] == pytest.approx(0.3 * 0.08, rel=1e-9)
+    assert np.all(W[31:n - 30] == 0)
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f64", 2e-7)])  # f64 returned as f32
+@pytest.mark.parametrize("n", [64, 96, 127])
+def test_rweight_cosine_rows_are_eigenvectors(prec, tol, n):
+    """A pure cosine at bin k is multiplied by W(k): the filter is diagonal in
+    the Fourier basis (the definition of apply_rweighting, main.rs:333-381)."""
+    W = O.radial_weights(n, 0.35, 0.05, prec="f64")
+    x = np.arange(n)
+    for k in (0, 1, 5, n // 4, n // 2 - 1):
+        row = np.cos(2 * np.pi * k * x / n + 0.3).astype(np.float32)
+        out = O.rweight(row[None, :], prec=prec)[0]
+        assert np.max(np.abs(out - W[k] * row)) < tol * max(1.0, n / 64)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rweight_linear_and_shift_equivariant(prec):
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(4, 80)).astype(np.float32)
+    b = rng.normal(size=(4, 80)).astype(np.float32)
+    fa, fb = O.rweight(a, prec=prec), O.rweight(b, prec=prec)
+    fab = O.rweight((a + 2 * b).astype(np.float32), prec=prec)
+    tol = 1e-5 if prec == "f32" else 1e-6
+    assert np.max(np.abs(fab - (fa + 2 * fb))) < tol * np.max(np.abs(fab))
+    sh = O.rweight(np.roll(a, 7, axis=1), prec=prec)
+    assert np.max(np.abs(sh - np.roll(fa, 7, axis=1))) < tol * np.max(np.abs(fa))
+    # constant rows vanish (DC weight 0)
+    assert np.max(np.abs(O.rweight(np.full((2, 80), 3.0, np.float32), prec=prec))) < 1e-5
+
+
+def test_rweight_matches_numpy_fft():
+    rng = np.random.default_rng(4)
+    rows = rng.normal(size=(6, 120)).astype(np.float32)
+    W = O.radial_weights(120, 0.35, 0.05, prec="f64")
+    ref = np.real(np.fft.ifft(np.fft.fft(rows.astype(np.float64), axis=1) * W, axis=1))
+    assert np.max(np.abs(O.rweight(rows, prec="f64") - ref)) < 1e-6
+    assert np.max(np.abs(O.rweight(rows, prec="f32") - ref)) < 2e-5
+
+
+# ----------------------------------------------------------- cosine stretch --
+
+@pytest.mark.parametrize("cubic", [False, True])
+def test_cosine_stretch_identity_and_limits(cubic):
+    rng = np.random.default_rng(5)
+    P = rng.normal(size=(4, 3, 50)).astype(np.float32)
+    ang = np.array([0.0, 85.0, -85.0, 40.0], np.float32)
+    S = O.cosine_stretch(P, ang, cubic=cubic)
+    assert np.array_equal(S[0], P[0])          # theta = 0: identity
+    assert np.array_equal(S[1:3], P[1:3])      # |theta| > 80: untouched
+    assert not np.array_equal(S[3], P[3])
+
+
+@pytest.mark.parametrize("cubic", [False, True])
+def test_cosine_stretch_linear_ramp(cubic):
+    n = 64
+    ramp = (0.25 * np.arange(n) + 1.0).astype(np.float32)
+    P = np.tile(ramp, (1, 2, 1)).astype(np.float32)
+    deg = 40.0
+    S = O.cosine_stretch(P, np.array([deg], np.float32), cubic=cubic, prec="f64")[0, 0]
+    s = 1.0 / np.cos(np.deg2rad(deg))
+    src = (np.arange(n) - n / 2) * s + n / 2
+    ok = (src >= 0) & (src <= n - 1)
+    # a linear ramp stays exactly linear where the source is inside the row ...
+    assert np.allclose(S[ok], 0.25 * src[ok] + 1.0, atol=1e-9)
+    # ... and is zero past the row (main.rs:481-483)
+    assert np.all(S[src >= n] == 0) and np.all(S[src < 0] == 0)
+
+
+# ---------------------------------------------------------- backprojection --
+
+def _tab(angles, xtilt=None, zf=None):
+    return O.proj_params(angles, xtilt, zf, prec="f32")
+
+
+def test_bp_bruteforce_fixture():
+    g = np.load(os.path.join(GOLDEN, "bp_bruteforce.npz"))
+    P, ang, means, V = g["P"], g["angles"], g["means"], g["V"]
+    nx, nz = int(g["out_nx"]), int(g["out_nz"])
+    o64 = O.backproject(P, O.proj_params(ang, prec="f32"), means, nx, nz, fill="mean", prec="f64")
+    o32 = O.backproject(P, _tab(ang), means, nx, nz, fill="mean", prec="f32")
+    rms = np.sqrt(np.mean(V ** 2))
+    # f64 oracle: geometry in f64 from the f32 angles, as the brute force
+    assert np.max(np.abs(o64 - V)) < 1e-6 * rms
+    assert np.max(np.abs(o32 - V)) < 1e-5 * rms
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_bp_constant_projections(prec):
+    """All projections c, Mean fill: the volume is c (weights sum to 1, fill = c)."""
+    P = np.full((9, 4, 40), 2.5, np.float32)
+    ang = np.linspace(-60, 60, 9).astype(np.float32)
+    V = O.backproject(P, _tab(ang), np.full(9, 2.5, np.float32), 56, 30, prec=prec)
+    assert np.max(np.abs(V - 2.5)) < 1e-5
+
+
+def test_bp_single_view_identity():
+    """One view at 0 deg, out_nx = in_nx: V[iy][oz][ox] = P[iy][ox] for
+    ox <= n-2 and the view mean at ox = n-1 (floor(px)+1 = n is out of range)."""
+    rng = np.random.default_rng(6)
+    P = rng.normal(size=(1, 5, 16)).astype(np.float32)
+    m = np.float32(0.123)
+    V = O.backproject(P, _tab(np.zeros(1, np.float32)), np.array([m]), 16, 3)
+    for oz in range(3):
+        assert np.array_equal(V[:, oz, :15], P[0, :, :15])
+        assert np.all(V[:, oz, 15] == m)
+
+
+def test_bp_linear_in_projections():
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(7, 3, 30)).astype(np.float32)
+    b = rng.normal(size=(7, 3, 30)).astype(np.float32)
+    ang = np.linspace(-45, 45, 7).astype(np.float32)
+    z = np.zeros(7, np.float32)
+    A = O.backproject(a, _tab(ang), z, 30, 12, fill="zero", prec="f64")
+    B = O.backproject(b, _tab(ang), z, 30, 12, fill="zero", prec="f64")
+    AB = O.backproject((a + 3 * b).astype(np.float32), _tab(ang), z, 30, 12, fill="zero", prec="f64")
+    assert np.max(np.abs(AB - (A + 3 * B))) < 1e-5
+
+
+def test_bp_f32_f64_agree():
+    rng = np.random.default_rng(8)
+    P = rng.normal(size=(11, 4, 48)).astype(np.float32)
+    ang = np.linspace(-60, 60, 11).astype(np.float32)
+    means = P.reshape(11, -1).mean(axis=1).astype(np.float32)
+    a = O.backproject(P, _tab(ang), means, 48, 20, prec="f32")
+    b = O.backproject(P, _tab(ang), means, 48, 20, prec="f64")
+    assert np.sqrt(np.mean((a - b) ** 2)) < 1e-5 * np.sqrt(np.mean(b ** 2))
+
+
+# ---------------------------------------------------- forward projection --
+
+def test_fp_is_adjoint_of_zero_fill_bp():
+    """<FP(V), S> = nz * <V, BP_zero-fill(S)> -- forward_project (main.rs:831-896)
+    is the transpose of the backprojection interpolation."""
+    rng = np.random.default_rng(9)
+    nz, ny, nx, onx, onz = 7, 3, 26, 22, 10
+    ang = np.linspace(-50, 50, nz).astype(np.float32)
+    V = rng.normal(size=(ny, onz, onx)).astype(np.float32)
+    S = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    tab = _tab(ang)
+    R = O.forward_project(V, tab, nx, nz, prec="f64")
+    B = O.backproject(S, tab, np.zeros(nz, np.float32), onx, onz, fill="zero", prec="f64")
+    lhs = float(np.sum(R.astype(np.float64) * S))
+    rhs = nz * float(np.sum(V.astype(np.float64) * B))
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+
+
+def test_fp_conserves_mass():
+    nz, ny, nx = 5, 1, 40
+    V = np.zeros((ny, 8, 30), np.float32)
+    V[0, 4, 15] = 2.0
+    ang = np.linspace(-30, 30, nz).astype(np.float32)
+    R = O.forward_project(V, _tab(ang), nx, nz, prec="f64")
+    assert np.allclose(R.reshape(nz, -1).sum(axis=1), 2.0, atol=1e-6)
+
+
+# --------------------------------------------------------- local alignment --
+
+def test_local_grid_bilinear_exact():
+    """interpolate (main.rs:172-198) is bilinear on the grid cell and clamps
+    outside; a bilinear field is reproduced exactly inside."""
+    xs, ys = [0.0, 10.0, 30.0], [0.0, 20.0]
+    f = lambda x, y: 0.5 * x - 0.25 * y + 0.01 * x * y + 1.0
+    patches = [(0, x, y, f(x, y), -f(x, y)) for x in xs for y in ys]
+    g = O.build_local_grids(patches, 2)
+    for (x, y) in [(5.0, 5.0), (12.5, 7.0), (29.0, 19.0)]:
+        dx, dy = O.local_interp(g, 0, x, y, prec="f64")
+        assert dx == pytest.approx(f(x, y), abs=1e-5) and dy == pytest.approx(-f(x, y), abs=1e-5)
+    assert O.local_interp(g, 0, -5.0, -5.0, prec="f64")[0] == pytest.approx(f(0, 0))
+    assert O.local_interp(g, 0, 50.0, 50.0, prec="f64")[0] == pytest.approx(f(30, 20))
+    assert O.local_interp(g, 1, 3.0, 3.0) == (0.0, 0.0)  # no patches
+
+
+def test_local_grid_dedup():
+    # positions within 0.01 of the last kept one merge (Vec::dedup_by, main.rs:281-283)
+    g = O.build_local_grids([(0, 1.0, 0.0, 1, 1), (0, 1.005, 0.0, 2, 2), (0, 2.0, 0.0, 3, 3)], 1)
+    assert list(g.xs[0]) == [1.0, 2.0]
+
+
+# ------------------------------------------------------------------- SIRT --
+
+def test_sirt_relax_zero_is_wbp():
+    rng = np.random.default_rng(10)
+    P = rng.normal(size=(9, 3, 32)).astype(np.float32)
+    ang = np.linspace(-40, 40, 9).astype(np.float32)
+    w, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=16))
+    s, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=16, sirt=2, sirt_relax=0.0))
+    assert np.array_equal(w, s)
+
+
+def test_sirt_one_iteration_by_composition():
+    """V1 = V0 + relax * BP(RW(O - FP(V0))) with the residual unstretched and
+    the fill taken from the original means (main.rs:919-953)."""
+    rng = np.random.default_rng(11)
+    P = rng.normal(size=(7, 2, 24)).astype(np.float32)
+    ang = np.linspace(-30, 30, 7).astype(np.float32)
+    means = np.array([O.min_max_mean(P[i])[2] for i in range(7)], np.float32)
+    tab = _tab(ang)
+    Pf = O.cosine_stretch(O.rweight(P.reshape(-1, 24)).reshape(P.shape), ang)
+    V0 = O.backproject(Pf, tab, means, 24, 10)
+    D = (P - O.forward_project(V0, tab, 24, 7)).astype(np.float32)
+    Df = O.rweight(D.reshape(-1, 24)).reshape(D.shape)
+    V1 = V0 + np.float32(0.7) * O.backproject(Df, tab, means, 24, 10)
+    got, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=10, sirt=1, sirt_relax=0.7))
+    assert np.max(np.abs(got - V1)) < 1e-5 * np.sqrt(np.mean(V1 ** 2))
+
+
+def test_fake_sirt_constant_volume_unchanged():
+    P = np.full((5, 4, 20), 1.5, np.float32)
+    ang = np.linspace(-30, 30, 5).astype(np.float32)
+    v, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=8, no_weight=True, no_cosine_stretch=True,
+                                            fake_sirt=3))
+    assert np.max(np.abs(v - 1.5)) < 1e-5
+
+
+def test_output_transform_and_header_stats():
+    rng = np.random.default_rng(12)
+    P = rng.normal(size=(5, 3, 20)).astype(np.float32)
+    ang = np.linspace(-30, 30, 5).astype(np.float32)
+    v0, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=8))
+    v1, (mn, mx, mean) = O.tilt_run(P, ang, O.TiltOptions(thickness=8, densweight=2.0,
+                                                          outscale=0.5, outadd=3.0))
+    assert np.array_equal(v1, (v0 * np.float32(2.0)) * np.float32(0.5) + np.float32(3.0))
+    assert mn == v1.min() and mx == v1.max()
+    assert mean == pytest.approx(float(v1.astype(np.float64).mean()), rel=1e-6)
