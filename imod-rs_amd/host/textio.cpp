@@ -1,6 +1,7 @@
 // Text inputs of `tilt`: .tlt angles and per-view files.
 //   read_tilt_file      crates/imod-transforms/src/tilt.rs:6-23
 //   load_per_view_file  apps/tilt/src/main.rs:310-326
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -9,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <vector>
 
 #include "../../include/tiltgpu.h"
 #include "textio.h"
@@ -109,4 +111,120 @@ extern "C" int tiltgpu_read_per_view_file(const char* path, float* out, int32_t 
         ++n;
     }
     return n;
+}
+
+// ------------------------------------------------------------ local alignment
+// load_local_alignment, apps/tilt/src/main.rs:232-307: lines "view x y dx dy"
+// ('#' and blank lines skipped, short lines skipped, each field unwrap_or(0)),
+// grouped by view < nz into grids of sorted unique xs / ys (a position within
+// 0.01 of the last kept one merges), dx/dy row-major [iy*nx + ix].
+namespace tiltgpu_host {
+
+struct LocalPatch {
+    long long view;
+    float x, y, dx, dy;
+};
+
+static std::vector<float> sorted_unique(std::vector<float> v) {
+    std::sort(v.begin(), v.end());
+    std::vector<float> out;
+    for (float a : v)
+        if (out.empty() || !(std::fabs(a - out.back()) < 0.01f)) out.push_back(a);
+    return out;
+}
+
+bool load_local_grids(const std::string& path, int nz, LocalGrids* g, std::string* err) {
+    std::ifstream f(path);
+    if (!f) {
+        *err = "cannot open " + path;
+        return false;
+    }
+    std::vector<LocalPatch> patches;
+    std::string line;
+    while (std::getline(f, line)) {
+        const std::string t = trim(line);
+        if (t.empty() || t[0] == '#') continue;
+        std::vector<std::string> parts;
+        size_t i = 0;
+        while (i < t.size()) {
+            while (i < t.size() && std::isspace(static_cast<unsigned char>(t[i]))) ++i;
+            size_t j = i;
+            while (j < t.size() && !std::isspace(static_cast<unsigned char>(t[j]))) ++j;
+            if (j > i) parts.push_back(t.substr(i, j - i));
+            i = j;
+        }
+        if (parts.size() < 5) continue;
+        LocalPatch p{};
+        if (!parse_usize(parts[0], &p.view)) p.view = 0;
+        float* dst[4] = {&p.x, &p.y, &p.dx, &p.dy};
+        for (int k = 0; k < 4; ++k)
+            if (!parse_f32(parts[k + 1], dst[k])) *dst[k] = 0.f;
+        patches.push_back(p);
+    }
+    g->nx.assign(nz, 0);
+    g->ny.assign(nz, 0);
+    g->xs.clear();
+    g->ys.clear();
+    g->dx.clear();
+    g->dy.clear();
+    std::vector<std::vector<const LocalPatch*>> by_view(nz);
+    for (const auto& p : patches)
+        if (p.view >= 0 && p.view < nz) by_view[p.view].push_back(&p);
+    for (int v = 0; v < nz; ++v) {
+        if (by_view[v].empty()) continue;
+        std::vector<float> xs, ys;
+        for (auto* p : by_view[v]) {
+            xs.push_back(p->x);
+            ys.push_back(p->y);
+        }
+        xs = sorted_unique(xs);
+        ys = sorted_unique(ys);
+        const int nx = static_cast<int>(xs.size()), ny = static_cast<int>(ys.size());
+        std::vector<float> gdx(static_cast<size_t>(nx) * ny, 0.f), gdy(gdx.size(), 0.f);
+        for (auto* p : by_view[v]) {
+            int ix = 0, iy = 0;
+            for (int k = 0; k < nx; ++k)
+                if (std::fabs(xs[k] - p->x) < 0.01f) {
+                    ix = k;
+                    break;
+                }
+            for (int k = 0; k < ny; ++k)
+                if (std::fabs(ys[k] - p->y) < 0.01f) {
+                    iy = k;
+                    break;
+                }
+            gdx[static_cast<size_t>(iy) * nx + ix] = p->dx;
+            gdy[static_cast<size_t>(iy) * nx + ix] = p->dy;
+        }
+        g->nx[v] = nx;
+        g->ny[v] = ny;
+        g->xs.insert(g->xs.end(), xs.begin(), xs.end());
+        g->ys.insert(g->ys.end(), ys.begin(), ys.end());
+        g->dx.insert(g->dx.end(), gdx.begin(), gdx.end());
+        g->dy.insert(g->dy.end(), gdy.begin(), gdy.end());
+    }
+    return true;
+}
+
+}  // namespace tiltgpu_host
+
+extern "C" int tiltgpu_parse_local_file(const char* path, int32_t nz, int32_t* sizes, int32_t* nx,
+                                        int32_t* ny, float* xs, float* ys, float* dx, float* dy) {
+    if (!path || nz < 0 || !sizes) return TILTGPU_ERR_ARG;
+    tiltgpu_host::LocalGrids g;
+    std::string err;
+    if (!tiltgpu_host::load_local_grids(path, nz, &g, &err)) return TILTGPU_ERR_IO;
+    sizes[0] = static_cast<int32_t>(g.xs.size());
+    sizes[1] = static_cast<int32_t>(g.ys.size());
+    sizes[2] = static_cast<int32_t>(g.dx.size());
+    if (!nx) return TILTGPU_OK;
+    if (!ny || (sizes[0] && !xs) || (sizes[1] && !ys) || (sizes[2] && (!dx || !dy)))
+        return TILTGPU_ERR_ARG;
+    std::copy(g.nx.begin(), g.nx.end(), nx);
+    std::copy(g.ny.begin(), g.ny.end(), ny);
+    std::copy(g.xs.begin(), g.xs.end(), xs);
+    std::copy(g.ys.begin(), g.ys.end(), ys);
+    std::copy(g.dx.begin(), g.dx.end(), dx);
+    std::copy(g.dy.begin(), g.dy.end(), dy);
+    return TILTGPU_OK;
 }

@@ -71,6 +71,15 @@ class Phantom(C.Structure):
                 ("noise_rel", C.c_float), ("seed", C.c_uint64)]
 
 
+class MrcInfo(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("mode", C.c_int32),
+                ("mx", C.c_int32), ("my", C.c_int32), ("mz", C.c_int32), ("next", C.c_int32),
+                ("nlabl", C.c_int32), ("swapped", C.c_int32), ("is_imod", C.c_int32),
+                ("xlen", C.c_float), ("ylen", C.c_float), ("zlen", C.c_float),
+                ("amin", C.c_float), ("amax", C.c_float), ("amean", C.c_float),
+                ("label0", C.c_char * 81)]
+
+
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -109,6 +118,12 @@ SIGNATURES = [
     ("tiltgpu_reconstruct_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
+    ("tiltgpu_parse_local_file", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _fp, _fp,
+                                           _fp, _fp]),
+    ("tiltgpu_mrc_info_read", C.c_int, [C.c_char_p, C.POINTER(MrcInfo)]),
+    ("tiltgpu_mrc_read_sections", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _fp, C.c_int64]),
+    ("tiltgpu_mrc_write_float", C.c_int, [C.c_char_p, _fp, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_float, C.c_float, C.c_float, C.c_char_p]),
     ("tiltgpu_synth_series", C.c_int, [C.POINTER(Phantom), C.c_int32, C.c_int32, C.c_int32,
                                        _fp, _fp, _fp, _fp, C.c_int32, _fp]),
 ]
@@ -402,3 +417,40 @@ def synth_series(nx, ny, angles, vol_nz, n_ellipsoids=200, n_beads=30, seed=0, n
     _check(lib().tiltgpu_synth_series(C.byref(ph), nx, ny, ang.size, _f(ang), _f(xt), _f(sx),
                                       _f(sy), nthreads, _f(out)))
     return out
+
+
+def parse_local_file(path, nz):
+    """Per-view local-alignment grids (nx, ny, xs, ys, dx, dy) of a --local-file."""
+    sizes = np.zeros(3, np.int32)
+    _check(lib().tiltgpu_parse_local_file(path.encode(), nz, sizes.ctypes.data_as(_ip), None,
+                                          None, None, None, None, None))
+    nx = np.zeros(nz, np.int32)
+    ny = np.zeros(nz, np.int32)
+    xs, ys = np.zeros(max(1, sizes[0]), np.float32), np.zeros(max(1, sizes[1]), np.float32)
+    dx, dy = np.zeros(max(1, sizes[2]), np.float32), np.zeros(max(1, sizes[2]), np.float32)
+    _check(lib().tiltgpu_parse_local_file(path.encode(), nz, sizes.ctypes.data_as(_ip),
+                                          nx.ctypes.data_as(_ip), ny.ctypes.data_as(_ip),
+                                          _f(xs), _f(ys), _f(dx), _f(dy)))
+    return nx, ny, xs[:sizes[0]], ys[:sizes[1]], dx[:sizes[2]], dy[:sizes[2]]
+
+
+def mrc_info(path):
+    info = MrcInfo()
+    _check(lib().tiltgpu_mrc_info_read(path.encode(), C.byref(info)))
+    return info
+
+
+def mrc_read(path):
+    """All sections of an MRC file as float32 [nz][ny][nx] (modes with one value
+    per pixel)."""
+    info = mrc_info(path)
+    out = np.empty((info.nz, info.ny, info.nx), np.float32)
+    _check(lib().tiltgpu_mrc_read_sections(path.encode(), 0, info.nz, _f(out), out.size))
+    return out
+
+
+def mrc_write(path, vol, xlen=0.0, ylen=0.0, zlen=0.0, label=None):
+    v = _f32(vol)
+    nz, ny, nx = v.shape
+    _check(lib().tiltgpu_mrc_write_float(path.encode(), _f(v), nx, ny, nz, xlen, ylen, zlen,
+                                         None if label is None else label.encode()))

@@ -189,6 +189,31 @@ int tiltgpu_read_tilt_file(const char* path, float* out, int32_t cap);
 /* load_per_view_file (main.rs:310-326); a parse error is TILTGPU_ERR_IO. */
 int tiltgpu_read_per_view_file(const char* path, float* out, int32_t cap);
 
+/* --local-file (load_local_alignment, main.rs:232-307) parsed into per-view
+ * grids for tiltgpu_set_local_alignment.  sizes[3] receives the total lengths of
+ * xs, ys and dx/dy; with nx == NULL only sizes are written.  nx, ny: nz ints. */
+int tiltgpu_parse_local_file(const char* path, int32_t nz, int32_t* sizes, int32_t* nx,
+                             int32_t* ny, float* xs, float* ys, float* dx, float* dy);
+
+/* MRC 2014 files: the subset of crates/imod-mrc used by `tilt`. */
+typedef struct {
+    int32_t nx, ny, nz, mode, mx, my, mz, next, nlabl;
+    int32_t swapped;  /* big-endian file (read.rs:36-52)   */
+    int32_t is_imod;  /* IMOD stamp present (header.rs:137) */
+    float xlen, ylen, zlen, amin, amax, amean;
+    char label0[81];
+} tiltgpu_mrc_info;
+/* MrcReader::open (crates/imod-mrc/src/read.rs:28-93) header fields. */
+int tiltgpu_mrc_info_read(const char* path, tiltgpu_mrc_info* out);
+/* read_slice_f32 (read.rs:127-231) of sections [z0, z1) into out (cap floats). */
+int tiltgpu_mrc_read_sections(const char* path, int32_t z0, int32_t z1, float* out,
+                              int64_t cap);
+/* MrcHeader::new + add_label + MrcWriter create / write_slice_f32 / finish
+ * (header.rs:181-258, write.rs:20-153) of an nx x ny x nz float volume, with the
+ * header statistics computed as main.rs:1080-1093 does. */
+int tiltgpu_mrc_write_float(const char* path, const float* data, int32_t nx, int32_t ny,
+                            int32_t nz, float xlen, float ylen, float zlen, const char* label);
+
 /* Synthetic tilt series: the seeded ellipsoid + bead phantom projected
  * analytically with the tilt geometry (per-view angle, x-axis tilt, in-plane
  * shift), plus Gaussian noise of sigma = noise_rel * std of the noiseless

@@ -1,0 +1,154 @@
+"""MRC I/O, local-alignment file parsing and the `tilt` command line.
+
+The MRC cases restate crates/imod-mrc/src/tests.rs (roundtrip_header :13-36,
+roundtrip_float_file :68-89, read_big_endian_float_file :113-149,
+read_big_endian_short_file :151-181, roundtrip_byte_file :39-66) with files
+written here; the big-endian files are built byte by byte as those tests do."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tiltgpu as tg
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TILT = os.path.join(REPO, "imod-rs_amd", "_build", "tilt")
+
+
+def _header_bytes(nx, ny, nz, mode, big=False, label=b""):
+    """A 1024-byte MRC2014 header in the layout of crates/imod-mrc/src/header.rs."""
+    e = ">" if big else "<"
+    h = bytearray(1024)
+    struct.pack_into(e + "10i", h, 0, nx, ny, nz, mode, 0, 0, 0, nx, ny, nz)
+    struct.pack_into(e + "6f", h, 40, nx, ny, nz, 90, 90, 90)
+    struct.pack_into(e + "3i", h, 64, 1, 2, 3)
+    struct.pack_into(e + "i", h, 108, 20140)
+    h[208:212] = b"MAP "
+    h[212:216] = b"\x11\x11\x00\x00" if big else b"\x44\x44\x00\x00"
+    if label:
+        struct.pack_into(e + "i", h, 220, 1)
+        h[224:224 + len(label)] = label
+    return bytes(h)
+
+
+def test_roundtrip_header_and_float_file(tmp_path):
+    p = str(tmp_path / "a.mrc")
+    vol = np.arange(2 * 3 * 4, dtype=np.float32).reshape(2, 3, 4) * 0.5 - 1.5
+    tg.mrc_write(p, vol, label="Test label written by imod-mrc")
+    assert os.path.getsize(p) == 1024 + vol.nbytes  # header is 1024 bytes
+    info = tg.mrc_info(p)
+    assert (info.nx, info.ny, info.nz, info.mode) == (4, 3, 2, 2)
+    assert info.is_imod == 1 and info.swapped == 0
+    assert info.label0.decode() == "Test label written by imod-mrc"
+    assert info.amin == -1.5 and info.amax == vol.max()
+    assert info.amean == pytest.approx(float(vol.mean()), abs=1e-6)
+    assert np.array_equal(tg.mrc_read(p), vol)
+
+
+def test_read_big_endian_float_file(tmp_path):
+    p = tmp_path / "be.mrc"
+    data = np.arange(6 * 5 * 2, dtype=np.float32).reshape(2, 5, 6) / 7.0
+    p.write_bytes(_header_bytes(6, 5, 2, 2, big=True) + data.astype(">f4").tobytes())
+    info = tg.mrc_info(str(p))
+    assert info.swapped == 1 and (info.nx, info.ny, info.nz) == (6, 5, 2)
+    assert np.max(np.abs(tg.mrc_read(str(p)) - data)) < 1e-6
+
+
+def test_read_big_endian_short_and_byte_files(tmp_path):
+    s = (np.arange(20, dtype=np.int16) - 7).reshape(1, 4, 5)
+    p = tmp_path / "s.mrc"
+    p.write_bytes(_header_bytes(5, 4, 1, 1, big=True) + s.astype(">i2").tobytes())
+    assert np.array_equal(tg.mrc_read(str(p)), s.astype(np.float32))
+    b = np.arange(12, dtype=np.uint8).reshape(2, 2, 3) * 20
+    q = tmp_path / "b.mrc"
+    q.write_bytes(_header_bytes(3, 2, 2, 0) + b.tobytes())
+    assert np.array_equal(tg.mrc_read(str(q)), b.astype(np.float32))
+    h = np.array([0.5, -2.0, 65504.0, 6e-8], np.float16).reshape(1, 1, 4)
+    r = tmp_path / "h.mrc"
+    r.write_bytes(_header_bytes(4, 1, 1, 12) + h.tobytes())
+    assert np.array_equal(tg.mrc_read(str(r)), h.astype(np.float32))
+
+
+def test_mrc_errors(tmp_path):
+    p = tmp_path / "short.mrc"
+    p.write_bytes(b"\0" * 100)
+    with pytest.raises(tg.TiltGpuError):
+        tg.mrc_info(str(p))
+    with pytest.raises(tg.TiltGpuError):
+        tg.mrc_info(str(tmp_path / "missing.mrc"))
+
+
+def test_local_file_parse_matches_oracle(tmp_path):
+    p = tmp_path / "local.txt"
+    p.write_text("# view x y dx dy\n"
+                 "0 10 5 1.5 -0.5\n0 30 5 2.0 0.25\n0 10 25 -1 1\n0 30 25 0 0\n"
+                 "1 20.004 5 3 3\n1 20 5 4 4\n"  # merges within 0.01; later patch wins
+                 "2 1 1\n"                       # short line: skipped
+                 "x 7 7 0.5 0.5\n"               # bad view -> 0
+                 "9 1 1 1 1\n")                  # view >= nz: dropped
+    got = tg.parse_local_file(str(p), 4)
+    ref = O.build_local_grids(O.parse_local_file(str(p)), 4).flat()
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    for a, b in zip(got[2:], ref[2:]):
+        assert np.array_equal(a, b[: len(a)])
+
+
+def test_cli_usage_errors():
+    r = subprocess.run([TILT], capture_output=True, text=True)
+    assert r.returncode == 2 and "required arguments" in r.stderr
+    r = subprocess.run([TILT, "-i", "a", "-o", "b", "-t", "c", "--bogus"], capture_output=True,
+                       text=True)
+    assert r.returncode == 2 and "unexpected argument" in r.stderr
+
+
+def test_cli_missing_files_exit_1(tmp_path):
+    t = tmp_path / "a.tlt"
+    r = subprocess.run([TILT, "-i", "a.mrc", "-o", "b.mrc", "-t", str(t)], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "tilt file" in r.stderr
+    t.write_text("0\n")
+    r = subprocess.run([TILT, "-i", str(tmp_path / "nope.mrc"), "-o", "b.mrc", "-t", str(t)],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "Error" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
+    """`tilt` with --xtilt-file, --local-file and a few options against the oracle
+    on the same MRC input; header statistics included."""
+    ang = np.linspace(-60, 60, 25).astype(np.float32)
+    rng = np.random.default_rng(3)
+    sx, sy = rng.normal(0, 1.5, 25).astype(np.float32), rng.normal(0, 1.5, 25).astype(np.float32)
+    P = gpu.synth_series(96, 14, ang, vol_nz=40, n_ellipsoids=25, n_beads=4, seed=3,
+                         xtilt=np.full(25, 0.5, np.float32), shift_x=sx, shift_y=sy,
+                         axis=(2.0, 9.0), bead_radius=2.0)
+    inp, out = str(tmp_path / "in.mrc"), str(tmp_path / "out.rec")
+    gpu.mrc_write(inp, P)
+    (tmp_path / "a.tlt").write_text("".join(f"{a:10.2f}\n" for a in ang))
+    (tmp_path / "xt").write_text("0.5\n" * 25)
+    (tmp_path / "loc").write_text("".join(f"{v} 48 7 {sx[v]} {sy[v]}\n" for v in range(25)))
+    cmd = [TILT, "-i", inp, "-o", out, "-t", str(tmp_path / "a.tlt"), "-z", "40",
+           "--xtilt-file", str(tmp_path / "xt"), "--local-file", str(tmp_path / "loc"),
+           "--densweight", "2", "--outadd=1", "--slab-rows", "5"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    vol = gpu.mrc_read(out)
+    info = gpu.mrc_info(out)
+    angs = O.read_tilt_file(str(tmp_path / "a.tlt"))
+    grids = O.build_local_grids(O.parse_local_file(str(tmp_path / "loc")), 25)
+    ref, (mn, mx, mean) = O.tilt_run(P, angs, O.TiltOptions(thickness=40, densweight=2.0,
+                                                            outadd=1.0),
+                                     xtilt=O.load_per_view_file(str(tmp_path / "xt")),
+                                     local=grids)
+    # file sections are Y rows, each [z][x] (main.rs:710-716)
+    assert (info.nx, info.ny, info.nz) == (96, 40, 14)
+    assert (info.mx, info.my, info.mz) == (96, 40, 14)
+    assert info.label0.decode() == "tilt: 25 projections, thickness 40"
+    assert rel(vol, ref) <= 1e-4
+    scale = float(np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
+    assert abs(info.amin - mn) <= 1e-4 * scale and abs(info.amax - mx) <= 1e-4 * scale
+    assert abs(info.amean - mean) <= 1e-4 * scale
