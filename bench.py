@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -130,6 +132,15 @@ def main():
     if tg.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.pmc_probe:
+        copy_bytes = 2 << 30
+        ms = tg.copy_calibration(copy_bytes, 1)
+        print(json.dumps({"copy_bytes": copy_bytes, "copy_ms": ms,
+                          "copy_gbs": 2 * copy_bytes / ms / 1e6}), flush=True)
+        P, ang, xt, local = c1_inputs(tg)
+        dt, tm = run_gpu(tg, P, ang, xt, local, "f32", 1, 0)
+        print(json.dumps({"step_s": dt, **tm}), flush=True)
+        return
     t0 = time.perf_counter()
     P, ang, xt, local = c1_inputs(tg)
     print(f"# inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)

@@ -446,3 +446,24 @@ hipError_t launch_slice_stats(const float* v, int rows, int64_t per_row, float* 
 }
 
 }  // namespace tiltgpu
+
+namespace tiltgpu {
+
+// Plain streaming copy (16 B per lane per iteration): the known-size reference
+// against which HBM byte counters are calibrated.
+__global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ src,
+                                                   float4* __restrict__ dst, int64_t n4) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n4;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
+    const int64_t n4 = bytes / 16;
+    if (n4 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st,
+                       static_cast<const float4*>(src), static_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
+}  // namespace tiltgpu

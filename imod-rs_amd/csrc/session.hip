@@ -1015,4 +1015,40 @@ int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out) {
     return tiltgpu_reconstruct_rows(s, iy, iy + 1, out);
 }
 
+int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy) {
+    if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bytes = bytes / 16 * 16;
+    int rc = TILTGPU_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(TILTGPU_ERR_HIP, "copy calibration: allocation failed");
+    } else {
+        float ms = 0.f;
+        if (launch_fill_f32(static_cast<float*>(a), bytes / 4, 1.0f, st) != hipSuccess ||
+            hipEventRecord(e0, st) != hipSuccess)
+            rc = fail(TILTGPU_ERR_HIP, "copy calibration: launch failed");
+        for (int i = 0; rc == TILTGPU_OK && i < iters; ++i)
+            if (launch_copy(a, b, bytes, st) != hipSuccess)
+                rc = fail(TILTGPU_ERR_HIP, "copy calibration: launch failed");
+        if (rc == TILTGPU_OK && (hipEventRecord(e1, st) != hipSuccess ||
+                                 hipStreamSynchronize(st) != hipSuccess ||
+                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = fail(TILTGPU_ERR_HIP, "copy calibration: timing failed");
+        *ms_per_copy = static_cast<double>(ms) / iters;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 }  // extern "C"

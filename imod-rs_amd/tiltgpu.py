@@ -116,6 +116,7 @@ SIGNATURES = [
                                          C.c_int32, C.c_int32, C.POINTER(_vp)]),
     ("tiltgpu_reconstruct_row", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_reconstruct_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_copy_calibration", C.c_int, [C.c_int64, C.c_int32, _dp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_parse_local_file", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _fp, _fp,
@@ -454,3 +455,10 @@ def mrc_write(path, vol, xlen=0.0, ylen=0.0, zlen=0.0, label=None):
     nz, ny, nx = v.shape
     _check(lib().tiltgpu_mrc_write_float(path.encode(), _f(v), nx, ny, nz, xlen, ylen, zlen,
                                          None if label is None else label.encode()))
+
+
+def copy_calibration(nbytes, iters=1):
+    """ms per launch of the library's plain device copy kernel (diagnostic)."""
+    ms = C.c_double()
+    _check(lib().tiltgpu_copy_calibration(int(nbytes), int(iters), C.byref(ms)))
+    return ms.value

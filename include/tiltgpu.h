@@ -181,6 +181,12 @@ int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out);
 /* The same for rows [y0, y1) in one launch. */
 int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out);
 
+/* Diagnostic (no counterpart in the reference): `iters` launches of a plain
+ * device-to-device copy kernel of `bytes` bytes; average ms per copy.  The known
+ * byte count calibrates hardware traffic counters and gives the attainable
+ * HBM bandwidth of the device. */
+int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy);
+
 /* -------------------------------------------------------- host utilities */
 
 /* read_tilt_file (crates/imod-transforms/src/tilt.rs:6-23).  Writes at most cap
