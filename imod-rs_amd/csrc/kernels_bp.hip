@@ -358,106 +358,158 @@ __device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int 
     }
 }
 
-template <int ZPT, bool TWO, bool LOCAL, int ST>
-__global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P) {
+// The kernel: one tile of one series per workgroup (BATCH = false, the
+// workgroup id picks the tile), or a persistent loop over the tiles of several
+// series (BATCH = true, continuous batching).  A tile is Y row iy, 64 x positions
+// and 4*ZPT z positions; its result depends only on (series, tile), so the same
+// tile computed alone or inside a batched launch is bit-identical.
+//
+// Batched tile claiming: the tiles are split into 8 contiguous chunks, one per
+// XCD; a workgroup drains its own XCD's chunk (L2 locality: consecutive tiles
+// share projection rows), then steals from the others.  Every workgroup visits
+// every chunk once and exits when all are drained.
+template <int ZPT, bool TWO, bool LOCAL, int ST, bool BATCH>
+__global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P1, BpBatch B) {
     extern __shared__ float4 lds4[];
+    __shared__ int s_tile;
     constexpr int TZ = 4 * ZPT;
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int logical = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = logical % P.ntx;
-    const int rest = logical / P.ntx;
-    const int tz = rest % P.ntz;
-    const int iy = P.y0 + rest / P.ntz;
-    const int ox = tx * kBpTileX + lane;
-    const int zt0 = tz * TZ;
-    const int zw0 = zt0 + wave * ZPT;
-    const float dx = static_cast<float>(ox) - P.ocx;
-
-    float acc[ZPT];
-#pragma unroll
-    for (int k = 0; k < ZPT; ++k) acc[k] = 0.f;
-
-    if (P.nviews > 0) {
-        unsigned raw[kSlots][4];
-        Vs cur = view_setup<TWO, LOCAL>(P, 0, dx, ox, iy, zt0, TZ);
-        stage_load<TWO, ST>(P, cur, 0, iy, raw);
-        for (int p = 0; p < P.nviews; ++p) {
-            if (cur.total > P.lds_cap) {
-                // host bound violated: report it (the host fails the call) and skip
-                // the view rather than overrun LDS
-                if (tid == 0) atomicOr(P.err, 1);
-                cur.total = 0;
-                cur.NR = 0;
-                cur.L = 0;
-                cur.row = -1;
-            }
-            // buffer (p & 1) was last read by the compute of view p-2; every thread
-            // has passed the barrier of view p-1 since, so it is free.
-            const int buf = (p & 1) * P.lds_cap;
-            stage_store<TWO, ST>(P, cur, p, iy, raw, lds4, buf);
+    int chunk_i = 0, lo = 0, hi = 0;
+    if constexpr (BATCH) {
+        lo = (blockIdx.x % 8) * B.chunk;
+        hi = min(B.total, lo + B.chunk);
+    }
+    for (;;) {
+        BpParams P;
+        int logical;
+        if constexpr (!BATCH) {
+            P = P1;
+            logical = xcd_remap(blockIdx.x, gridDim.x);
+        } else {
+            if (threadIdx.x == 0) s_tile = lo + atomicAdd(B.counters + (lo / max(B.chunk, 1)), 1);
             __syncthreads();
-            Vs nxt = cur;
-            if (p + 1 < P.nviews) {
-                nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
-                stage_load<TWO, ST>(P, nxt, p + 1, iy, raw);
+            const int t = s_tile;
+            __syncthreads();
+            if (t >= hi) {
+                if (++chunk_i == 8) break;
+                lo = ((blockIdx.x + chunk_i) % 8) * B.chunk;
+                hi = min(B.total, lo + B.chunk);
+                continue;
             }
-            const TILTGPU_CONSTANT float* zc =
-                cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
-            if constexpr (TWO) {
-                if (cur.row >= 0) {
-                    const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
-                    const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
-                    if (LOCAL && cur.shift) {
+            // series of tile t: last r with tile_start[r] <= t
+            int ra = 0, rb = B.nreq - 1;
+            while (ra < rb) {
+                const int m = (ra + rb + 1) / 2;
+                if (cptr(B.tile_start)[m] <= t)
+                    ra = m;
+                else
+                    rb = m - 1;
+            }
+            // the descriptor is workgroup-uniform: copy it through the scalar cache
+            const TILTGPU_CONSTANT unsigned* src =
+                reinterpret_cast<const TILTGPU_CONSTANT unsigned*>(cptr(B.descs) + ra);
+            unsigned* dst = reinterpret_cast<unsigned*>(&P);
 #pragma unroll
-                        for (int k = 0; k < ZPT; ++k) {
-                            const float px = ((a + zc[k]) + P.cx) + ldx;
-                            const float4 q = E[cvt_floor(px)];
-                            const float f = __builtin_amdgcn_fractf(px);
-                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < ZPT; ++k) {
-                            const float px = (a + zc[k]) + P.cx;
-                            const float4 q = E[cvt_floor(px)];
-                            const float f = __builtin_amdgcn_fractf(px);
-                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+            for (int i = 0; i < static_cast<int>(sizeof(BpParams) / 4); ++i) dst[i] = src[i];
+            logical = t - cptr(B.tile_start)[ra];
+        }
+        const int tid = threadIdx.x;
+        const int lane = tid & (kWave - 1);
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int tx = logical % P.ntx;
+        const int rest = logical / P.ntx;
+        const int tz = rest % P.ntz;
+        const int iy = P.y0 + rest / P.ntz;
+        const int ox = tx * kBpTileX + lane;
+        const int zt0 = tz * TZ;
+        const int zw0 = zt0 + wave * ZPT;
+        const float dx = static_cast<float>(ox) - P.ocx;
+
+        float acc[ZPT];
+    #pragma unroll
+        for (int k = 0; k < ZPT; ++k) acc[k] = 0.f;
+
+        if (P.nviews > 0) {
+            unsigned raw[kSlots][4];
+            Vs cur = view_setup<TWO, LOCAL>(P, 0, dx, ox, iy, zt0, TZ);
+            stage_load<TWO, ST>(P, cur, 0, iy, raw);
+            for (int p = 0; p < P.nviews; ++p) {
+                if (cur.total > P.lds_cap) {
+                    // host bound violated: report it (the host fails the call) and skip
+                    // the view rather than overrun LDS
+                    if (tid == 0) atomicOr(P.err, 1);
+                    cur.total = 0;
+                    cur.NR = 0;
+                    cur.L = 0;
+                    cur.row = -1;
+                }
+                // buffer (p & 1) was last read by the compute of view p-2; every thread
+                // has passed the barrier of view p-1 since, so it is free.
+                const int buf = (p & 1) * P.lds_cap;
+                stage_store<TWO, ST>(P, cur, p, iy, raw, lds4, buf);
+                __syncthreads();
+                Vs nxt = cur;
+                if (p + 1 < P.nviews) {
+                    nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
+                    stage_load<TWO, ST>(P, nxt, p + 1, iy, raw);
+                }
+                const TILTGPU_CONSTANT float* zc =
+                    cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
+                if constexpr (TWO) {
+                    if (cur.row >= 0) {
+                        const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
+                        const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
+                        if (LOCAL && cur.shift) {
+    #pragma unroll
+                            for (int k = 0; k < ZPT; ++k) {
+                                const float px = ((a + zc[k]) + P.cx) + ldx;
+                                const float4 q = E[cvt_floor(px)];
+                                const float f = __builtin_amdgcn_fractf(px);
+                                acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                            }
+                        } else {
+    #pragma unroll
+                            for (int k = 0; k < ZPT; ++k) {
+                                const float px = (a + zc[k]) + P.cx;
+                                const float4 q = E[cvt_floor(px)];
+                                const float f = __builtin_amdgcn_fractf(px);
+                                acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                            }
                         }
                     }
+                } else if (cur.total > 0) {
+                    const float2* E = reinterpret_cast<const float2*>(lds4) + buf - cur.seg_lo;
+                    const float a = cur.a;
+    #pragma unroll
+                    for (int k = 0; k < ZPT; ++k) {
+                        const float px = (a + zc[k]) + P.cx;
+                        const float2 q = E[cvt_floor(px)];
+                        const float f = __builtin_amdgcn_fractf(px);
+                        acc[k] += fmaf(f, q.y, q.x);
+                    }
                 }
-            } else if (cur.total > 0) {
-                const float2* E = reinterpret_cast<const float2*>(lds4) + buf - cur.seg_lo;
-                const float a = cur.a;
-#pragma unroll
-                for (int k = 0; k < ZPT; ++k) {
-                    const float px = (a + zc[k]) + P.cx;
-                    const float2 q = E[cvt_floor(px)];
-                    const float f = __builtin_amdgcn_fractf(px);
-                    acc[k] += fmaf(f, q.y, q.x);
-                }
+                cur = nxt;
             }
-            cur = nxt;
         }
-    }
 
-    if (ox >= P.out_nx) return;
-    const long slice = static_cast<long>(P.out_nz) * P.out_nx;
-    float* dst = P.out + static_cast<long>(iy - P.y0) * slice + ox;
-#pragma unroll
-    for (int k = 0; k < ZPT; ++k) {
-        const int oz = zw0 + k;
-        if (oz < P.out_nz) {
-            const float v = acc[k] * P.inv_n;
-            float* o = dst + static_cast<long>(oz) * P.out_nx;
-            if (P.out_mode == 0)
-                *o = v;
-            else if (P.out_mode == 1)
-                *o = (v * P.dw) * P.os + P.oa;
-            else
-                *o = *o + P.relax * v;
+        // no early return: in the persistent batched kernel the workgroup goes on
+        // to further tiles and its barriers
+        const long slice = static_cast<long>(P.out_nz) * P.out_nx;
+        float* dst = P.out + static_cast<long>(iy - P.y0) * slice + ox;
+    #pragma unroll
+        for (int k = 0; k < ZPT; ++k) {
+            const int oz = zw0 + k;
+            if (oz < P.out_nz && ox < P.out_nx) {
+                const float v = acc[k] * P.inv_n;
+                float* o = dst + static_cast<long>(oz) * P.out_nx;
+                if (P.out_mode == 0)
+                    *o = v;
+                else if (P.out_mode == 1)
+                    *o = (v * P.dw) * P.os + P.oa;
+                else
+                    *o = *o + P.relax * v;
+            }
         }
+        if constexpr (!BATCH) break;
     }
 }
 
@@ -470,8 +522,8 @@ static hipError_t launch_bp_t(const BpParams& p, hipStream_t st) {
     const long nblk = static_cast<long>(p.ntx) * p.ntz * (p.y1 - p.y0);
     if (nblk <= 0) return hipSuccess;
     const size_t lds = bp_lds_bytes(p.lds_cap, TWO);
-    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST>), dim3(static_cast<unsigned>(nblk)),
-                       dim3(kBpThreads), lds, st, p);
+    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, false>), dim3(static_cast<unsigned>(nblk)),
+                       dim3(kBpThreads), lds, st, p, BpBatch{});
     return hipGetLastError();
 }
 
@@ -493,6 +545,44 @@ static hipError_t launch_bp_z(const BpParams& p, int storage, int two_row, hipSt
     TILTGPU_BP_CASE(true, true, 2)
 #undef TILTGPU_BP_CASE
     return hipErrorInvalidValue;
+}
+
+template <int ZPT, bool TWO, bool LOCAL, int ST>
+static hipError_t launch_bp_batch_t(const BpBatch& b, int grid, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, true>), dim3(grid), dim3(kBpThreads), lds,
+                       st, BpParams{}, b);
+    return hipGetLastError();
+}
+
+template <int ZPT>
+static hipError_t launch_bp_batch_z(const BpBatch& b, int storage, int two_row, int local,
+                                    int grid, size_t lds, hipStream_t st) {
+    if (!two_row && local) return hipErrorInvalidValue;
+#define TILTGPU_BPB_CASE(TW, LC, S)                          \
+    if (two_row == TW && local == LC && storage == S)        \
+        return launch_bp_batch_t<ZPT, TW, LC, S>(b, grid, lds, st);
+    TILTGPU_BPB_CASE(0, 0, 0)
+    TILTGPU_BPB_CASE(0, 0, 1)
+    TILTGPU_BPB_CASE(0, 0, 2)
+    TILTGPU_BPB_CASE(1, 0, 0)
+    TILTGPU_BPB_CASE(1, 0, 1)
+    TILTGPU_BPB_CASE(1, 0, 2)
+    TILTGPU_BPB_CASE(1, 1, 0)
+    TILTGPU_BPB_CASE(1, 1, 1)
+    TILTGPU_BPB_CASE(1, 1, 2)
+#undef TILTGPU_BPB_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, int local,
+                                    int tile_z, int grid, size_t lds, hipStream_t st) {
+    if (b.total <= 0) return hipSuccess;
+    switch (tile_z) {
+        case 32: return launch_bp_batch_z<8>(b, storage, two_row, local, grid, lds, st);
+        case 64: return launch_bp_batch_z<16>(b, storage, two_row, local, grid, lds, st);
+        case 128: return launch_bp_batch_z<32>(b, storage, two_row, local, grid, lds, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z,

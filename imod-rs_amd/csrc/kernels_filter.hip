@@ -198,13 +198,14 @@ __device__ float stretch_sample(const float2* row, int c, int n, float s, int cu
     return 0.f;
 }
 
-__global__ void __launch_bounds__(256) filter_kernel(FilterParams P) {
+// Workgroup `blk` of one filter launch: row pairs [blk*ppw, (blk+1)*ppw).
+__device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
     extern __shared__ float2 sm[];
     const int n = P.n;
     const int ppw = P.pairs_per_wg;
     float2* b0 = sm;
     float2* b1 = sm + ppw * n;
-    const int pair0 = blockIdx.x * ppw;
+    const int pair0 = blk * ppw;
     const int total_pairs = (P.nrows + 1) / 2;
     const int pairs = min(ppw, total_pairs - pair0);
     const int tid = threadIdx.x;
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterParams P) {
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) P.sq_partial[blockIdx.x] = red[0];
+        if (tid == 0) P.sq_partial[blk] = red[0];
     }
     __syncthreads();
 
@@ -281,6 +282,35 @@ __global__ void __launch_bounds__(256) filter_kernel(FilterParams P) {
             P.dst[static_cast<long>(r) * n + ox] = v;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) filter_kernel(FilterParams P) { filter_body(P, blockIdx.x); }
+
+// Several series in one launch: workgroup b belongs to the series r with
+// block_start[r] <= b < block_start[r+1]; each series keeps its own length,
+// plan and options, so its rows are filtered exactly as in a launch of its own.
+__global__ void __launch_bounds__(256) filter_batch_kernel(FilterBatch B) {
+    int a = 0, c = B.nreq - 1;
+    const int b = blockIdx.x;
+    while (a < c) {
+        const int m = (a + c + 1) / 2;
+        if (B.block_start[m] <= b)
+            a = m;
+        else
+            c = m - 1;
+    }
+    filter_body(B.descs[a], b - B.block_start[a]);
+}
+
+hipError_t launch_filter_batch(const FilterBatch& b, int total_blocks, size_t lds, hipStream_t st) {
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(filter_batch_kernel, dim3(total_blocks), dim3(256), lds, st, b);
+    return hipGetLastError();
+}
+
+int filter_blocks(const FilterParams& p) {
+    const int total_pairs = (p.nrows + 1) / 2;
+    return (total_pairs + p.pairs_per_wg - 1) / p.pairs_per_wg;
 }
 
 size_t filter_lds_bytes(int n, int pairs_per_wg) {

@@ -139,10 +139,7 @@ hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
 // reference's per-bin order (oz ascending, then ox ascending), adding
 // val*frac / val*(1-frac) with the reference's rounding: the result equals
 // the serial scatter bit for bit.
-__global__ void __launch_bounds__(256) fp_gather_kernel(FpParams P) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y;
-    const int p = blockIdx.z;
+__device__ __forceinline__ void fp_gather_bin(const FpParams& P, int x, int iy, int p) {
     if (x >= P.in_nx) return;
     const float4 g = P.vgeom[p];
     const float c = g.x * g.z;
@@ -177,6 +174,40 @@ __global__ void __launch_bounds__(256) fp_gather_kernel(FpParams P) {
         }
     }
     P.sino[(static_cast<int64_t>(p) * P.in_ny + iy) * P.in_nx + x] = acc;
+}
+
+__global__ void __launch_bounds__(256) fp_gather_kernel(FpParams P) {
+    fp_gather_bin(P, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Several series in one launch: workgroup b of series r covers 256 bins of one
+// (view, row); per-bin work and order are those of a launch of its own.
+__global__ void __launch_bounds__(256) fp_gather_batch_kernel(FpBatch B) {
+    int a = 0, c = B.nreq - 1;
+    const int b = blockIdx.x;
+    while (a < c) {
+        const int m = (a + c + 1) / 2;
+        if (B.block_start[m] <= b)
+            a = m;
+        else
+            c = m - 1;
+    }
+    const FpParams& P = B.descs[a];
+    const int local = b - B.block_start[a];
+    const int nbx = (P.in_nx + 255) / 256;
+    const int bx = local % nbx;
+    const int rest = local / nbx;
+    fp_gather_bin(P, bx * 256 + static_cast<int>(threadIdx.x), rest % P.in_ny, rest / P.in_ny);
+}
+
+int fp_gather_blocks(const FpParams& p) {
+    return ((p.in_nx + 255) / 256) * p.in_ny * p.nviews;
+}
+
+hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, hipStream_t st) {
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fp_gather_batch_kernel, dim3(total_blocks), dim3(256), 0, st, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {

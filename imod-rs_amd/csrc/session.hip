@@ -16,12 +16,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tiltgpu.h"
-#include "tiltgpu_internal.h"
+#include "session_internal.h"
 
-using namespace tiltgpu;
-
-namespace {
+namespace tiltgpu_sess {
 
 thread_local std::string g_last_error;
 
@@ -35,154 +32,11 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                 \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(TILTGPU_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
-                        hipGetErrorString(e_), __FILE__, __LINE__);                   \
-    } while (0)
+}  // namespace tiltgpu_sess
 
-struct TimedRegion {
-    hipEvent_t a, b;
-    int cls;
-};
+using namespace tiltgpu_sess;
 
-}  // namespace
-
-enum { kClsPrep = 0, kClsFilter, kClsBp, kClsFp, kClsOther, kNumCls };
-
-struct tiltgpu_session {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    tiltgpu_dims d{};
-    tiltgpu_options o{};
-    bool seam = false;  // created by tiltgpu_bp_session_new
-
-    std::vector<float> angles, xtilt, zfac;
-    bool has_xtilt = false, has_zfac = false, has_angles = false;
-    int nviews = 0;
-    bool geometry_ready = false;
-
-    // local alignment (host copy)
-    bool has_local = false;
-    std::vector<int> lnx, lny, lxoff, lyoff, lgoff;
-    std::vector<float> lxs, lys, ldx, ldy;
-
-    // device buffers
-    float* raw = nullptr;      // uploaded projections (never modified)
-    float* logd = nullptr;     // --log output (when enabled)
-    float* filt = nullptr;     // filtered + stretched, f32
-    void* store = nullptr;     // f16 / bf16 copy of filt
-    float* scale = nullptr;    // [nz] dequantisation scale
-    unsigned* absmax = nullptr;
-    float* vol = nullptr;      // [out_ny][out_nz][out_nx]
-    float* vol_tmp = nullptr;  // fake-SIRT ping-pong
-    float* sino = nullptr;     // SIRT reprojection [nz][in_ny][in_nx]
-    float4* vgeom = nullptr;
-    BpView* views = nullptr;
-    float* zc = nullptr;
-    float* means = nullptr;
-    double* partial = nullptr;
-    int partial_n = 0;
-    float* stretch = nullptr;
-    float* weights = nullptr;
-    float2* tw = nullptr;
-    int* err = nullptr;
-    double* resid = nullptr;
-    float* stats = nullptr;
-    int stats_rows = 0;
-    int* d_lnx = nullptr;
-    int* d_lny = nullptr;
-    int* d_lxoff = nullptr;
-    int* d_lyoff = nullptr;
-    int* d_lgoff = nullptr;
-    float* d_lxs = nullptr;
-    float* d_lys = nullptr;
-    float* d_ldx = nullptr;
-    float* d_ldy = nullptr;
-
-    int nz_pad = 0;
-    int tile_z = 64;
-    int tile_z_override = 0;  // TILTGPU_TILE_Z (tuning): force the backprojection tile depth
-    int lds_cap = 0;
-    bool two_row = false;
-    int radix[kMaxRadix];
-    int nradix = 0;
-    bool uploaded = false;
-    bool ran = false;
-    int sirt_done = 0;
-    int64_t bytes = 0;
-    std::vector<void*> allocs;
-
-    std::vector<TimedRegion> pending;
-    std::vector<hipEvent_t> event_pool;
-    double ms[kNumCls] = {0, 0, 0, 0, 0};
-    int64_t n_bp = 0, n_filter = 0, n_fp = 0;
-};
-
-namespace {
-
-template <typename T>
-int dev_alloc(tiltgpu_session* s, T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    void* q = nullptr;
-    const size_t bytes = count * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess)
-        return fail(e == hipErrorOutOfMemory ? TILTGPU_ERR_BUDGET : TILTGPU_ERR_HIP,
-                    "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
-    s->allocs.push_back(q);
-    s->bytes += static_cast<int64_t>(bytes);
-    *p = static_cast<T*>(q);
-    return TILTGPU_OK;
-}
-
-template <typename T>
-void dev_free(tiltgpu_session* s, T*& p) {
-    if (!p) return;
-    auto it = std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(p));
-    if (it != s->allocs.end()) s->allocs.erase(it);
-    (void)hipFree(p);
-    p = nullptr;
-}
-
-hipEvent_t take_event(tiltgpu_session* s) {
-    if (!s->event_pool.empty()) {
-        hipEvent_t e = s->event_pool.back();
-        s->event_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    (void)hipEventCreate(&e);
-    return e;
-}
-
-struct Scope {
-    tiltgpu_session* s;
-    TimedRegion r;
-    Scope(tiltgpu_session* s_, int cls) : s(s_) {
-        r.a = take_event(s);
-        r.b = take_event(s);
-        r.cls = cls;
-        (void)hipEventRecord(r.a, s->stream);
-    }
-    ~Scope() {
-        (void)hipEventRecord(r.b, s->stream);
-        s->pending.push_back(r);
-    }
-};
-
-void collect_timing(tiltgpu_session* s) {
-    for (auto& r : s->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) s->ms[r.cls] += ms;
-        s->event_pool.push_back(r.a);
-        s->event_pool.push_back(r.b);
-    }
-    s->pending.clear();
-}
+namespace tiltgpu_sess {
 
 float f32_proj_cos(float deg) {
     const float rad = deg * static_cast<float>(M_PI) / 180.0f;
@@ -459,10 +313,8 @@ int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int
 
 // Filter (and optionally stretch) rows of `src` (minus `sub`) into s->filt and,
 // for FP16/BF16 storage, convert into s->store.
-int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
-               double* sq_out) {
+FilterParams make_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch) {
     const int n = s->d.in_nx;
-    const int nrows = s->d.nz * s->d.in_ny;
     FilterParams f{};
     f.src = src;
     f.src_sub = sub;
@@ -471,20 +323,33 @@ int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stre
     f.tw = s->tw;
     f.stretch = (stretch && !s->o.no_cosine_stretch) ? s->stretch : nullptr;
     f.n = n;
-    f.nrows = nrows;
+    f.nrows = s->d.nz * s->d.in_ny;
     f.rows_per_view = s->d.in_ny;
     f.pairs_per_wg = pick_pairs_per_wg(n);
     f.cubic = s->o.cosine_cubic;
     f.nradix = s->nradix;
     for (int i = 0; i < s->nradix; ++i) f.radix[i] = s->radix[i];
-    const int nblk = ((nrows + 1) / 2 + f.pairs_per_wg - 1) / f.pairs_per_wg;
+    return f;
+}
+
+// Per-workgroup partial buffer for the residual sum of squares of a filter launch.
+static int ensure_partial(tiltgpu_session* s, int nblk) {
+    if (s->partial_n < nblk) {
+        dev_free(s, s->partial);
+        int rc = dev_alloc(s, &s->partial, nblk);
+        if (rc) return rc;
+        s->partial_n = nblk;
+    }
+    return TILTGPU_OK;
+}
+
+int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
+               double* sq_out) {
+    FilterParams f = make_filter(s, src, sub, stretch);
+    const int nblk = filter_blocks(f);
     if (sq_out) {
-        if (s->partial_n < nblk) {
-            dev_free(s, s->partial);
-            int rc = dev_alloc(s, &s->partial, nblk);
-            if (rc) return rc;
-            s->partial_n = nblk;
-        }
+        int rc = ensure_partial(s, nblk);
+        if (rc) return rc;
         f.sq_partial = s->partial;
     }
     {
@@ -492,7 +357,7 @@ int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stre
         HIP_TRY(launch_filter(f, s->stream));
         if (s->o.storage != TILTGPU_STORE_F32)
             HIP_TRY(launch_convert_storage(s->filt, s->store, s->scale, s->absmax,
-                                           static_cast<int64_t>(n) * s->d.in_ny, s->d.nz,
+                                           static_cast<int64_t>(f.n) * s->d.in_ny, s->d.nz,
                                            s->o.storage, s->stream));
         s->n_filter++;
     }
@@ -500,6 +365,36 @@ int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stre
         Scope sc(s, kClsOther);
         HIP_TRY(launch_sum_partials(s->partial, nblk, sq_out, 1, s->stream));
     }
+    return TILTGPU_OK;
+}
+
+FpParams make_fp(tiltgpu_session* s) {
+    FpParams p{};
+    p.vol = s->vol;
+    p.sino = s->sino;
+    p.vgeom = s->vgeom;
+    p.zc_tab = s->zc;
+    p.in_nx = s->d.in_nx;
+    p.in_ny = s->d.in_ny;
+    p.nviews = s->nviews;
+    p.out_nx = s->d.out_nx;
+    p.out_nz = s->d.out_nz;
+    p.nz_pad = s->nz_pad;
+    p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
+    p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
+    return p;
+}
+
+// --log and per-view means (main.rs:607-623)
+int run_means(tiltgpu_session* s) {
+    const int nz = s->d.nz;
+    const int64_t nxy = static_cast<int64_t>(s->d.in_nx) * s->d.in_ny;
+    const int nblk = static_cast<int>(std::min<int64_t>(64, (nxy + 4095) / 4096));
+    int rc = ensure_partial(s, nblk * nz);
+    if (rc) return rc;
+    Scope sc(s, kClsPrep);
+    HIP_TRY(launch_log_means(s->raw, s->logd, nxy, nz, s->o.log, s->partial, nblk, s->means,
+                             s->stream));
     return TILTGPU_OK;
 }
 
@@ -539,20 +434,7 @@ int run_forward(tiltgpu_session* s) {
             HIP_TRY(launch_fill_f32(s->sino + s->nviews * per, (s->d.nz - s->nviews) * per, 0.f,
                                     s->stream));
         }
-        FpParams p{};
-        p.vol = s->vol;
-        p.sino = s->sino;
-        p.vgeom = s->vgeom;
-        p.zc_tab = s->zc;
-        p.in_nx = s->d.in_nx;
-        p.in_ny = s->d.in_ny;
-        p.nviews = s->nviews;
-        p.out_nx = s->d.out_nx;
-        p.out_nz = s->d.out_nz;
-        p.nz_pad = s->nz_pad;
-        p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
-        p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
-        HIP_TRY(launch_forward_gather(p, s->stream));
+        HIP_TRY(launch_forward_gather(make_fp(s), s->stream));
     }
     s->n_fp++;
     return TILTGPU_OK;
@@ -566,7 +448,7 @@ int validate_dims(const tiltgpu_dims* d) {
     return TILTGPU_OK;
 }
 
-int init_device(tiltgpu_session* s) {
+int init_device(tiltgpu_session* s, hipStream_t external) {
     if (const char* tz = std::getenv("TILTGPU_TILE_Z")) {
         const int v = std::atoi(tz);
         if (v == 32 || v == 64 || v == 128) s->tile_z_override = v;
@@ -575,14 +457,78 @@ int init_device(tiltgpu_session* s) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
         return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");
     HIP_TRY(hipGetDevice(&s->device));
-    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if (external) {
+        s->stream = external;
+        s->own_stream = false;
+    } else {
+        HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
     int rc;
     if ((rc = dev_alloc(s, &s->err, 1))) return rc;
     HIP_TRY(hipMemset(s->err, 0, sizeof(int)));
     return TILTGPU_OK;
 }
 
-}  // namespace
+}  // namespace tiltgpu_sess
+
+namespace tiltgpu_sess {
+
+int session_create(const tiltgpu_dims* dims, const tiltgpu_options* opts, hipStream_t stream,
+                   tiltgpu_session** out) {
+    if (!out) return fail(TILTGPU_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = validate_dims(dims);
+    if (rc) return rc;
+    auto* s = new tiltgpu_session();
+    s->d = *dims;
+    if (opts)
+        s->o = *opts;
+    else
+        tiltgpu_default_options(&s->o);
+    if (s->o.storage < 0 || s->o.storage > 2) {
+        delete s;
+        return fail(TILTGPU_ERR_ARG, "unknown storage %d", s->o.storage);
+    }
+    if (s->o.fill_mode < 0 || s->o.fill_mode > 2) s->o.fill_mode = TILTGPU_FILL_MEAN;
+    if ((rc = init_device(s, stream)) || (rc = prepare_filter(s))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    const int64_t nvol = static_cast<int64_t>(s->d.in_ny) * s->d.out_nz * s->d.out_nx;
+    if ((rc = dev_alloc(s, &s->raw, nproj)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
+        (rc = dev_alloc(s, &s->vol, nvol)) || (rc = dev_alloc(s, &s->means, s->d.nz)) ||
+        (rc = dev_alloc(s, &s->scale, s->d.nz)) || (rc = dev_alloc(s, &s->absmax, s->d.nz))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    if (s->o.log && (rc = dev_alloc(s, &s->logd, nproj))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    if (s->o.storage != TILTGPU_STORE_F32) {
+        unsigned short* st = nullptr;
+        if ((rc = dev_alloc(s, &st, nproj))) {
+            tiltgpu_destroy(s);
+            return rc;
+        }
+        s->store = st;
+    }
+    if (s->o.sirt > 0) {
+        if ((rc = dev_alloc(s, &s->sino, nproj)) || (rc = dev_alloc(s, &s->resid, s->o.sirt))) {
+            tiltgpu_destroy(s);
+            return rc;
+        }
+    }
+    if (s->o.fake_sirt > 0 && (rc = dev_alloc(s, &s->vol_tmp, nvol))) {
+        tiltgpu_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return TILTGPU_OK;
+}
+
+}  // namespace tiltgpu_sess
 
 // ======================================================================= ABI
 
@@ -663,57 +609,7 @@ int tiltgpu_radial_weights(int32_t n, int32_t exact_filter_size, float cutoff, f
 
 int tiltgpu_create(const tiltgpu_dims* dims, const tiltgpu_options* opts,
                    tiltgpu_session** out) {
-    if (!out) return fail(TILTGPU_ERR_ARG, "out is NULL");
-    *out = nullptr;
-    int rc = validate_dims(dims);
-    if (rc) return rc;
-    auto* s = new tiltgpu_session();
-    s->d = *dims;
-    if (opts)
-        s->o = *opts;
-    else
-        tiltgpu_default_options(&s->o);
-    if (s->o.storage < 0 || s->o.storage > 2) {
-        delete s;
-        return fail(TILTGPU_ERR_ARG, "unknown storage %d", s->o.storage);
-    }
-    if (s->o.fill_mode < 0 || s->o.fill_mode > 2) s->o.fill_mode = TILTGPU_FILL_MEAN;
-    if ((rc = init_device(s)) || (rc = prepare_filter(s))) {
-        tiltgpu_destroy(s);
-        return rc;
-    }
-    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
-    const int64_t nvol = static_cast<int64_t>(s->d.in_ny) * s->d.out_nz * s->d.out_nx;
-    if ((rc = dev_alloc(s, &s->raw, nproj)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
-        (rc = dev_alloc(s, &s->vol, nvol)) || (rc = dev_alloc(s, &s->means, s->d.nz)) ||
-        (rc = dev_alloc(s, &s->scale, s->d.nz)) || (rc = dev_alloc(s, &s->absmax, s->d.nz))) {
-        tiltgpu_destroy(s);
-        return rc;
-    }
-    if (s->o.log && (rc = dev_alloc(s, &s->logd, nproj))) {
-        tiltgpu_destroy(s);
-        return rc;
-    }
-    if (s->o.storage != TILTGPU_STORE_F32) {
-        unsigned short* st = nullptr;
-        if ((rc = dev_alloc(s, &st, nproj))) {
-            tiltgpu_destroy(s);
-            return rc;
-        }
-        s->store = st;
-    }
-    if (s->o.sirt > 0) {
-        if ((rc = dev_alloc(s, &s->sino, nproj)) || (rc = dev_alloc(s, &s->resid, s->o.sirt))) {
-            tiltgpu_destroy(s);
-            return rc;
-        }
-    }
-    if (s->o.fake_sirt > 0 && (rc = dev_alloc(s, &s->vol_tmp, nvol))) {
-        tiltgpu_destroy(s);
-        return rc;
-    }
-    *out = s;
-    return TILTGPU_OK;
+    return session_create(dims, opts, nullptr, out);
 }
 
 int tiltgpu_set_angles(tiltgpu_session* s, const float* a, int32_t n) {
@@ -787,22 +683,9 @@ int tiltgpu_run(tiltgpu_session* s) {
     if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
     int rc;
     if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
-    const int nz = s->d.nz;
-    const int64_t nxy = static_cast<int64_t>(s->d.in_nx) * s->d.in_ny;
     const float* O = s->o.log ? s->logd : s->raw;
     // log + means
-    {
-        const int nblk = static_cast<int>(std::min<int64_t>(64, (nxy + 4095) / 4096));
-        const int need = nblk * nz;
-        if (s->partial_n < need) {
-            dev_free(s, s->partial);
-            if ((rc = dev_alloc(s, &s->partial, need))) return rc;
-            s->partial_n = need;
-        }
-        Scope sc(s, kClsPrep);
-        HIP_TRY(launch_log_means(s->raw, s->logd, nxy, nz, s->o.log, s->partial, nblk, s->means,
-                                 s->stream));
-    }
+    if ((rc = run_means(s))) return rc;
     // R-weighting + cosine stretch (+ storage conversion)
     if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
@@ -950,7 +833,7 @@ void tiltgpu_destroy(tiltgpu_session* s) {
     }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
     for (void* p : s->allocs) (void)hipFree(p);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -974,7 +857,7 @@ int tiltgpu_bp_session_new(const float* projections, const tiltgpu_view_params* 
     s->o.fill_mode = (fill_mode >= 0 && fill_mode <= 2) ? fill_mode : TILTGPU_FILL_MEAN;
     const int64_t nproj = static_cast<int64_t>(nz) * in_ny * in_nx;
     std::vector<tiltgpu_view_params> vp(params, params + n_views);
-    if ((rc = init_device(s)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
+    if ((rc = init_device(s, nullptr)) || (rc = dev_alloc(s, &s->filt, nproj)) ||
         (rc = dev_alloc(s, &s->means, nz)) || (rc = setup_geometry(s, vp))) {
         tiltgpu_destroy(s);
         return rc;

@@ -1,0 +1,195 @@
+// Internal: the session object and the helpers shared by the single-series
+// ABI (session.hip) and the batching server (server.hip).  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/tiltgpu.h"
+#include "tiltgpu_internal.h"
+
+namespace tiltgpu_sess {
+
+// Records the calling thread's last error message; returns code.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return ::tiltgpu_sess::fail(TILTGPU_ERR_HIP, "%s failed: %s (%s:%d)", #expr, \
+                                        hipGetErrorString(e_), __FILE__, __LINE__);   \
+    } while (0)
+
+struct TimedRegion {
+    hipEvent_t a, b;
+    int cls;
+};
+
+}  // namespace tiltgpu_sess
+
+using tiltgpu_sess::TimedRegion;
+using namespace tiltgpu;
+
+enum { kClsPrep = 0, kClsFilter, kClsBp, kClsFp, kClsOther, kNumCls };
+
+struct tiltgpu_session {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;  // false when a server lends its stream
+    tiltgpu_dims d{};
+    tiltgpu_options o{};
+    bool seam = false;  // created by tiltgpu_bp_session_new
+
+    std::vector<float> angles, xtilt, zfac;
+    bool has_xtilt = false, has_zfac = false, has_angles = false;
+    int nviews = 0;
+    bool geometry_ready = false;
+
+    // local alignment (host copy)
+    bool has_local = false;
+    std::vector<int> lnx, lny, lxoff, lyoff, lgoff;
+    std::vector<float> lxs, lys, ldx, ldy;
+
+    // device buffers
+    float* raw = nullptr;      // uploaded projections (never modified)
+    float* logd = nullptr;     // --log output (when enabled)
+    float* filt = nullptr;     // filtered + stretched, f32
+    void* store = nullptr;     // f16 / bf16 copy of filt
+    float* scale = nullptr;    // [nz] dequantisation scale
+    unsigned* absmax = nullptr;
+    float* vol = nullptr;      // [out_ny][out_nz][out_nx]
+    float* vol_tmp = nullptr;  // fake-SIRT ping-pong
+    float* sino = nullptr;     // SIRT reprojection [nz][in_ny][in_nx]
+    float4* vgeom = nullptr;
+    BpView* views = nullptr;
+    float* zc = nullptr;
+    float* means = nullptr;
+    double* partial = nullptr;
+    int partial_n = 0;
+    float* stretch = nullptr;
+    float* weights = nullptr;
+    float2* tw = nullptr;
+    int* err = nullptr;
+    double* resid = nullptr;
+    float* stats = nullptr;
+    int stats_rows = 0;
+    int* d_lnx = nullptr;
+    int* d_lny = nullptr;
+    int* d_lxoff = nullptr;
+    int* d_lyoff = nullptr;
+    int* d_lgoff = nullptr;
+    float* d_lxs = nullptr;
+    float* d_lys = nullptr;
+    float* d_ldx = nullptr;
+    float* d_ldy = nullptr;
+
+    int nz_pad = 0;
+    int tile_z = 64;
+    int tile_z_override = 0;  // TILTGPU_TILE_Z (tuning): force the backprojection tile depth
+    int lds_cap = 0;
+    bool two_row = false;
+    int radix[kMaxRadix];
+    int nradix = 0;
+    bool uploaded = false;
+    bool ran = false;
+    int sirt_done = 0;
+    int64_t bytes = 0;
+    std::vector<void*> allocs;
+
+    std::vector<TimedRegion> pending;
+    std::vector<hipEvent_t> event_pool;
+    double ms[kNumCls] = {0, 0, 0, 0, 0};
+    int64_t n_bp = 0, n_filter = 0, n_fp = 0;
+};
+
+
+namespace tiltgpu_sess {
+
+template <typename T>
+inline int dev_alloc(tiltgpu_session* s, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? TILTGPU_ERR_BUDGET : TILTGPU_ERR_HIP,
+                    "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    s->allocs.push_back(q);
+    s->bytes += static_cast<int64_t>(bytes);
+    *p = static_cast<T*>(q);
+    return TILTGPU_OK;
+}
+
+template <typename T>
+inline void dev_free(tiltgpu_session* s, T*& p) {
+    if (!p) return;
+    auto it = std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(p));
+    if (it != s->allocs.end()) s->allocs.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+inline hipEvent_t take_event(tiltgpu_session* s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct Scope {
+    tiltgpu_session* s;
+    TimedRegion r;
+    Scope(tiltgpu_session* s_, int cls) : s(s_) {
+        r.a = take_event(s);
+        r.b = take_event(s);
+        r.cls = cls;
+        (void)hipEventRecord(r.a, s->stream);
+    }
+    ~Scope() {
+        (void)hipEventRecord(r.b, s->stream);
+        s->pending.push_back(r);
+    }
+};
+
+inline void collect_timing(tiltgpu_session* s) {
+    for (auto& r : s->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) s->ms[r.cls] += ms;
+        s->event_pool.push_back(r.a);
+        s->event_pool.push_back(r.b);
+    }
+    s->pending.clear();
+}
+
+
+int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& vp);
+int upload_local(tiltgpu_session* s);
+int prepare_pipeline_geometry(tiltgpu_session* s);
+int prepare_filter(tiltgpu_session* s);
+int check_kernel_error(tiltgpu_session* s);
+BpParams make_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
+                 int out_mode, float* out);
+FilterParams make_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch);
+FpParams make_fp(tiltgpu_session* s);
+int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
+           int out_mode, float* out);
+int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
+               double* sq_out);
+int run_forward(tiltgpu_session* s);
+int run_means(tiltgpu_session* s);
+int validate_dims(const tiltgpu_dims* d);
+// Allocates the session's buffers on `stream` (nullptr: create its own stream).
+int session_create(const tiltgpu_dims* dims, const tiltgpu_options* opts, hipStream_t stream,
+                   tiltgpu_session** out);
+int pick_pairs_per_wg(int n);
+
+}  // namespace tiltgpu_sess

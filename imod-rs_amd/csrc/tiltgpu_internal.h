@@ -59,6 +59,15 @@ struct BpParams {
     int* err;                 // set to 1 when a tile needs more than lds_cap entries
 };
 
+// Persistent batched backprojection: the tiles of nreq series (descs[r] covers
+// tiles [tile_start[r], tile_start[r+1])) claimed from 8 per-XCD chunk counters.
+struct BpBatch {
+    const BpParams* descs;   // [nreq] device copies
+    const int* tile_start;   // [nreq + 1]
+    int nreq, total, chunk;  // chunk = ceil(total / 8)
+    int* counters;           // [8], zero before the launch
+};
+
 // R-weighting filter + cosine stretch of rows of length n.
 struct FilterParams {
     const float* src;       // [nrows][n]
@@ -105,12 +114,29 @@ struct FpScatterParams {
     const float* ldy;
 };
 
+// Several series in one filter / forward-projection launch: workgroups
+// [block_start[r], block_start[r+1]) run series r.
+struct FilterBatch {
+    const FilterParams* descs;
+    const int* block_start;
+    int nreq;
+};
+struct FpBatch {
+    const FpParams* descs;
+    const int* block_start;
+    int nreq;
+};
+
 // ---- launchers (kernels_*.hip); all asynchronous on `st` ----
 hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z,
                               hipStream_t st);
 size_t bp_lds_bytes(int lds_cap, int two_row);
+hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, int local,
+                                    int tile_z, int grid, size_t lds, hipStream_t st);
 
 hipError_t launch_filter(const FilterParams& p, hipStream_t st);
+hipError_t launch_filter_batch(const FilterBatch& b, int total_blocks, size_t lds, hipStream_t st);
+int filter_blocks(const FilterParams& p);
 size_t filter_lds_bytes(int n, int pairs_per_wg);
 int filter_plan(int n, int* radix, int* nradix);  // 0 ok, -1 unsupported n
 
@@ -120,6 +146,8 @@ hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
                                   unsigned* absmax_bits, int64_t nxy, int nz, int storage,
                                   hipStream_t st);
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st);
+hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, hipStream_t st);
+int fp_gather_blocks(const FpParams& p);
 hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st);
 hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
                                hipStream_t st);

@@ -80,6 +80,14 @@ class MrcInfo(C.Structure):
                 ("label0", C.c_char * 81)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("rounds", C.c_int64),
+                ("bp_launches", C.c_int64), ("bp_batched_requests", C.c_int64),
+                ("max_active", C.c_int32), ("active", C.c_int32), ("queued", C.c_int32),
+                ("pad", C.c_int32), ("bytes_in_use", C.c_int64), ("budget_bytes", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double)]
+
+
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -116,6 +124,19 @@ SIGNATURES = [
                                          C.c_int32, C.c_int32, C.POINTER(_vp)]),
     ("tiltgpu_reconstruct_row", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_reconstruct_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_server_create", C.c_int, [C.c_int64, C.POINTER(_vp)]),
+    ("tiltgpu_server_submit", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
+                                        C.c_int32, _fp, _fp, C.POINTER(C.c_int64)]),
+    ("tiltgpu_server_add_series", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
+                                            C.c_int32, _fp, C.c_int32, _ip, _ip, _fp, _fp,
+                                            _fp, _fp, _fp, _ip]),
+    ("tiltgpu_server_submit_slab", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _fp,
+                                             C.POINTER(C.c_int64)]),
+    ("tiltgpu_server_poll", C.c_int, [_vp, C.c_int64]),
+    ("tiltgpu_server_wait", C.c_int, [_vp, C.c_int64, _dp]),
+    ("tiltgpu_server_get_stats", C.c_int, [_vp, C.POINTER(ServerStats)]),
+    ("tiltgpu_server_reset_stats", C.c_int, [_vp]),
+    ("tiltgpu_server_destroy", None, [_vp]),
     ("tiltgpu_copy_calibration", C.c_int, [C.c_int64, C.c_int32, _dp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
@@ -462,3 +483,99 @@ def copy_calibration(nbytes, iters=1):
     ms = C.c_double()
     _check(lib().tiltgpu_copy_calibration(int(nbytes), int(iters), C.byref(ms)))
     return ms.value
+
+
+class Server:
+    """Continuous-batching server (tiltgpu_server_*): requests from any thread are
+    admitted by device-memory budget and batched into shared launches."""
+
+    def __init__(self, budget_bytes):
+        h = _vp()
+        _check(lib().tiltgpu_server_create(int(budget_bytes), C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    def submit(self, P, angles, opts: TiltOptions | None = None):
+        """Queue a whole reconstruction; returns the request id."""
+        P = _f32(P)
+        ang = _f32(angles)
+        opts = opts or TiltOptions()
+        nz, ny, nx = P.shape
+        out_nx = opts.width or nx
+        out_nz = opts.thickness if opts.thickness > 0 else nx
+        d = Dims(nx, ny, nz, out_nx, out_nz)
+        o = opts.to_c()
+        out = np.empty((ny, out_nz, out_nx), np.float32)
+        rid = C.c_int64()
+        _check(lib().tiltgpu_server_submit(self.h, C.byref(d), C.byref(o), _f(ang), ang.size,
+                                           _f(P), _f(out), C.byref(rid)))
+        self._pending[rid.value] = out
+        return rid.value
+
+    def add_series(self, P, angles, opts: TiltOptions | None = None, xtilt=None, local=None):
+        P = _f32(P)
+        ang = _f32(angles)
+        opts = opts or TiltOptions()
+        nz, ny, nx = P.shape
+        d = Dims(nx, ny, nz, opts.width or nx, opts.thickness if opts.thickness > 0 else nx)
+        o = opts.to_c()
+        xt = _f32(xtilt)
+        loc = [None] * 6
+        if local is not None:
+            parts = local.flat() if hasattr(local, "flat") else local
+            loc = [np.ascontiguousarray(parts[0], np.int32), np.ascontiguousarray(parts[1], np.int32)] + \
+                  [_f32(v) if np.size(v) else np.zeros(1, np.float32) for v in parts[2:]]
+        self._series_keep = getattr(self, "_series_keep", []) + [loc, xt]
+        sid = C.c_int32()
+        _check(lib().tiltgpu_server_add_series(
+            self.h, C.byref(d), C.byref(o), _f(ang), ang.size, _f(xt),
+            0 if xt is None else xt.size,
+            None if loc[0] is None else loc[0].ctypes.data_as(_ip),
+            None if loc[1] is None else loc[1].ctypes.data_as(_ip),
+            *[None if v is None else _f(v) for v in loc[2:]], _f(P), C.byref(sid)))
+        self._series_shape = getattr(self, "_series_shape", {})
+        self._series_shape[sid.value] = (d.out_nz, d.out_nx)
+        return sid.value
+
+    def submit_slab(self, series_id, y0, y1):
+        onz, onx = self._series_shape[series_id]
+        out = np.empty((y1 - y0, onz, onx), np.float32)
+        rid = C.c_int64()
+        _check(lib().tiltgpu_server_submit_slab(self.h, series_id, y0, y1, _f(out), C.byref(rid)))
+        self._pending[rid.value] = out
+        return rid.value
+
+    def poll(self, rid):
+        return _check(lib().tiltgpu_server_poll(self.h, rid))
+
+    def wait(self, rid):
+        """Blocks until done; returns (output array, latency in ms)."""
+        lat = C.c_double()
+        out = self._pending.pop(rid)
+        _check(lib().tiltgpu_server_wait(self.h, rid, C.byref(lat)))
+        return out, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().tiltgpu_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "pad"}
+
+    def reset_stats(self):
+        _check(lib().tiltgpu_server_reset_stats(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tiltgpu_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -181,6 +181,47 @@ int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out);
 /* The same for rows [y0, y1) in one launch. */
 int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out);
 
+/* ------------------------------------------------- continuous batching */
+/* A server owns the device through one worker thread.  Requests from any host
+ * thread are admitted in FIFO order while their device memory fits the budget;
+ * every round batches the in-flight requests' filter, forward-projection and
+ * backprojection work into shared launches (persistent per-XCD tile queues for
+ * the backprojection).  A request's result is bit-identical alone and in any
+ * mix.  No counterpart in the reference: a build addition for serving. */
+typedef struct tiltgpu_server tiltgpu_server;
+typedef struct {
+    int64_t completed, failed, rounds, bp_launches, bp_batched_requests;
+    int32_t max_active, active, queued, pad;
+    int64_t bytes_in_use, budget_bytes;
+    double p50_ms, p99_ms, mean_ms;  /* submit-to-result latency of completed requests */
+} tiltgpu_server_stats;
+
+int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out);
+/* A whole reconstruction (the pipeline of tiltgpu_run); projections are copied at
+ * submit; `out` receives [in_ny][out_nz][out_nx] and must stay valid until the
+ * request is waited for. */
+int tiltgpu_server_submit(tiltgpu_server* s, const tiltgpu_dims* dims,
+                          const tiltgpu_options* opts, const float* angles, int32_t n_angles,
+                          const float* projections, float* out, int64_t* id);
+/* Keep a series resident (filtered once) for slab requests; returns when ready.
+ * xtilt and the local grids (as tiltgpu_set_local_alignment) may be NULL. */
+int tiltgpu_server_add_series(tiltgpu_server* s, const tiltgpu_dims* dims,
+                              const tiltgpu_options* opts, const float* angles, int32_t n_angles,
+                              const float* xtilt, int32_t n_xtilt, const int32_t* lnx,
+                              const int32_t* lny, const float* lxs, const float* lys,
+                              const float* ldx, const float* ldy, const float* projections,
+                              int32_t* series_id);
+/* Output rows [y0, y1) of a resident series: out = [y1-y0][out_nz][out_nx]. */
+int tiltgpu_server_submit_slab(tiltgpu_server* s, int32_t series_id, int32_t y0, int32_t y1,
+                               float* out, int64_t* id);
+/* 1 done, 0 pending, < 0 the request's error. */
+int tiltgpu_server_poll(tiltgpu_server* s, int64_t id);
+/* Blocks until the request is done; its latency (submit to result in `out`). */
+int tiltgpu_server_wait(tiltgpu_server* s, int64_t id, double* latency_ms);
+int tiltgpu_server_get_stats(tiltgpu_server* s, tiltgpu_server_stats* st);
+int tiltgpu_server_reset_stats(tiltgpu_server* s);
+void tiltgpu_server_destroy(tiltgpu_server* s);
+
 /* Diagnostic (no counterpart in the reference): `iters` launches of a plain
  * device-to-device copy kernel of `bytes` bytes; average ms per copy.  The known
  * byte count calibrates hardware traffic counters and gives the attainable
