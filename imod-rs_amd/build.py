@@ -19,8 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["csrc/kernels_bp.hip", "csrc/kernels_filter.hip", "csrc/kernels_misc.hip",
-               "csrc/session.hip", "csrc/server.hip"]
+HIP_SOURCES = ["csrc/kernels_bp.hip", "csrc/kernels_filter.hip", "csrc/kernels_fp.hip",
+               "csrc/kernels_misc.hip", "csrc/session.hip", "csrc/server.hip"]
 HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp"]
 CLI_SOURCES = ["host/tilt_main.cpp"]
 

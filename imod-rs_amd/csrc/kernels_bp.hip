@@ -40,15 +40,6 @@ __device__ __forceinline__ const TILTGPU_CONSTANT T* cptr(const T* p) {
     return (const TILTGPU_CONSTANT T*)(p);
 }
 
-// Workgroup id -> logical tile.  Workgroups are dispatched round-robin over the
-// 8 XCDs; giving each XCD a contiguous run of logical tiles keeps the tiles of
-// one Y row (which read the same projection rows) on one XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
-    const int X = 8;
-    const int q = nblk / X, r = nblk % X;
-    const int xcd = bid % X, j = bid / X;
-    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
-}
 
 // Raw projection sample as stored (f32 bits or 16-bit code); decoded after the
 // compute of the previous view so the loads stay in flight meanwhile.
@@ -94,12 +85,6 @@ __device__ __forceinline__ float lane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// floor(x) as an int in one instruction (x is already clamped to a small range).
-__device__ __forceinline__ int cvt_floor(float x) {
-    int i;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
-    return i;
-}
 
 // ViewPatches::find_bracket, main.rs:200-228.
 __device__ __forceinline__ void find_bracket(const float* pos, int len, float val, int& idx,

@@ -1,0 +1,464 @@
+// Forward projection of the SIRT iteration (main.rs:830-896) for gfx950.
+//
+// Single-row geometry (no x-axis tilt, no local alignment): an ordered voxel
+// walk.  Each lane owns B consecutive sinogram bins of one (view, row) and keeps
+// their sums in lane-private LDS slots.  For every z row, in ascending oz, it
+// walks the voxels whose projection can land on its bins in ascending ox and
+// adds v*(1-frac) to bin floor(px) and v*frac to bin floor(px)+1, exactly as the
+// serial scatter of main.rs:877-887 does.  A bin therefore receives its terms
+// in the reference's order (oz, then ox) and with the reference's roundings:
+// the sinogram equals the serial scatter bit for bit.  Every voxel is evaluated
+// once per lane window instead of once per bin it touches, and the loop has no
+// per-bin range search.
+//
+// Two-row geometry (x-axis tilt / local alignment): scatter with float atomics,
+// same coordinates and weights as main.rs:840-889; the per-bin summation order
+// is not the reference's, so results agree to rounding, not bit for bit.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+
+#include "tiltgpu_internal.h"
+
+namespace tiltgpu {
+
+namespace {
+
+constexpr int kFpWG = 256;          // threads per workgroup: 4 waves = 4 views
+constexpr int kFpViewsPerWG = kFpWG / 64;
+constexpr float kWalkMinC = 0.05f;  // views with cos_t*cos_xt below this use fp_bin_scan
+
+// One bin by direct scan (main.rs:855-887 restricted to the voxels that can land
+// on bin x, visited in the reference's order).  Used for views whose projected
+// voxel spacing cos_t*cos_xt is tiny or negative, where the walk's window bound
+// does not apply.
+__device__ void fp_bin_scan(const FpParams& P, int x, int iy, int p, float* out) {
+    const float4 g = P.vgeom[p];
+    const float c = g.x * g.z;
+    const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
+    const float* vrow = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx;
+    const float fx = static_cast<float>(x);
+    const float fn = static_cast<float>(P.in_nx);
+    const float fnx = static_cast<float>(P.out_nx);
+    float acc = 0.f;
+    for (int oz = 0; oz < P.out_nz; ++oz) {
+        const float zcv = zc[oz];
+        int lo = 0, hi = P.out_nx - 1;
+        if (fabsf(c) > 1e-4f) {
+            const float t0 = P.ocx + ((fx - 1.0f) - P.cx - zcv) / c;
+            const float t1 = P.ocx + ((fx + 1.0f) - P.cx - zcv) / c;
+            const float tl = fminf(t0, t1), th = fmaxf(t0, t1);
+            lo = static_cast<int>(fminf(fmaxf(floorf(tl) - 1.0f, 0.0f), fnx));
+            hi = static_cast<int>(fmaxf(fminf(ceilf(th) + 1.0f, fnx - 1.0f), -1.0f));
+        }
+        const float* vz = vrow + static_cast<int64_t>(oz) * P.out_nx;
+        for (int ox = lo; ox <= hi; ++ox) {
+            const float dx = static_cast<float>(ox) - P.ocx;
+            const float px = ((dx * g.x) * g.z + zcv) + P.cx;
+            const float fl = floorf(px);
+            if (fl >= 0.f && fl + 1.f < fn) {
+                const float frac = px - fl;
+                if (fl == fx)
+                    acc += vz[ox] * (1.0f - frac);
+                else if (fl == fx - 1.0f)
+                    acc += vz[ox] * frac;
+            }
+        }
+    }
+    *out = acc;
+}
+
+// Per-lane constants of one walk window: bins x0 .. x0+B-1 of (view p, row iy).
+// The lane's LDS slot s is at byte s * 4 * kFpWG + 4 * threadIdx.x; slots
+// 2..B+1 hold the bins, slots 0, 1, B+2 and B+3 absorb the terms of voxels that
+// land outside the window.
+struct WalkLane {
+    float ct, cxt, inv_c, cx, ocx, fnx;
+    float k_lo, k_hi;  // px = x0-1 and px = x0+B bound the voxels that reach the window
+    float mbase;
+    unsigned nvalid;   // in_nx - 1
+    int lo, hi;        // x0 - 2, x0 + B: clamp of floor(px)
+    unsigned qoff;     // byte offset of the slot of floor(px) = 0 (mod 2^32)
+    int last;
+};
+
+__device__ __forceinline__ WalkLane walk_lane(const FpParams& P, float4 g, int x0, int B) {
+    WalkLane L;
+    L.ct = g.x;
+    L.cxt = g.z;
+    L.inv_c = 1.0f / (g.x * g.z);
+    L.cx = P.cx;
+    L.ocx = P.ocx;
+    L.fnx = static_cast<float>(P.out_nx);
+    L.k_lo = static_cast<float>(x0 - 1) - P.cx;
+    L.k_hi = static_cast<float>(x0 + B) - P.cx;
+    // Bound on |computed - exact| of px and of the inverted window ends, in px
+    // units: a few roundings of magnitudes below in_nx + out_nx + |zc|.
+    L.mbase = 3e-7f * (4.f * L.fnx + 5.f * static_cast<float>(P.in_nx) + 64.f);
+    L.nvalid = static_cast<unsigned>(P.in_nx - 1);
+    L.lo = x0 - 2;
+    L.hi = x0 + B;
+    L.qoff = 4u * threadIdx.x + static_cast<unsigned>(2 - x0) * (4u * kFpWG);
+    L.last = P.out_nx - 1;
+    return L;
+}
+
+// Voxel window of z row zc for the lane: [e0, e1) holds every voxel whose
+// floor(px) lies in [x0-1, x0+B-1] (plus at most a few that land in the dummy
+// slots).
+__device__ __forceinline__ void walk_window(const WalkLane& L, float zcv, int& e0, int& e1) {
+    const float margin = (L.mbase + 1.8e-6f * fabsf(zcv)) * L.inv_c + 1e-3f;
+    const float tlo = (L.k_lo - zcv) * L.inv_c + L.ocx;
+    const float thi = (L.k_hi - zcv) * L.inv_c + L.ocx;
+    e0 = static_cast<int>(fminf(fmaxf(ceilf(tlo - margin), 0.f), L.fnx));
+    e1 = static_cast<int>(fminf(fmaxf(floorf(thi + margin) + 1.f, 0.f), L.fnx));
+}
+
+// One voxel, main.rs:860-887 with px0 = floor(px): v*(1-frac) to bin px0 and
+// v*frac to bin px0+1.  A voxel outside 0 <= px0 < in_nx-1 adds nothing in the
+// reference; here it adds +0.0 (its value is replaced by 0), which leaves a sum
+// that started at +0.0 bitwise unchanged.  frac = px - floor(px) is exact, so
+// v_fract gives the reference's value wherever px >= 0.
+template <int B>
+__device__ __forceinline__ void walk_voxel(const WalkLane& L, char* lds, float dx, float zcv,
+                                           float v) {
+    const float px = ((dx * L.ct) * L.cxt + zcv) + L.cx;
+    const int fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    const float w = static_cast<unsigned>(fi) < L.nvalid ? v : 0.0f;
+    int m;  // clamp(fi, x0-2, x0+B) in one instruction
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(m) : "v"(fi), "v"(L.lo), "v"(L.hi));
+    float* q = reinterpret_cast<float*>(lds + (static_cast<unsigned>(m) * (4u * kFpWG) + L.qoff));
+    q[0] += w * (1.0f - frac);
+    q[kFpWG] += w * frac;
+}
+
+// Walk of one z row.  STAGED: vz is the row in LDS; otherwise vz is the row in
+// HBM and the next voxel is fetched one step ahead.
+template <int B, bool STAGED>
+__device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
+                                         const float* __restrict__ vz, float zcv) {
+    int e0, e1;
+    walk_window(L, zcv, e0, e1);
+    // dx = ox - ocx is exact in f32 and advances exactly by 1 per voxel
+    float dx = static_cast<float>(e0) - L.ocx;
+    const float dx_end = static_cast<float>(e1) - L.ocx;
+    const float* vp = vz + e0;
+    if constexpr (STAGED) {
+        // the staged row is padded, so the read one past e1 <= out_nx stays in it
+        float vn = *vp;
+        for (; dx < dx_end; dx += 1.0f) {
+            const float v = vn;
+            vn = *++vp;
+            walk_voxel<B>(L, lds, dx, zcv, v);
+        }
+    } else {
+        const float* vlast = vz + L.last;
+        float vn = *(vp < vlast ? vp : vlast);
+        for (; dx < dx_end; dx += 1.0f) {
+            const float v = vn;
+            ++vp;
+            vn = *(vp < vlast ? vp : vlast);
+            walk_voxel<B>(L, lds, dx, zcv, v);
+        }
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void walk_store(const FpParams& P, const float* a, int x0, int iy,
+                                           int p) {
+    float* row = P.sino + (static_cast<int64_t>(p) * P.in_ny + iy) * P.in_nx;
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+        if (x0 + j < P.in_nx) row[x0 + j] = a[(j + 2) * kFpWG];
+}
+
+// Unstaged form: each wave walks its own (view, row) reading the volume from
+// HBM, or scans bin by bin for views with cos_t*cos_xt < kWalkMinC.
+template <int B>
+__device__ __forceinline__ void fp_walk_global(const FpParams& P, float* lds, int x0, int iy,
+                                               int p) {
+    float* a = lds + threadIdx.x;
+    const float4 g = P.vgeom[p];
+    if (!(g.x * g.z >= kWalkMinC)) {
+        float* row = P.sino + (static_cast<int64_t>(p) * P.in_ny + iy) * P.in_nx;
+        for (int j = 0; j < B; ++j)
+            if (x0 + j < P.in_nx) fp_bin_scan(P, x0 + j, iy, p, row + x0 + j);
+        return;
+    }
+    const WalkLane L = walk_lane(P, g, x0, B);
+#pragma unroll
+    for (int s = 2; s < B + 2; ++s) a[s * kFpWG] = 0.f;
+    const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
+    const float* vrow = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx;
+    float zcv = zc[0];
+    for (int oz = 0; oz < P.out_nz; ++oz) {
+        const float zc_next = zc[min(oz + 1, P.out_nz - 1)];
+        walk_row<B, false>(L, reinterpret_cast<char*>(lds),
+                           vrow + static_cast<int64_t>(oz) * P.out_nx, zcv);
+        zcv = zc_next;
+    }
+    walk_store<B>(P, a, x0, iy, p);
+}
+
+// Rows of at most kStageNx voxels are staged in LDS: the 4 waves of a workgroup
+// walk 4 views of the same volume row, which the workgroup reads from HBM once,
+// coalesced, one z row ahead (registers) and double-buffered (LDS).
+constexpr int kStageRegs = 16;
+constexpr int kStageNx = kStageRegs * kFpWG;
+__host__ __device__ constexpr int fp_stage_stride(int nx) { return nx + 4; }
+
+template <int B>
+__device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int blk) {
+    // blk = ((iy * nvg) + vg) * nbx + bx: the view groups of one row are
+    // adjacent, so the blocks that read one volume slice run together.
+    const int nbx = (P.in_nx + 64 * B - 1) / (64 * B);
+    const int nvg = (P.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
+    const int bx = blk % nbx;
+    const int rest = blk / nbx;
+    const int vg = rest % nvg;
+    const int iy = rest / nvg;
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    // wave-uniform: view tables are read through the scalar cache
+    const int p = __builtin_amdgcn_readfirstlane(vg * kFpViewsPerWG + wave);
+    const int x0 = (bx * 64 + lane) * B;
+    float* a = lds + threadIdx.x;
+    // workgroup-uniform choice of the staged form
+    bool staged = P.out_nx <= kStageNx;
+    for (int w = 0; w < kFpViewsPerWG; ++w) {
+        const int pw = vg * kFpViewsPerWG + w;
+        if (pw < P.nviews) {
+            const float4 gw = P.vgeom[pw];
+            if (!(gw.x * gw.z >= kWalkMinC)) staged = false;
+        }
+    }
+    if (!staged) {
+        if (p >= P.nviews || x0 >= P.in_nx) return;
+        fp_walk_global<B>(P, lds, x0, iy, p);
+        return;
+    }
+    // Every thread takes part in staging and barriers; only live lanes walk.
+    const bool wave_live = p < P.nviews;
+    const bool live = wave_live && x0 < P.in_nx;
+    const int pv = wave_live ? p : 0;
+    const float4 g = P.vgeom[pv];
+    const WalkLane L = walk_lane(P, g, x0, B);
+#pragma unroll
+    for (int s = 2; s < B + 2; ++s) a[s * kFpWG] = 0.f;
+    const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    const int nx = P.out_nx;
+    const int nz = P.out_nz;
+    const int t = static_cast<int>(threadIdx.x);
+    float* stg = lds + (B + 4) * kFpWG;
+    const int ns = fp_stage_stride(nx);
+    const float* vrow = P.vol + static_cast<int64_t>(iy) * nz * nx;
+    float r[kStageRegs];
+    auto fetch = [&](int oz) {
+        const float* src = vrow + static_cast<int64_t>(oz) * nx;
+#pragma unroll
+        for (int k = 0; k < kStageRegs; ++k)
+            if (t + k * kFpWG < nx) r[k] = src[t + k * kFpWG];
+    };
+    auto put = [&](int buf) {
+        float* dst = stg + buf * ns;
+#pragma unroll
+        for (int k = 0; k < kStageRegs; ++k)
+            if (t + k * kFpWG < nx) dst[t + k * kFpWG] = r[k];
+    };
+    fetch(0);
+    put(0);
+    if (nz > 1) fetch(1);
+    __syncthreads();
+    float zcv = zc[0];
+    for (int oz = 0; oz < nz; ++oz) {
+        const float zc_next = zc[min(oz + 1, nz - 1)];
+        if (oz + 1 < nz) put((oz + 1) & 1);
+        if (oz + 2 < nz) fetch(oz + 2);
+        if (live) walk_row<B, true>(L, reinterpret_cast<char*>(lds), stg + (oz & 1) * ns, zcv);
+        zcv = zc_next;
+        __syncthreads();
+    }
+    if (live) walk_store<B>(P, a, x0, iy, p);
+}
+
+__device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk) {
+    switch (P.bins_per_lane) {
+        case 4: fp_walk_block<4>(P, lds, blk); break;
+        case 8: fp_walk_block<8>(P, lds, blk); break;
+        default: fp_walk_block<16>(P, lds, blk); break;
+    }
+}
+
+__global__ void __launch_bounds__(kFpWG) fp_walk_kernel(FpParams P, int nblk) {
+    extern __shared__ float fp_lds[];
+    fp_block(P, fp_lds, xcd_remap(blockIdx.x, nblk));
+}
+
+// Several series in one launch; a workgroup's work and order are those of a
+// launch of its own.
+__global__ void __launch_bounds__(kFpWG) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
+    extern __shared__ float fp_lds[];
+    const int b = xcd_remap(blockIdx.x, nblk);
+    int lo = 0, hi = Bt.nreq - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) / 2;
+        if (Bt.block_start[m] <= b)
+            lo = m;
+        else
+            hi = m - 1;
+    }
+    fp_block(Bt.descs[lo], fp_lds, b - Bt.block_start[lo]);
+}
+
+// ---------------------------------------------------------------- two-row
+__device__ __forceinline__ void fb_find_bracket(const float* pos, int len, float val, int& idx,
+                                                float& frac) {
+    idx = 0;
+    frac = 0.f;
+    if (len == 1) return;
+    int lo = 0, hi = len - 1;
+    if (val <= pos[lo]) return;
+    if (val >= pos[hi]) {
+        idx = hi;
+        return;
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (pos[mid] <= val)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const float span = pos[hi] - pos[lo];
+    idx = lo;
+    frac = span > 1e-12f ? (val - pos[lo]) / span : 0.f;
+}
+
+// ViewPatches::interpolate (main.rs:166-224) on the device tables.
+__device__ bool fp_local_shift(const FpScatterParams& P, int view, float x, float y, float& odx,
+                               float& ody) {
+    const int nx = P.lnx[view], ny = P.lny[view];
+    odx = 0.f;
+    ody = 0.f;
+    if (nx == 0) return false;
+    if (ny == 0) return true;
+    const float* xs = P.lxs + P.lxoff[view];
+    const float* ys = P.lys + P.lyoff[view];
+    const float* gx = P.ldx + P.lgoff[view];
+    const float* gy = P.ldy + P.lgoff[view];
+    int ix0, iy0;
+    float fx, fy;
+    fb_find_bracket(xs, nx, x, ix0, fx);
+    fb_find_bracket(ys, ny, y, iy0, fy);
+    const int ix1 = min(ix0 + 1, nx - 1);
+    const int iy1 = min(iy0 + 1, ny - 1);
+    const float d00x = gx[iy0 * nx + ix0], d00y = gy[iy0 * nx + ix0];
+    const float d10x = gx[iy0 * nx + ix1], d10y = gy[iy0 * nx + ix1];
+    const float d01x = gx[iy1 * nx + ix0], d01y = gy[iy1 * nx + ix0];
+    const float d11x = gx[iy1 * nx + ix1], d11y = gy[iy1 * nx + ix1];
+    const float ax = d00x + fx * (d10x - d00x), bx = d01x + fx * (d11x - d01x);
+    const float ay = d00y + fx * (d10y - d00y), by = d01y + fx * (d11y - d01y);
+    odx = ax + fy * (bx - ax);
+    ody = ay + fy * (by - ay);
+    return true;
+}
+
+__global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
+    const int ox = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    const int p = blockIdx.z;
+    if (ox >= P.out_nx) return;
+    const float4 g = P.vgeom[p];
+    const float dx = static_cast<float>(ox) - P.ocx;
+    const float a = (dx * g.x) * g.z;
+    const float yo = dx * g.w;
+    float ldx = 0.f, ldy = 0.f;
+    bool shifted = false;
+    if (P.has_local)
+        shifted = fp_local_shift(P, p, static_cast<float>(ox), static_cast<float>(iy), ldx, ldy);
+    float sy = static_cast<float>(iy) + yo;
+    if (shifted) sy = sy + ldy;
+    const float fl_y = floorf(sy);
+    const float fy = sy - fl_y;
+    const float fny = static_cast<float>(P.in_ny);
+    const bool two = P.has_local || fabsf(yo) > 1e-6f;
+    int mode = 0;  // 0 none, 1 single row, 2 two rows
+    if (fl_y >= 0.f && fl_y + 1.f < fny && two)
+        mode = 2;
+    else if (fl_y >= 0.f && fl_y < fny)
+        mode = 1;
+    if (mode == 0) return;
+    const int sy0 = static_cast<int>(fl_y);
+    const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
+    const float fn = static_cast<float>(P.in_nx);
+    const float* vcol = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx + ox;
+    float* r0 = P.sino + (static_cast<int64_t>(p) * P.in_ny + sy0) * P.in_nx;
+    float* r1 = r0 + P.in_nx;
+    const float w0 = 1.0f - fy, w1 = fy;
+    for (int oz = 0; oz < P.out_nz; ++oz) {
+        float px = (a + zc[oz]) + P.cx;
+        if (shifted) px = px + ldx;
+        const float fl = floorf(px);
+        if (!(fl >= 0.f && fl + 1.f < fn)) continue;
+        const int px0 = static_cast<int>(fl);
+        const float frac = px - fl;
+        const float val = vcol[static_cast<int64_t>(oz) * P.out_nx];
+        if (mode == 2) {
+            atomicAdd(r0 + px0, val * (1.0f - frac) * w0);
+            atomicAdd(r0 + px0 + 1, val * frac * w0);
+            atomicAdd(r1 + px0, val * (1.0f - frac) * w1);
+            atomicAdd(r1 + px0 + 1, val * frac * w1);
+        } else {
+            atomicAdd(r0 + px0, val * (1.0f - frac));
+            atomicAdd(r0 + px0 + 1, val * frac);
+        }
+    }
+}
+
+}  // namespace
+
+size_t fp_lds_bytes(const FpParams& p) {
+    size_t b = static_cast<size_t>(p.bins_per_lane + 4) * kFpWG * sizeof(float);
+    if (p.out_nx <= kStageNx) b += 2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float);
+    return b;
+}
+
+int fp_bins_per_lane(int in_nx) {
+    if (in_nx <= 256) return 4;
+    if (in_nx <= 512) return 8;
+    return 16;
+}
+
+int fp_gather_blocks(const FpParams& p) {
+    const int B = p.bins_per_lane;
+    const int nbx = (p.in_nx + 64 * B - 1) / (64 * B);
+    const int nvg = (p.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
+    return nbx * nvg * p.in_ny;
+}
+
+hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {
+    if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
+    if (p.bins_per_lane != 4 && p.bins_per_lane != 8 && p.bins_per_lane != 16)
+        return hipErrorInvalidValue;
+    const int nblk = fp_gather_blocks(p);
+    hipLaunchKernelGGL(fp_walk_kernel, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_t lds,
+                                       hipStream_t st) {
+    if (total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fp_walk_batch_kernel, dim3(total_blocks), dim3(kFpWG), lds, st, b,
+                       total_blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st) {
+    if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
+    dim3 grid((p.out_nx + 255) / 256, p.in_ny, p.nviews);
+    hipLaunchKernelGGL(fp_scatter_kernel, grid, dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace tiltgpu

@@ -1,6 +1,6 @@
 // Memory-bound helper kernels for gfx950: --log and per-view means, FP16/BF16
-// storage conversion, forward projection (SIRT), output transform, fake-SIRT
-// low-pass, per-section statistics.
+// storage conversion, output transform, fake-SIRT low-pass, per-section
+// statistics, FP64 partial-sum reduction.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <algorithm>
@@ -132,215 +132,21 @@ hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
     return hipGetLastError();
 }
 
-// ------------------------------------------------- forward projection (SIRT)
-// forward_project, main.rs:831-896, in gather form for the single-row geometry
-// (no x-axis tilt, no local alignment).  One lane owns one sinogram bin
-// (view, iy, x) and visits the voxels whose floor(proj_x) is x-1 or x, in the
-// reference's per-bin order (oz ascending, then ox ascending), adding
-// val*frac / val*(1-frac) with the reference's rounding: the result equals
-// the serial scatter bit for bit.
-__device__ __forceinline__ void fp_gather_bin(const FpParams& P, int x, int iy, int p) {
-    if (x >= P.in_nx) return;
-    const float4 g = P.vgeom[p];
-    const float c = g.x * g.z;
-    const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
-    const float* vrow = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx;
-    const float fx = static_cast<float>(x);
-    const float fn = static_cast<float>(P.in_nx);
-    float acc = 0.f;
-    for (int oz = 0; oz < P.out_nz; ++oz) {
-        const float zcv = zc[oz];
-        int lo = 0, hi = P.out_nx - 1;
-        if (fabsf(c) > 1e-4f) {
-            const float t0 = P.ocx + ((fx - 1.0f) - P.cx - zcv) / c;
-            const float t1 = P.ocx + ((fx + 1.0f) - P.cx - zcv) / c;
-            const float tl = fminf(t0, t1), th = fmaxf(t0, t1);
-            const float fnx = static_cast<float>(P.out_nx);
-            lo = static_cast<int>(fminf(fmaxf(floorf(tl) - 1.0f, 0.0f), fnx));
-            hi = static_cast<int>(fmaxf(fminf(ceilf(th) + 1.0f, fnx - 1.0f), -1.0f));
-        }
-        const float* vz = vrow + static_cast<int64_t>(oz) * P.out_nx;
-        for (int ox = lo; ox <= hi; ++ox) {
-            const float dx = static_cast<float>(ox) - P.ocx;
-            const float px = ((dx * g.x) * g.z + zcv) + P.cx;
-            const float fl = floorf(px);
-            if (fl >= 0.f && fl + 1.f < fn) {
-                const float frac = px - fl;
-                if (fl == fx)
-                    acc += vz[ox] * (1.0f - frac);
-                else if (fl == fx - 1.0f)
-                    acc += vz[ox] * frac;
-            }
-        }
-    }
-    P.sino[(static_cast<int64_t>(p) * P.in_ny + iy) * P.in_nx + x] = acc;
-}
-
-__global__ void __launch_bounds__(256) fp_gather_kernel(FpParams P) {
-    fp_gather_bin(P, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, blockIdx.z);
-}
-
-// Several series in one launch: workgroup b of series r covers 256 bins of one
-// (view, row); per-bin work and order are those of a launch of its own.
-__global__ void __launch_bounds__(256) fp_gather_batch_kernel(FpBatch B) {
-    int a = 0, c = B.nreq - 1;
-    const int b = blockIdx.x;
-    while (a < c) {
-        const int m = (a + c + 1) / 2;
-        if (B.block_start[m] <= b)
-            a = m;
-        else
-            c = m - 1;
-    }
-    const FpParams& P = B.descs[a];
-    const int local = b - B.block_start[a];
-    const int nbx = (P.in_nx + 255) / 256;
-    const int bx = local % nbx;
-    const int rest = local / nbx;
-    fp_gather_bin(P, bx * 256 + static_cast<int>(threadIdx.x), rest % P.in_ny, rest / P.in_ny);
-}
-
-int fp_gather_blocks(const FpParams& p) {
-    return ((p.in_nx + 255) / 256) * p.in_ny * p.nviews;
-}
-
-hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, hipStream_t st) {
-    if (total_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fp_gather_batch_kernel, dim3(total_blocks), dim3(256), 0, st, b);
-    return hipGetLastError();
-}
-
-hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {
-    dim3 grid((p.in_nx + 255) / 256, p.in_ny, p.nviews);
-    if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(256), 0, st, p);
-    return hipGetLastError();
-}
-
-// Two-row geometry (x-axis tilt / local alignment): scatter with float atomics.
-// Same coordinates and weights as main.rs:867-889; the summation order per bin
-// is not the reference's (results agree to rounding, not bit for bit).
-__device__ __forceinline__ void fb_find_bracket(const float* pos, int len, float val, int& idx,
-                                                float& frac) {
-    idx = 0;
-    frac = 0.f;
-    if (len == 1) return;
-    int lo = 0, hi = len - 1;
-    if (val <= pos[lo]) return;
-    if (val >= pos[hi]) {
-        idx = hi;
-        return;
-    }
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) / 2;
-        if (pos[mid] <= val)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const float span = pos[hi] - pos[lo];
-    idx = lo;
-    frac = span > 1e-12f ? (val - pos[lo]) / span : 0.f;
-}
-
-__device__ bool fp_local_shift(const FpScatterParams& P, int view, float x, float y,
-                               float& odx, float& ody) {
-    const int nx = P.lnx[view], ny = P.lny[view];
-    odx = 0.f;
-    ody = 0.f;
-    if (nx == 0) return false;
-    if (ny == 0) return true;
-    const float* xs = P.lxs + P.lxoff[view];
-    const float* ys = P.lys + P.lyoff[view];
-    const float* gx = P.ldx + P.lgoff[view];
-    const float* gy = P.ldy + P.lgoff[view];
-    int ix0, iy0;
-    float fx, fy;
-    fb_find_bracket(xs, nx, x, ix0, fx);
-    fb_find_bracket(ys, ny, y, iy0, fy);
-    const int ix1 = min(ix0 + 1, nx - 1);
-    const int iy1 = min(iy0 + 1, ny - 1);
-    const float d00x = gx[iy0 * nx + ix0], d00y = gy[iy0 * nx + ix0];
-    const float d10x = gx[iy0 * nx + ix1], d10y = gy[iy0 * nx + ix1];
-    const float d01x = gx[iy1 * nx + ix0], d01y = gy[iy1 * nx + ix0];
-    const float d11x = gx[iy1 * nx + ix1], d11y = gy[iy1 * nx + ix1];
-    const float ax = d00x + fx * (d10x - d00x), bx = d01x + fx * (d11x - d01x);
-    const float ay = d00y + fx * (d10y - d00y), by = d01y + fx * (d11y - d01y);
-    odx = ax + fy * (bx - ax);
-    ody = ay + fy * (by - ay);
-    return true;
-}
-
-__global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
-    const int ox = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y;
-    const int p = blockIdx.z;
-    if (ox >= P.out_nx) return;
-    const float4 g = P.vgeom[p];
-    const float dx = static_cast<float>(ox) - P.ocx;
-    const float a = (dx * g.x) * g.z;
-    const float yo = dx * g.w;
-    float ldx = 0.f, ldy = 0.f;
-    bool shifted = false;
-    if (P.has_local)
-        shifted = fp_local_shift(P, p, static_cast<float>(ox), static_cast<float>(iy), ldx, ldy);
-    float sy = static_cast<float>(iy) + yo;
-    if (shifted) sy = sy + ldy;
-    const float fl_y = floorf(sy);
-    const float fy = sy - fl_y;
-    const float fny = static_cast<float>(P.in_ny);
-    const bool two = P.has_local || fabsf(yo) > 1e-6f;
-    int mode = 0;  // 0 none, 1 single row, 2 two rows
-    if (fl_y >= 0.f && fl_y + 1.f < fny && two)
-        mode = 2;
-    else if (fl_y >= 0.f && fl_y < fny)
-        mode = 1;
-    if (mode == 0) return;
-    const int sy0 = static_cast<int>(fl_y);
-    const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
-    const float fn = static_cast<float>(P.in_nx);
-    const float* vcol = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx + ox;
-    float* r0 = P.sino + (static_cast<int64_t>(p) * P.in_ny + sy0) * P.in_nx;
-    float* r1 = r0 + P.in_nx;
-    const float w0 = 1.0f - fy, w1 = fy;
-    for (int oz = 0; oz < P.out_nz; ++oz) {
-        float px = (a + zc[oz]) + P.cx;
-        if (shifted) px = px + ldx;
-        const float fl = floorf(px);
-        if (!(fl >= 0.f && fl + 1.f < fn)) continue;
-        const int px0 = static_cast<int>(fl);
-        const float frac = px - fl;
-        const float val = vcol[static_cast<int64_t>(oz) * P.out_nx];
-        if (mode == 2) {
-            atomicAdd(r0 + px0, val * (1.0f - frac) * w0);
-            atomicAdd(r0 + px0 + 1, val * frac * w0);
-            atomicAdd(r1 + px0, val * (1.0f - frac) * w1);
-            atomicAdd(r1 + px0 + 1, val * frac * w1);
-        } else {
-            atomicAdd(r0 + px0, val * (1.0f - frac));
-            atomicAdd(r0 + px0 + 1, val * frac);
-        }
-    }
-}
-
-hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st) {
-    if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
-    dim3 grid((p.out_nx + 255) / 256, p.in_ny, p.nviews);
-    hipLaunchKernelGGL(fp_scatter_kernel, grid, dim3(256), 0, st, p);
-    return hipGetLastError();
-}
-
 // ----------------------------------------------------------- small helpers
-__global__ void sum_partials_kernel(const double* partial, int n, double* out, int sq) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// FP64 sum of n partials in a fixed order (per-thread strided sums, then a
+// tree): the same inputs give the same bits on every run.
+__global__ void __launch_bounds__(256) sum_partials_kernel(const double* partial, int n,
+                                                           double* out, int sq) {
+    __shared__ double red[256];
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += partial[i];
-    *out = sq ? sqrt(s) : s;
+    for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *out = sq ? sqrt(s) : s;
 }
 
 hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
                                hipStream_t st) {
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, n, out,
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, n, out,
                        sqrt_result);
     return hipGetLastError();
 }

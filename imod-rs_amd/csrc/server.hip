@@ -269,16 +269,18 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
             auto* hp = reinterpret_cast<FpParams*>(h);
             auto* hb = reinterpret_cast<int*>(h + off_b);
             int total = 0;
+            size_t lds = 0;
             for (int i = 0; i < n; ++i) {
                 hp[i] = g[i];
                 hb[i] = total;
                 total += fp_gather_blocks(g[i]);
+                lds = std::max(lds, fp_lds_bytes(g[i]));
             }
             hb[n] = total;
             HIP_TRY(hipMemcpyAsync(S->fp_bufs.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
             FpBatch b{reinterpret_cast<const FpParams*>(S->fp_bufs.dev),
                       reinterpret_cast<const int*>(static_cast<char*>(S->fp_bufs.dev) + off_b), n};
-            HIP_TRY(launch_forward_gather_batch(b, total, S->stream));
+            HIP_TRY(launch_forward_gather_batch(b, total, lds, S->stream));
         }
     }
     // 3. filter: new series (R-weighting + stretch) and SIRT residuals O - R

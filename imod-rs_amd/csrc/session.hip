@@ -382,6 +382,7 @@ FpParams make_fp(tiltgpu_session* s) {
     p.nz_pad = s->nz_pad;
     p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
     p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
+    p.bins_per_lane = fp_bins_per_lane(s->d.in_nx);
     return p;
 }
 
@@ -691,14 +692,30 @@ int tiltgpu_run(tiltgpu_session* s) {
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
                                                          : s->store;
     const int out_ny = s->d.in_ny;
-    const bool fused_transform = s->o.sirt == 0 && s->o.fake_sirt == 0;
+    const int trows = s->trace_y1 - s->trace_y0;
+    const int64_t plane = static_cast<int64_t>(s->d.out_nz) * s->d.out_nx;
+    if (trows > 0) {
+        dev_free(s, s->trace);
+        s->trace = nullptr;
+        if ((rc = dev_alloc(s, &s->trace, (s->o.sirt + 1) * trows * plane))) return rc;
+    }
+    auto keep_trace = [&](int k) -> int {
+        if (trows <= 0) return TILTGPU_OK;
+        HIP_TRY(hipMemcpyAsync(s->trace + k * trows * plane, s->vol + s->trace_y0 * plane,
+                               trows * plane * sizeof(float), hipMemcpyDeviceToDevice,
+                               s->stream));
+        return TILTGPU_OK;
+    };
+    const bool fused_transform = s->o.sirt == 0 && s->o.fake_sirt == 0 && trows <= 0;
     if ((rc = run_bp(s, proj, s->scale, 0, out_ny, fused_transform ? 1 : 0, s->vol))) return rc;
+    if ((rc = keep_trace(0))) return rc;
     // SIRT (main.rs:919-953)
     s->sirt_done = 0;
     for (int it = 0; it < s->o.sirt; ++it) {
         if ((rc = run_forward(s))) return rc;
         if ((rc = run_filter(s, O, s->sino, false, s->resid + it))) return rc;
         if ((rc = run_bp(s, proj, s->scale, 0, out_ny, 2, s->vol))) return rc;
+        if ((rc = keep_trace(it + 1))) return rc;
         s->sirt_done++;
     }
     // fake-SIRT low-pass (main.rs:955-962, 976-985)
@@ -754,6 +771,41 @@ int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n) {
     const int k = std::min<int>(n, s->sirt_done);
     if (k > 0) HIP_TRY(hipMemcpy(out, s->resid, k * sizeof(double), hipMemcpyDeviceToHost));
     return k;
+}
+
+int tiltgpu_set_sirt_trace(tiltgpu_session* s, int32_t y0, int32_t y1) {
+    if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    if (y0 < 0 || y1 < y0 || y1 > s->d.in_ny) return fail(TILTGPU_ERR_ARG, "bad row range");
+    s->trace_y0 = y0;
+    s->trace_y1 = y1;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_read_sirt_trace(tiltgpu_session* s, int32_t k, float* out) {
+    if (!s || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    const int trows = s->trace_y1 - s->trace_y0;
+    if (trows <= 0 || !s->trace) return fail(TILTGPU_ERR_STATE, "no SIRT trace recorded");
+    if (k < 0 || k > s->sirt_done) return fail(TILTGPU_ERR_ARG, "trace %d not recorded", k);
+    const int64_t n = static_cast<int64_t>(trows) * s->d.out_nz * s->d.out_nx;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->trace + k * n, n * sizeof(float), hipMemcpyDeviceToHost));
+    return TILTGPU_OK;
+}
+
+int tiltgpu_reproject(tiltgpu_session* s, const float* vol, float* out) {
+    if (!s || !vol || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    if (s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    const int64_t nvol = static_cast<int64_t>(s->d.in_ny) * s->d.out_nz * s->d.out_nx;
+    if (!s->sino && (rc = dev_alloc(s, &s->sino, nproj))) return rc;
+    HIP_TRY(hipMemcpyAsync(s->vol, vol, nvol * 4, hipMemcpyHostToDevice, s->stream));
+    if ((rc = run_forward(s))) return rc;
+    HIP_TRY(hipMemcpyAsync(out, s->sino, nproj * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    return check_kernel_error(s);
 }
 
 int tiltgpu_read_filtered(tiltgpu_session* s, float* out) {

@@ -75,6 +75,8 @@ struct tiltgpu_session {
     float2* tw = nullptr;
     int* err = nullptr;
     double* resid = nullptr;
+    float* trace = nullptr;  // [sirt+1][trace_y1-trace_y0][out_nz][out_nx]
+    int trace_y0 = 0, trace_y1 = 0;
     float* stats = nullptr;
     int stats_rows = 0;
     int* d_lnx = nullptr;

@@ -12,6 +12,23 @@ constexpr int kBpThreads = 256;  // 4 waves per backprojection workgroup
 constexpr int kBpTileX = 64;     // one wave spans 64 output x positions
 constexpr int kMaxRadix = 20;
 
+// floor(x) as an int in one instruction (|x| well inside the int range).
+__device__ __forceinline__ int cvt_floor(float x) {
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
+// Workgroup id -> logical work item.  Workgroups are dispatched round-robin over
+// the 8 XCDs; giving each XCD a contiguous run of logical items keeps items that
+// read the same rows (projection rows, volume slices) on one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    const int X = 8;
+    const int q = nblk / X, r = nblk % X;
+    const int xcd = bid % X, j = bid / X;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + j;
+}
+
 // Per-view constants of the backprojection, read through the scalar cache.
 // shift_mode: 0 = no local shift, 1 = constant shift (sdx, sdy) (a one-patch
 // grid: ViewPatches::interpolate returns its single value), 2 = patch grid
@@ -84,7 +101,8 @@ struct FilterParams {
     int radix[kMaxRadix];
 };
 
-// Gather-form forward projection (no x-axis tilt, no local alignment).
+// Ordered-walk forward projection (no x-axis tilt, no local alignment).
+// Layout: vol [in_ny][out_nz][out_nx] f32, sino [nz][in_ny][in_nx] f32.
 struct FpParams {
     const float* vol;     // [ny][out_nz][out_nx]
     float* sino;          // [nz][ny][in_nx]
@@ -92,6 +110,7 @@ struct FpParams {
     const float* zc_tab;
     int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
     float cx, ocx;
+    int bins_per_lane;    // 4, 8 or 16: fp_bins_per_lane(in_nx)
 };
 
 // Scatter-form forward projection (x-axis tilt and/or local alignment).
@@ -146,8 +165,11 @@ hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
                                   unsigned* absmax_bits, int64_t nxy, int nz, int storage,
                                   hipStream_t st);
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st);
-hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, hipStream_t st);
+hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_t lds,
+                                       hipStream_t st);
+size_t fp_lds_bytes(const FpParams& p);  // dynamic LDS of one series' workgroups
 int fp_gather_blocks(const FpParams& p);
+int fp_bins_per_lane(int in_nx);
 hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st);
 hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
                                hipStream_t st);

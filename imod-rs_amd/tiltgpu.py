@@ -112,6 +112,9 @@ SIGNATURES = [
     ("tiltgpu_run", C.c_int, [_vp]),
     ("tiltgpu_read_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
     ("tiltgpu_sirt_residuals", C.c_int, [_vp, _dp, C.c_int32]),
+    ("tiltgpu_reproject", C.c_int, [_vp, _fp, _fp]),
+    ("tiltgpu_set_sirt_trace", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("tiltgpu_read_sirt_trace", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_read_filtered", C.c_int, [_vp, _fp]),
     ("tiltgpu_read_means", C.c_int, [_vp, _fp]),
     ("tiltgpu_synchronize", C.c_int, [_vp]),
@@ -250,6 +253,7 @@ class Session:
         self.out_nx = self.opts.width or in_nx
         self.out_nz = self.opts.thickness if self.opts.thickness > 0 else in_nx
         self.dims = Dims(in_nx, in_ny, nz, self.out_nx, self.out_nz)
+        self._trace_rows = (0, 0)
         self._copts = self.opts.to_c()
         h = _vp()
         _check(lib().tiltgpu_create(C.byref(self.dims), C.byref(self._copts), C.byref(h)))
@@ -310,6 +314,30 @@ class Session:
         k = _check(lib().tiltgpu_sirt_residuals(self.h, out.ctypes.data_as(_dp), out.size))
         return out[:k]
 
+    def set_sirt_trace(self, y0, y1):
+        """Record rows [y0, y1) after the WBP and every SIRT iteration of run()."""
+        _check(lib().tiltgpu_set_sirt_trace(self.h, y0, y1))
+        self._trace_rows = (y0, y1)
+
+    def sirt_trace(self):
+        """[sirt_done+1][rows][out_nz][out_nx] recorded by the last run()."""
+        y0, y1 = self._trace_rows
+        k = self.residuals().size
+        out = np.empty((k + 1, y1 - y0, self.out_nz, self.out_nx), np.float32)
+        for i in range(k + 1):
+            _check(lib().tiltgpu_read_sirt_trace(self.h, i, _f(out[i])))
+        return out
+
+    def reproject(self, vol):
+        """Forward projection of vol [in_ny][out_nz][out_nx] (main.rs:830-896)."""
+        vol = np.ascontiguousarray(vol, np.float32)
+        if vol.shape != (self.dims.in_ny, self.out_nz, self.out_nx):
+            raise ValueError(f"volume shape {vol.shape} != "
+                             f"{(self.dims.in_ny, self.out_nz, self.out_nx)}")
+        out = np.empty((self.dims.nz, self.dims.in_ny, self.dims.in_nx), np.float32)
+        _check(lib().tiltgpu_reproject(self.h, _f(vol), _f(out)))
+        return out
+
     def read_filtered(self):
         out = np.empty((self.dims.nz, self.dims.in_ny, self.dims.in_nx), np.float32)
         _check(lib().tiltgpu_read_filtered(self.h, _f(out)))
@@ -348,9 +376,11 @@ def header_stats(row_stats, per_row):
 
 
 def reconstruct(P, angles, opts: TiltOptions | None = None, xtilt=None, zfactor=None,
-                local=None, return_session=False):
+                local=None, return_session=False, trace_rows=None):
     """The whole reconstruction on the GPU.  P: raw projections [nz][ny][nx];
-    local: (nx, ny, xs, ys, dx, dy) flat per-view grids or an object with .flat()."""
+    local: (nx, ny, xs, ys, dx, dy) flat per-view grids or an object with .flat();
+    trace_rows=(y0, y1) records those rows after every SIRT iteration
+    (Session.sirt_trace, with return_session=True)."""
     P = _f32(P)
     nz, ny, nx = P.shape
     s = Session(nx, ny, nz, opts)
@@ -361,6 +391,8 @@ def reconstruct(P, angles, opts: TiltOptions | None = None, xtilt=None, zfactor=
         s.set_zfactor(zfactor)
     if local is not None:
         s.set_local(*(local.flat() if hasattr(local, "flat") else local))
+    if trace_rows is not None:
+        s.set_sirt_trace(*trace_rows)
     s.upload(P)
     s.run()
     vol, st = s.read_rows()

@@ -152,6 +152,23 @@ int tiltgpu_read_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out,
  * device (a build addition: the reference computes none, main.rs:928-953). */
 int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n);
 
+/* Keep output rows [y0, y1) of the volume after the initial WBP and after every
+ * SIRT iteration of the next tiltgpu_run (before the output transform), for
+ * per-iteration monitoring of the iteration of apps/tilt/src/main.rs:927-953,
+ * which reports only the iteration count (:951).  y0 == y1 turns it off.  Holds
+ * (sirt+1) x (y1-y0) x out_nz x out_nx floats of device memory. */
+int tiltgpu_set_sirt_trace(tiltgpu_session* s, int32_t y0, int32_t y1);
+/* Copy trace k (0 = initial WBP, k = after iteration k) as [y1-y0][out_nz][out_nx]. */
+int tiltgpu_read_sirt_trace(tiltgpu_session* s, int32_t k, float* out);
+
+/* Forward projection of a caller-supplied volume [in_ny][out_nz][out_nx] with the
+ * session's geometry into out [nz][in_ny][in_nx]: the reprojection step of the
+ * SIRT loop, the `forward_project` closure of apps/tilt/src/main.rs:830-896.
+ * Bit-identical to that closure's serial scatter for the single-row geometry
+ * (no x-axis tilt, no local alignment); equal to rounding otherwise.  Replaces
+ * the session's volume; sections beyond the tilt angles come out zero. */
+int tiltgpu_reproject(tiltgpu_session* s, const float* vol, float* out);
+
 /* Copy the filtered (and stretched) projections back, dequantised to FP32,
  * [nz][in_ny][in_nx]; for parity tests of the filter stage. */
 int tiltgpu_read_filtered(tiltgpu_session* s, float* out);

@@ -70,11 +70,13 @@ def _declare(L):
         getattr(L, f"or_tilt_run_rows_{suf}").argtypes = [
             C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
             _ip, _ip, _fp, _fp, _fp, _fp, C.c_int, C.c_int, _fp, _fp]
+        getattr(L, f"or_sirt_trace_{suf}").argtypes = [
+            C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, C.c_int, C.c_int, _fp, _dp]
         getattr(L, f"or_backproject_simple_{suf}").argtypes = [
             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
             _fp, _fp, _fp, C.c_int, C.c_int, _fp, C.c_int]
         getattr(L, f"or_forward_project_simple_{suf}").argtypes = [
-            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]
+            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp, C.c_int]
         getattr(L, f"or_rweight_float_{suf}").argtypes = [
             _fp, C.c_long, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
         getattr(L, f"or_cosine_stretch_float_{suf}").argtypes = [
@@ -305,14 +307,14 @@ def backproject(P, ptab, means, out_nx, out_nz, fill="mean", y0=0, y1=None,
     return out
 
 
-def forward_project(V, ptab, in_nx, nz, prec="f32"):
+def forward_project(V, ptab, in_nx, nz, prec="f32", nthreads=8):
     V = _f32c(V)
     ny, out_nz, out_nx = V.shape
     ptab = np.ascontiguousarray(ptab, np.float32)
     nviews = ptab.shape[1]
     R = np.zeros((nz, ny, in_nx), np.float32)
     getattr(lib(), f"or_forward_project_simple_{prec}")(
-        in_nx, ny, nz, nviews, out_nx, out_nz, _f(ptab), _f(V), _f(R))
+        in_nx, ny, nz, nviews, out_nx, out_nz, _f(ptab), _f(V), _f(R), nthreads)
     return R
 
 
@@ -350,6 +352,38 @@ class TiltOptions:
     fake_sirt: int = 0
 
 
+def _or_opts(opts, nx, ny, nz, out_nx, out_nz, nthreads):
+    return OrOpts(nx, ny, nz, out_nx, out_nz, opts.radial[0], opts.radial[1],
+                  int(opts.no_weight), int(opts.exact_filter_size or 0), int(opts.hamming),
+                  int(opts.no_cosine_stretch), int(opts.cosine_interp.lower() == "cubic"),
+                  opts.max_cosine_tilt, int(opts.log), FILL.get(opts.fill_mode.lower(), 0),
+                  opts.densweight, opts.outscale, opts.outadd, int(opts.sirt),
+                  opts.sirt_relax, int(opts.fake_sirt), nthreads)
+
+
+def sirt_trace(P, angles, opts: TiltOptions, rows, zfactor=None, prec="f64", nthreads=8):
+    """SIRT on output rows (y0, y1) of the single-row geometry (no x-axis tilt, no
+    local alignment), with the volume after the initial WBP and after each of the
+    opts.sirt iterations: returns (trace [sirt+1][rows][out_nz][out_nx] float32,
+    resid [sirt] float64 = |O - FP(V)| over those rows)."""
+    P = np.array(P, dtype=np.float32, copy=True)
+    nz, ny, nx = P.shape
+    ang = _f32c(angles)
+    out_nx = opts.width if opts.width else nx
+    out_nz = opts.thickness if opts.thickness > 0 else nx
+    if not opts.no_cosine_stretch and ang.size < nz:
+        raise IndexError("fewer tilt angles than sections (main.rs:450 would panic)")
+    y0, y1 = rows
+    o = _or_opts(opts, nx, ny, nz, out_nx, out_nz, nthreads)
+    trace = np.zeros((opts.sirt + 1, y1 - y0, out_nz, out_nx), np.float32)
+    resid = np.zeros(max(opts.sirt, 1), np.float64)
+    zf = _f32c(zfactor)
+    getattr(lib(), f"or_sirt_trace_{prec}")(
+        C.byref(o), _f(P), _f(ang), ang.size, _f(zf), 0 if zf is None else zf.size, y0, y1,
+        _f(trace), resid.ctypes.data_as(_dp))
+    return trace, resid[:opts.sirt]
+
+
 def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
              prec="f32", nthreads=8, rows=None):
     """The whole reconstruction (main.rs:601-1093 after reading).  P: [nz][ny][nx]
@@ -362,12 +396,7 @@ def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
     out_nz = opts.thickness if opts.thickness > 0 else nx
     if not opts.no_cosine_stretch and ang.size < nz:
         raise IndexError("fewer tilt angles than sections (main.rs:450 would panic)")
-    o = OrOpts(nx, ny, nz, out_nx, out_nz, opts.radial[0], opts.radial[1],
-               int(opts.no_weight), int(opts.exact_filter_size or 0), int(opts.hamming),
-               int(opts.no_cosine_stretch), int(opts.cosine_interp.lower() == "cubic"),
-               opts.max_cosine_tilt, int(opts.log), FILL.get(opts.fill_mode.lower(), 0),
-               opts.densweight, opts.outscale, opts.outadd, int(opts.sirt),
-               opts.sirt_relax, int(opts.fake_sirt), nthreads)
+    o = _or_opts(opts, nx, ny, nz, out_nx, out_nz, nthreads)
     y0, y1 = (0, ny) if rows is None else rows
     vol = np.zeros((y1 - y0, out_nz, out_nx), np.float32)
     stats = np.zeros(3, np.float32)

@@ -464,13 +464,23 @@ void F(or_backproject)(const F(or_geom) * g, const REAL* P, int y0, int y1, REAL
 /* forward_project, main.rs:831-896: the adjoint scatter of the BP interpolation,
  * serial, loop order iy -> view -> oz -> ox.  V = [in_ny][out_nz][out_nx],
  * R = [nz][in_ny][in_nx] (zeroed here). */
-void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R) {
+typedef struct {
+    const F(or_geom) * g;
+    const REAL* V;
+    REAL* R;
+} F(fp_ctx);
+
+/* Views [v0, v1) of forward_project.  A view writes only its own sinogram
+ * section, and within a view the loop order is the reference's (iy, oz, ox), so
+ * splitting the views over threads leaves every sum bitwise unchanged. */
+static void F(fp_views)(void* vctx, long v0, long v1) {
+    F(fp_ctx)* c = (F(fp_ctx)*)vctx;
+    const F(or_geom)* g = c->g;
     int in_nx = g->in_nx, in_ny = g->in_ny;
     long nxy = (long)in_nx * in_ny;
-    memset(R, 0, sizeof(REAL) * (size_t)g->nz * nxy);
     for (int iy = 0; iy < in_ny; iy++) {
-        const REAL* slice = V + (long)iy * g->out_nz * g->out_nx;
-        for (int pi = 0; pi < g->nviews; pi++) {
+        const REAL* slice = c->V + (long)iy * g->out_nz * g->out_nx;
+        for (int pi = (int)v0; pi < (int)v1; pi++) {
             for (int oz = 0; oz < g->out_nz; oz++) {
                 for (int ox = 0; ox < g->out_nx; ox++) {
                     REAL px, sy;
@@ -483,7 +493,7 @@ void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R) {
                     long sy0 = (long)floor(sy);
                     REAL fy = sy - (REAL)sy0;
                     if (sy0 >= 0 && sy0 + 1 < in_ny && two) {
-                        REAL* r0 = R + pi * nxy + sy0 * in_nx;
+                        REAL* r0 = c->R + pi * nxy + sy0 * in_nx;
                         REAL* r1 = r0 + in_nx;
                         REAL w0 = (REAL)1 - fy, w1 = fy;
                         r0[px0] += val * ((REAL)1 - frac) * w0;
@@ -491,7 +501,7 @@ void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R) {
                         r1[px0] += val * ((REAL)1 - frac) * w1;
                         r1[px0 + 1] += val * frac * w1;
                     } else if (sy0 >= 0 && sy0 < in_ny) {
-                        REAL* r0 = R + pi * nxy + sy0 * in_nx;
+                        REAL* r0 = c->R + pi * nxy + sy0 * in_nx;
                         r0[px0] += val * ((REAL)1 - frac);
                         r0[px0 + 1] += val * frac;
                     }
@@ -499,6 +509,13 @@ void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R) {
             }
         }
     }
+}
+
+/* forward_project, main.rs:830-896 (loop order iy, view, oz, ox). */
+void F(or_forward_project)(const F(or_geom) * g, const REAL* V, REAL* R, int nthreads) {
+    memset(R, 0, sizeof(REAL) * (size_t)g->nz * g->in_nx * g->in_ny);
+    F(fp_ctx) c = {g, V, R};
+    or_parallel_for(g->nviews, 1, nthreads, F(fp_views), &c);
 }
 
 /* lowpass_3d_volume, main.rs:493-538: one separable 3x3x3 box pass (X, Z, Y)
@@ -560,6 +577,25 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
                         lxs, lys, ldx, ldy, 0, o->in_ny, vol, stats);
 }
 
+/* Per-view means of the (logged) projections, main.rs:615-623 (min_max_mean's
+ * f64 sum, rounded to f32 in the FP32 oracle). */
+static REAL* F(view_means)(const or_opts* o, const float* P_in) {
+    long nxy = (long)o->in_nx * o->in_ny;
+    REAL* means = (REAL*)malloc(sizeof(REAL) * (size_t)(o->nz > 0 ? o->nz : 1));
+    for (int p = 0; p < o->nz; p++) {
+        float mmm[3];
+        or_min_max_mean(P_in + p * nxy, nxy, mmm);
+#if REAL_IS_DOUBLE
+        double s = 0;
+        for (long i = 0; i < nxy; i++) s += P_in[p * nxy + i];
+        means[p] = nxy ? s / (double)nxy : 0;
+#else
+        means[p] = mmm[2];
+#endif
+    }
+    return means;
+}
+
 /* The same for output rows [y0, y1) only (WBP: rows are independent); SIRT and
  * fake-SIRT always reconstruct every row first.  vol = [y1-y0][out_nz][out_nx]. */
 void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int n_angles,
@@ -574,18 +610,7 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
             float v = P_in[i] > 0.001f ? P_in[i] : 0.001f;
             P_in[i] = log10f(v);
         }
-    REAL* means = (REAL*)malloc(sizeof(REAL) * (size_t)(o->nz > 0 ? o->nz : 1));
-    for (int p = 0; p < o->nz; p++) {
-        float mmm[3];
-        or_min_max_mean(P_in + p * nxy, nxy, mmm);
-#if REAL_IS_DOUBLE
-        double s = 0;
-        for (long i = 0; i < nxy; i++) s += P_in[p * nxy + i];
-        means[p] = nxy ? s / (double)nxy : 0;
-#else
-        means[p] = mmm[2];
-#endif
-    }
+    REAL* means = F(view_means)(o, P_in);
     REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)total);
     for (long i = 0; i < total; i++) P[i] = (REAL)P_in[i];
     REAL* O = NULL;
@@ -649,7 +674,7 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
         REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)total);
         REAL* D = (REAL*)malloc(sizeof(REAL) * (size_t)(out_ny * nv > 0 ? out_ny * nv : 1));
         for (int it = 0; it < o->sirt; it++) {
-            F(or_forward_project)(&g, V, R);
+            F(or_forward_project)(&g, V, R, o->nthreads);
             for (long i = 0; i < total; i++) R[i] = O[i] - R[i];
             if (!o->no_weight)
                 F(or_rweight)(R, (long)o->nz * o->in_ny, o->in_nx, o->cutoff, o->falloff,
@@ -691,6 +716,94 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
     free(lgoff);
 }
 
+/* SIRT (main.rs:919-953) on output rows [y0, y1) only, keeping the volume after
+ * the initial WBP and after every iteration: trace = [sirt+1][y1-y0][out_nz]
+ * [out_nx] (before the output transform), and resid[it] = the L2 norm of
+ * O - FP(V) over those rows before iteration it's R-weighting.
+ *
+ * Only for the single-row geometry (no x-axis tilt, no local alignment): there
+ * row iy of the volume projects into row iy of every view and back, the
+ * R-weighting and the cosine stretch act along rows, so the rows evolve
+ * independently and a row subset equals those rows of the whole run.  The
+ * per-view means (fill values) come from the full images, as in the whole run.
+ * P_in is the full [nz][in_ny][in_nx] series (logged in place with --log). */
+void F(or_sirt_trace)(const or_opts* o, float* P_in, const float* angles, int n_angles,
+                      const float* zfac, int n_zfac, int y0, int y1, float* trace,
+                      double* resid) {
+    long nxy = (long)o->in_nx * o->in_ny;
+    long total = nxy * o->nz;
+    if (o->log_)
+        for (long i = 0; i < total; i++) {
+            float v = P_in[i] > 0.001f ? P_in[i] : 0.001f;
+            P_in[i] = log10f(v);
+        }
+    REAL* means = F(view_means)(o, P_in);
+    const int rows = y1 - y0;
+    const long sub_xy = (long)o->in_nx * rows;
+    const long sub = sub_xy * o->nz;
+    REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)(sub > 0 ? sub : 1));
+    for (int p = 0; p < o->nz; p++)
+        for (long i = 0; i < sub_xy; i++) P[p * sub_xy + i] = (REAL)P_in[p * nxy + y0 * (long)o->in_nx + i];
+    REAL* O = (REAL*)malloc(sizeof(REAL) * (size_t)(sub > 0 ? sub : 1));
+    memcpy(O, P, sizeof(REAL) * (size_t)sub);
+    if (!o->no_weight)
+        F(or_rweight)(P, (long)o->nz * rows, o->in_nx, o->cutoff, o->falloff,
+                      o->exact_filter_size, o->hamming, o->nthreads);
+    if (!o->no_cosine_stretch)
+        F(or_cosine_stretch)(P, o->in_nx, rows, o->nz, angles, o->cosine_cubic,
+                             o->max_cosine_tilt);
+    int nviews = n_angles < o->nz ? n_angles : o->nz;
+    REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews > 0 ? nviews : 1));
+    F(or_proj_params)(angles, nviews, NULL, 0, zfac, n_zfac, tab, tab + nviews,
+                      tab + 2 * nviews, tab + 3 * nviews, tab + 4 * nviews);
+    F(or_geom) g;
+    memset(&g, 0, sizeof(g));
+    g.in_nx = o->in_nx;
+    g.in_ny = rows;
+    g.nz = o->nz;
+    g.nviews = nviews;
+    g.out_nx = o->out_nx;
+    g.out_nz = o->out_nz;
+    g.fill_mode = o->fill_mode;
+    g.cos_t = tab;
+    g.sin_t = tab + nviews;
+    g.cos_xt = tab + 2 * nviews;
+    g.sin_xt = tab + 3 * nviews;
+    g.zf = tab + 4 * nviews;
+    g.means = means;
+    const long nv = (long)o->out_nx * o->out_nz;
+    const long nvol = nv * rows;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(nvol > 0 ? nvol : 1));
+    REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)(sub > 0 ? sub : 1));
+    REAL* D = (REAL*)malloc(sizeof(REAL) * (size_t)(nvol > 0 ? nvol : 1));
+    F(or_backproject)(&g, P, 0, rows, V, o->nthreads);
+    for (long i = 0; i < nvol; i++) trace[i] = (float)V[i];
+    for (int it = 0; it < o->sirt; it++) {
+        F(or_forward_project)(&g, V, R, o->nthreads);
+        double ss = 0;
+        for (long i = 0; i < sub; i++) {
+            R[i] = O[i] - R[i];
+            ss += (double)R[i] * (double)R[i];
+        }
+        if (resid) resid[it] = sqrt(ss);
+        if (!o->no_weight)
+            F(or_rweight)(R, (long)o->nz * rows, o->in_nx, o->cutoff, o->falloff,
+                          o->exact_filter_size, o->hamming, o->nthreads);
+        F(or_backproject)(&g, R, 0, rows, D, o->nthreads);
+        REAL relax = (REAL)o->sirt_relax;
+        for (long i = 0; i < nvol; i++) V[i] += relax * D[i];
+        float* dst = trace + (long)(it + 1) * nvol;
+        for (long i = 0; i < nvol; i++) dst[i] = (float)V[i];
+    }
+    free(D);
+    free(R);
+    free(V);
+    free(tab);
+    free(O);
+    free(P);
+    free(means);
+}
+
 /* Thin wrappers so tests can drive BP / FP on caller-given float arrays. */
 void F(or_backproject_simple)(int in_nx, int in_ny, int nz, int nviews, int out_nx,
                               int out_nz, int fill_mode, const float* ptab /* 5*nviews */,
@@ -718,7 +831,7 @@ void F(or_backproject_simple)(int in_nx, int in_ny, int nz, int nviews, int out_
 
 void F(or_forward_project_simple)(int in_nx, int in_ny, int nz, int nviews, int out_nx,
                                   int out_nz, const float* ptab, const float* V_in,
-                                  float* R_out) {
+                                  float* R_out, int nthreads) {
     long nvol = (long)in_ny * out_nx * out_nz;
     long total = (long)in_nx * in_ny * nz;
     REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(nvol ? nvol : 1));
@@ -733,7 +846,7 @@ void F(or_forward_project_simple)(int in_nx, int in_ny, int nz, int nviews, int 
     g.out_nx = out_nx; g.out_nz = out_nz; g.fill_mode = 1;
     g.cos_t = tab; g.sin_t = tab + nviews; g.cos_xt = tab + 2 * nviews;
     g.sin_xt = tab + 3 * nviews; g.zf = tab + 4 * nviews; g.means = &zero;
-    F(or_forward_project)(&g, V, R);
+    F(or_forward_project)(&g, V, R, nthreads);
     for (long i = 0; i < total; i++) R_out[i] = (float)R[i];
     free(tab); free(R); free(V);
 }
