@@ -112,9 +112,84 @@ def cpu_baseline(P, ang, xt, local, rows=32):
                       f"projection rows +-{halo} halo, filter+stretch+BP, {dt:.1f} s"}
 
 
+# --------------------------------------------------------------------- SIRT
+
+def c2_inputs(tg, nx=1024, thickness=300, n_views=61, seed=2, nthreads=16):
+    """SIRT configuration: 1024x1024 projections, 61 views -60..60 in 2 deg
+    steps, thickness 300, the phantom of the 512x512x128 case scaled to the size
+    (400 ellipsoids, 60 beads), noise sigma 0.1 x stack std."""
+    ang = np.linspace(-60.0, 60.0, n_views).astype(np.float32)
+    P = tg.synth_series(nx, nx, ang, vol_nz=thickness, n_ellipsoids=400, n_beads=60,
+                        seed=seed, noise_rel=0.1, nthreads=nthreads)
+    return P, ang
+
+
+def rel_errors(x, ref):
+    ref = ref.astype(np.float64)
+    d = x.astype(np.float64) - ref
+    rms = float(np.sqrt(np.mean(ref ** 2)))
+    return float(np.max(np.abs(d))) / rms, float(np.sqrt(np.mean(d ** 2))) / rms
+
+
+def sirt_section(tg, iters=20, rows=(504, 520), oracle=True):
+    """SIRT-20 of the 1024^2 x 61 series: FP32 and BF16 projection storage (per-view
+    scale, FP32 accumulation); FP64 residual norms from the GPU; every iterate of
+    output rows [504, 520) against the FP64 oracle's iterates of those rows."""
+    t0 = time.perf_counter()
+    P, ang = c2_inputs(tg)
+    print(f"# SIRT inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    nv, ny, nx = P.shape
+    th = 300
+    vv = float(nx) * ny * th * nv
+    out = {"config": {"workload": f"SIRT-{iters} 1024x1024 x61 views (-60..60, 2 deg) -> "
+                                  f"1024x1024x300, relax 1.0", "traced_rows": list(rows)}}
+    ref = None
+    if oracle:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O  # noqa: E402 -- checker for the error figures only
+        t1 = time.perf_counter()
+        ref, ref_res = O.sirt_trace(P, ang, O.TiltOptions(thickness=th, sirt=iters), rows,
+                                    prec="f64", nthreads=min(16, os.cpu_count() or 1))
+        out["oracle_f64_seconds"] = round(time.perf_counter() - t1, 2)
+        print(f"# FP64 oracle trace: {out['oracle_f64_seconds']} s", file=sys.stderr, flush=True)
+    for storage in ("f32", "bf16"):
+        opts = tg.TiltOptions(thickness=th, sirt=iters, sirt_relax=1.0, storage=storage)
+        s = tg.Session(nx, ny, nv, opts)
+        s.set_angles(ang)
+        s.set_sirt_trace(*rows)
+        s.upload(P)
+        s.run()  # warm-up
+        s.synchronize()
+        s.reset_timing()
+        t1 = time.perf_counter()
+        s.run()
+        dt = time.perf_counter() - t1
+        tm = s.timing()
+        res = s.residuals()
+        tr = s.sirt_trace()
+        s.close()
+        # work: the initial WBP plus one FP and one BP per iteration
+        work = vv * (1 + 2 * iters)
+        entry = {"seconds": round(dt, 4), "ms_per_iteration": round((dt * 1e3 - tm["prep_ms"]) / (iters + 0.5), 3),
+                 "gvoxel_view_per_s": round(work / dt / 1e9, 1),
+                 "breakdown_ms": {"fp": round(tm["fp_ms"], 2), "bp": round(tm["bp_ms"], 2),
+                                  "filter": round(tm["filter_ms"], 2), "other": round(tm["other_ms"] + tm["prep_ms"], 2)},
+                 "fp_launch_ms": round(tm["fp_ms"] / max(1, tm["fp_launches"]), 3),
+                 "bp_launch_ms": round(tm["bp_ms"] / max(1, tm["bp_launches"]), 3),
+                 "residual_norm_f64": [float(f"{r:.9g}") for r in res]}
+        if ref is not None:
+            errs = [rel_errors(tr[k], ref[k]) for k in range(iters + 1)]
+            entry["error_vs_f64_oracle"] = {
+                "per_iterate_max_abs_over_rms": [float(f"{e[0]:.3g}") for e in errs],
+                "per_iterate_rms_over_rms": [float(f"{e[1]:.3g}") for e in errs]}
+        out[storage] = entry
+        print(f"# SIRT {storage}: {dt:.3f} s, {entry['gvoxel_view_per_s']} Gvv/s", file=sys.stderr, flush=True)
+    return out
+
+
 # ------------------------------------------------------------------ serving
 
-def c3_series(tg, n_distinct=16, seed=3000, nthreads=16):
+def c3_series(tg, n_distinct=64, seed=3000, nthreads=16):
     """Series of the batched configuration: width in {256, 512, 768, 1024}
     (square), views uniform 41..121 over +-60 deg, thickness uniform 64..300,
     the seeded phantom (ellipsoid count scaled with the area), noise 0.1."""
@@ -271,7 +346,10 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-only", action="store_true", help="only the serving section")
-    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--stream-requests", type=int, default=24)
+    ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
+    ap.add_argument("--sirt-only", action="store_true", help="only the SIRT section")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
     a = ap.parse_args()
@@ -295,8 +373,11 @@ def main():
         print(json.dumps({"step_s": dt, **tm}), flush=True)
         return
     if a.serve_only:
-        print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.serve_requests)}),
+        print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.stream_requests)}),
               flush=True)
+        return
+    if a.sirt_only:
+        print(json.dumps({"sirt": sirt_section(tg, oracle=not a.no_cpu)}), flush=True)
         return
     t0 = time.perf_counter()
     P, ang, xt, local = c1_inputs(tg)
@@ -356,9 +437,12 @@ def main():
         }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
-    if not a.no_serve:
+    if not a.no_sirt or not a.no_serve:
         del P
-        line["serving"] = serving_section(tg, a.serve_requests, a.serve_requests)
+    if not a.no_sirt:
+        line["sirt"] = sirt_section(tg, oracle=not a.no_cpu)
+    if not a.no_serve:
+        line["serving"] = serving_section(tg, a.serve_requests, a.stream_requests)
     print(json.dumps(line), flush=True)
 
 

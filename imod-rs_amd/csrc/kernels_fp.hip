@@ -295,11 +295,22 @@ __global__ void __launch_bounds__(kFpWG) fp_walk_kernel(FpParams P, int nblk) {
     fp_block(P, fp_lds, xcd_remap(blockIdx.x, nblk));
 }
 
+__host__ __device__ inline int fp_blocks(const FpParams& p) {
+    const int B = p.bins_per_lane;
+    const int nbx = (p.in_nx + 64 * B - 1) / (64 * B);
+    const int nvg = (p.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
+    return nbx * nvg * p.in_ny;
+}
+
 // Several series in one launch; a workgroup's work and order are those of a
-// launch of its own.
+// launch of its own.  Series r owns workgroups [block_start[r], block_start[r+1]),
+// a range that starts at a multiple of 8 and is padded to one, so that its
+// workgroups spread evenly over the 8 XCDs (a contiguous run of each series per
+// XCD); padding workgroups exit at once.
 __global__ void __launch_bounds__(kFpWG) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
     extern __shared__ float fp_lds[];
-    const int b = xcd_remap(blockIdx.x, nblk);
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
     int lo = 0, hi = Bt.nreq - 1;
     while (lo < hi) {
         const int m = (lo + hi + 1) / 2;
@@ -308,7 +319,12 @@ __global__ void __launch_bounds__(kFpWG) fp_walk_batch_kernel(FpBatch Bt, int nb
         else
             hi = m - 1;
     }
-    fp_block(Bt.descs[lo], fp_lds, b - Bt.block_start[lo]);
+    const FpParams& P = Bt.descs[lo];
+    const int local = b - Bt.block_start[lo];
+    const int per = (Bt.block_start[lo + 1] - Bt.block_start[lo]) / 8;
+    const int logical = (local % 8) * per + local / 8;
+    if (logical >= fp_blocks(P)) return;
+    fp_block(P, fp_lds, logical);
 }
 
 // ---------------------------------------------------------------- two-row
@@ -430,12 +446,7 @@ int fp_bins_per_lane(int in_nx) {
     return 16;
 }
 
-int fp_gather_blocks(const FpParams& p) {
-    const int B = p.bins_per_lane;
-    const int nbx = (p.in_nx + 64 * B - 1) / (64 * B);
-    const int nvg = (p.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
-    return nbx * nvg * p.in_ny;
-}
+int fp_gather_blocks(const FpParams& p) { return fp_blocks(p); }
 
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {
     if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;

@@ -2,14 +2,21 @@
 //
 // Requests -- whole reconstructions (WBP or SIRT) of series of any shape, and
 // row slabs of series kept resident in HBM -- are queued by any host thread.
-// One worker thread owns the device.  It admits queued requests in FIFO order
-// while their device memory fits the budget, then runs rounds.  A round issues
-// one batched launch per stage across every request in flight:
+// A scheduler thread admits queued requests in FIFO order while their device
+// memory fits the budget and issues rounds.  A round is one batched launch per
+// stage across every request in flight:
 //   forward projection (SIRT iterations) -> filter (new series: R-weighting +
 //   stretch; SIRT: residual O - R) -> backprojection (persistent, per-XCD
 //   tile queues over the tiles of all requests).
-// Requests that arrive while a round is on the device join the next round.  A
+// Requests that arrive while rounds are on the device join the next round.  A
 // WBP request needs one round, a SIRT-k request k+1, a slab request one.
+//
+// The host never waits for a round before issuing the next one (up to kSlots
+// rounds are in flight): admission work -- buffers from a device-buffer cache
+// that never calls hipFree on the hot path, geometry tables, the projection
+// upload on a stream of its own -- overlaps the rounds already on the device.
+// Finisher threads wait for each round, copy finished results to the callers'
+// buffers (one copy stream each) and return their buffers to the cache.
 //
 // Tile shapes and every per-tile computation depend only on the request, so a
 // request's result is bit-identical alone and in any mix.
@@ -20,6 +27,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -37,13 +45,18 @@ using Clock = std::chrono::steady_clock;
 
 enum { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3 };
 enum { kRecon = 0, kSlab = 1, kSeries = 2 };
+constexpr int kSlots = 3;  // rounds in flight
+constexpr size_t kAdmitMax = 8;              // admissions per round issued
+constexpr int64_t kAdmitBytes = 2ll << 30;   // projection bytes uploaded per round issued
 
 struct Req {
     int64_t id = 0;
     int kind = kRecon;
     tiltgpu_dims d{};
     tiltgpu_options o{};
-    std::vector<float> angles, xtilt, zfac, proj;
+    std::vector<float> angles, xtilt, zfac;
+    const float* proj = nullptr;  // caller's projections, read at admission
+    size_t proj_n = 0;
     bool has_xtilt = false, has_zfac = false, has_local = false;
     std::vector<int32_t> lnx, lny;
     std::vector<float> lxs, lys, ldx, ldy;
@@ -52,6 +65,7 @@ struct Req {
     int64_t need = 0;
     tiltgpu_session* s = nullptr;
     float* slab = nullptr;  // device slab output
+    size_t slab_bytes = 0;
     int phase = 0;          // 0: first round; k: SIRT iteration k
     int state = kQueued;
     int rc = 0;
@@ -76,18 +90,138 @@ struct BatchBufs {
     size_t cap = 0;
 };
 
+// Grows only when a round needs more descriptor space than any before it (the
+// slot's previous round has completed by then).
 int ensure_batch(BatchBufs& b, size_t bytes) {
     if (bytes <= b.cap) return TILTGPU_OK;
     if (b.dev) (void)hipFree(b.dev);
     if (b.host) (void)hipHostFree(b.host);
     b.dev = b.host = nullptr;
     b.cap = 0;
-    const size_t cap = std::max<size_t>(bytes, 1 << 16);
+    const size_t cap = std::max<size_t>(bytes, 1 << 20);
     if (hipMalloc(&b.dev, cap) != hipSuccess || hipHostMalloc(&b.host, cap) != hipSuccess)
         return fail(TILTGPU_ERR_HIP, "server: descriptor buffer allocation failed");
     b.cap = cap;
     return TILTGPU_OK;
 }
+
+// Device-buffer cache.  Sizes are rounded up to quarter-octave classes so that
+// buffers of similar requests are reused.  A returned buffer becomes reusable
+// once the newest round issued at its return has completed, so no reuse can
+// overtake work still reading or writing it.  hipFree (which synchronises the
+// device) runs only to trim the cache when it would exceed its cap.
+class DevCache : public BufferPool {
+public:
+    int64_t cap = 0;
+
+    void* get(size_t bytes) override {
+        std::lock_guard<std::mutex> lk(mu_);
+        reclaim();
+        const size_t cls = size_class(bytes);
+        auto it = free_.lower_bound(cls);
+        if (it != free_.end() && it->first <= cls + cls / 2) {
+            void* p = it->second;
+            live_[p] = it->first;
+            live_bytes_ += static_cast<int64_t>(it->first);
+            cached_bytes_ -= static_cast<int64_t>(it->first);
+            free_.erase(it);
+            return p;
+        }
+        while (!free_.empty() && live_bytes_ + cached_bytes_ + static_cast<int64_t>(cls) > cap) {
+            auto last = std::prev(free_.end());
+            (void)hipFree(last->second);
+            cached_bytes_ -= static_cast<int64_t>(last->first);
+            free_.erase(last);
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, cls) != hipSuccess) {
+            for (auto& kv : free_) (void)hipFree(kv.second);
+            for (auto& kv : free_) cached_bytes_ -= static_cast<int64_t>(kv.first);
+            free_.clear();
+            if (hipMalloc(&p, cls) != hipSuccess) return nullptr;
+        }
+        live_[p] = cls;
+        live_bytes_ += static_cast<int64_t>(cls);
+        return p;
+    }
+
+    void put(void* p, size_t) override {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        auto it = live_.find(p);
+        if (it == live_.end()) return;
+        pending_.push_back({p, it->second, enq_seq_});
+        live_bytes_ -= static_cast<int64_t>(it->second);
+        cached_bytes_ += static_cast<int64_t>(it->second);
+        live_.erase(it);
+    }
+
+    void set_enqueued(uint64_t seq) {
+        std::lock_guard<std::mutex> lk(mu_);
+        enq_seq_ = seq;
+    }
+    void set_done(uint64_t seq) {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_seq_ = std::max(done_seq_, seq);
+    }
+    void release_all() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto& kv : free_) (void)hipFree(kv.second);
+        for (auto& e : pending_) (void)hipFree(e.p);
+        for (auto& kv : live_) (void)hipFree(kv.first);
+        free_.clear();
+        pending_.clear();
+        live_.clear();
+        live_bytes_ = cached_bytes_ = 0;
+    }
+
+private:
+    struct Pending {
+        void* p;
+        size_t cls;
+        uint64_t seq;
+    };
+    static size_t size_class(size_t bytes) {
+        if (bytes <= (64u << 10)) return 64u << 10;
+        int k = 63 - __builtin_clzll(static_cast<unsigned long long>(bytes));
+        const size_t step = static_cast<size_t>(1) << (k - 2);
+        return (bytes + step - 1) / step * step;
+    }
+    void reclaim() {
+        size_t w = 0;
+        for (size_t i = 0; i < pending_.size(); ++i) {
+            if (pending_[i].seq <= done_seq_)
+                free_.emplace(pending_[i].cls, pending_[i].p);
+            else
+                pending_[w++] = pending_[i];
+        }
+        pending_.resize(w);
+    }
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+    std::vector<Pending> pending_;
+    std::unordered_map<void*, size_t> live_;
+    int64_t live_bytes_ = 0, cached_bytes_ = 0;
+    uint64_t enq_seq_ = 0, done_seq_ = 0;
+};
+
+struct Slot {
+    BatchBufs bp, filt, fp;
+    hipEvent_t done = nullptr;
+    uint64_t seq = 0;  // round last issued with this slot (0: never)
+};
+
+// A request in its last round, handed to a finisher thread; the round's end
+// event is shared by the requests of that round.
+struct FinItem {
+    std::shared_ptr<hipEvent_t> ev;
+    uint64_t seq = 0;
+    int rc = 0;
+    std::string msg;
+    Req* r = nullptr;
+};
+
+constexpr int kFinishers = 4;  // result copies of pageable buffers proceed in parallel
 
 }  // namespace
 
@@ -95,17 +229,24 @@ struct tiltgpu_server {
     int device = 0;
     int n_cu = 256;
     int64_t budget = 0, used = 0;
-    hipStream_t stream = nullptr;
-    std::thread worker;
+    hipStream_t stream = nullptr;     // rounds
+    hipStream_t up_stream = nullptr;  // projection uploads
+    hipStream_t down_stream[kFinishers] = {};  // result copies, one per finisher
+    std::thread worker, finisher[kFinishers];
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    bool stop = false;
+    std::condition_variable cv_work, cv_done, cv_fin;
+    bool stop = false, fin_stop = false;
     std::deque<Req*> queue;
     std::vector<Req*> active;
     std::unordered_map<int64_t, Req*> reqs;
     std::vector<Req*> series;  // resident series (kind kSeries, state kDone)
     int64_t next_id = 1;
-    BatchBufs bp_bufs, filt_bufs, fp_bufs;
+    int finishing = 0;         // requests handed to the finisher, not yet released
+    Slot slots[kSlots];
+    uint64_t round_seq = 0;
+    std::deque<FinItem> fin;
+    std::vector<hipEvent_t> events;  // spare events (under mu)
+    DevCache cache;
     // statistics
     std::vector<double> lat_ms;
     int64_t completed = 0, failed = 0, rounds = 0, bp_launches = 0, bp_batched = 0;
@@ -114,21 +255,42 @@ struct tiltgpu_server {
 
 namespace {
 
-void finish(tiltgpu_server* S, Req* r, int rc, const char* msg) {
+hipEvent_t take_event(tiltgpu_server* S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->events.empty()) {
+        hipEvent_t e = S->events.back();
+        S->events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    return e;
+}
+
+void give_event(tiltgpu_server* S, hipEvent_t e) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->events.push_back(e);
+}
+
+void finish(tiltgpu_server*, Req* r, int rc, const char* msg) {
     r->rc = rc;
     if (rc < 0) r->err = msg ? msg : "";
     r->t_done = Clock::now();
 }
 
-// Allocate and fill the device state of an admitted request (worker thread).
+// Device state of an admitted request (scheduler thread): buffers from the
+// cache, geometry, and the projection upload on the upload stream, which the
+// round stream waits for before its next kernels.
 int setup(tiltgpu_server* S, Req* r) {
     if (r->kind == kSlab) {
         Req* sr = S->series[r->series];
         const int64_t per = static_cast<int64_t>(sr->d.out_nz) * sr->d.out_nx;
-        HIP_TRY(hipMalloc(&r->slab, static_cast<size_t>(r->y1 - r->y0) * per * 4));
+        r->slab_bytes = static_cast<size_t>(r->y1 - r->y0) * per * 4;
+        r->slab = static_cast<float*>(S->cache.get(r->slab_bytes));
+        if (!r->slab) return fail(TILTGPU_ERR_BUDGET, "no device memory for the slab");
         return TILTGPU_OK;
     }
-    int rc = session_create(&r->d, &r->o, S->stream, &r->s);
+    int rc = session_create(&r->d, &r->o, S->stream, &r->s, &S->cache);
     if (rc) return rc;
     tiltgpu_session* s = r->s;
     if ((rc = tiltgpu_set_angles(s, r->angles.data(), static_cast<int32_t>(r->angles.size()))))
@@ -145,9 +307,13 @@ int setup(tiltgpu_server* S, Req* r) {
         return rc;
     if ((rc = prepare_pipeline_geometry(s))) return rc;
     if (r->kind == kSeries) dev_free(s, s->vol);  // slab requests bring their own outputs
-    HIP_TRY(hipMemcpyAsync(s->raw, r->proj.data(), r->proj.size() * 4, hipMemcpyHostToDevice,
-                           S->stream));
+    HIP_TRY(hipMemcpyAsync(s->raw, r->proj, r->proj_n * 4, hipMemcpyHostToDevice, S->up_stream));
+    hipEvent_t up = take_event(S);
+    HIP_TRY(hipEventRecord(up, S->up_stream));
+    HIP_TRY(hipStreamWaitEvent(S->stream, up, 0));
+    give_event(S, up);  // reusable: the wait has been enqueued
     s->uploaded = true;
+    r->proj = nullptr;
     return TILTGPU_OK;
 }
 
@@ -166,9 +332,10 @@ struct BpItem {
 };
 
 // One persistent launch per kernel variant; all groups' descriptors go to the
-// device in a single copy (the pinned staging is rewritten only after the
-// round's stream synchronisation).
-int launch_bp_groups(tiltgpu_server* S, std::vector<std::vector<BpItem>>& groups) {
+// device in a single copy from the slot's pinned staging (rewritten only once
+// the slot's previous round has completed).
+int launch_bp_groups(tiltgpu_server* S, std::vector<std::vector<BpItem>>& groups,
+                     BatchBufs& bufs) {
     if (groups.empty()) return TILTGPU_OK;
     struct Layout {
         size_t descs, tiles, counters;
@@ -190,9 +357,9 @@ int launch_bp_groups(tiltgpu_server* S, std::vector<std::vector<BpItem>>& groups
         bytes += 8 * sizeof(int);
         bytes = (bytes + 255) / 256 * 256;
     }
-    int rc = ensure_batch(S->bp_bufs, bytes);
+    int rc = ensure_batch(bufs, bytes);
     if (rc) return rc;
-    auto* h = static_cast<char*>(S->bp_bufs.host);
+    auto* h = static_cast<char*>(bufs.host);
     for (size_t g = 0; g < groups.size(); ++g) {
         Layout& L = lay[g];
         auto* hp = reinterpret_cast<BpParams*>(h + L.descs);
@@ -215,8 +382,8 @@ int launch_bp_groups(tiltgpu_server* S, std::vector<std::vector<BpItem>>& groups
         L.total = total;
         L.lds = lds;
     }
-    HIP_TRY(hipMemcpyAsync(S->bp_bufs.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
-    auto* d = static_cast<char*>(S->bp_bufs.dev);
+    HIP_TRY(hipMemcpyAsync(bufs.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
+    auto* d = static_cast<char*>(bufs.dev);
     for (size_t g = 0; g < groups.size(); ++g) {
         const Layout& L = lay[g];
         BpBatch b{};
@@ -235,7 +402,9 @@ int launch_bp_groups(tiltgpu_server* S, std::vector<std::vector<BpItem>>& groups
     return TILTGPU_OK;
 }
 
-int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
+// Issue one round (no waiting): stages 1-4 for every request in flight, then the
+// finishing work of requests whose last round this is.
+int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
     std::vector<Req*> prep, iter, slabs;
     for (Req* r : act) {
         if (r->state != kActive) continue;
@@ -264,22 +433,24 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
             const int n = static_cast<int>(g.size());
             const size_t off_b = n * sizeof(FpParams);
             const size_t bytes = off_b + (n + 1) * sizeof(int);
-            if ((rc = ensure_batch(S->fp_bufs, bytes))) return rc;
-            auto* h = static_cast<char*>(S->fp_bufs.host);
+            if ((rc = ensure_batch(sl.fp, bytes))) return rc;
+            auto* h = static_cast<char*>(sl.fp.host);
             auto* hp = reinterpret_cast<FpParams*>(h);
             auto* hb = reinterpret_cast<int*>(h + off_b);
+            // each series' workgroup range starts at a multiple of 8 and is
+            // padded to one: its workgroups then spread evenly over the 8 XCDs
             int total = 0;
             size_t lds = 0;
             for (int i = 0; i < n; ++i) {
                 hp[i] = g[i];
                 hb[i] = total;
-                total += fp_gather_blocks(g[i]);
+                total += (fp_gather_blocks(g[i]) + 7) / 8 * 8;
                 lds = std::max(lds, fp_lds_bytes(g[i]));
             }
             hb[n] = total;
-            HIP_TRY(hipMemcpyAsync(S->fp_bufs.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
-            FpBatch b{reinterpret_cast<const FpParams*>(S->fp_bufs.dev),
-                      reinterpret_cast<const int*>(static_cast<char*>(S->fp_bufs.dev) + off_b), n};
+            HIP_TRY(hipMemcpyAsync(sl.fp.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
+            FpBatch b{reinterpret_cast<const FpParams*>(sl.fp.dev),
+                      reinterpret_cast<const int*>(static_cast<char*>(sl.fp.dev) + off_b), n};
             HIP_TRY(launch_forward_gather_batch(b, total, lds, S->stream));
         }
     }
@@ -309,8 +480,8 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
             const int n = static_cast<int>(f.size());
             const size_t off_b = n * sizeof(FilterParams);
             const size_t bytes = off_b + (n + 1) * sizeof(int);
-            if ((rc = ensure_batch(S->filt_bufs, bytes))) return rc;
-            auto* h = static_cast<char*>(S->filt_bufs.host);
+            if ((rc = ensure_batch(sl.filt, bytes))) return rc;
+            auto* h = static_cast<char*>(sl.filt.host);
             auto* hp = reinterpret_cast<FilterParams*>(h);
             auto* hb = reinterpret_cast<int*>(h + off_b);
             int total = 0;
@@ -322,9 +493,9 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
                 lds = std::max(lds, filter_lds_bytes(f[i].n, f[i].pairs_per_wg));
             }
             hb[n] = total;
-            HIP_TRY(hipMemcpyAsync(S->filt_bufs.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
-            FilterBatch b{reinterpret_cast<const FilterParams*>(S->filt_bufs.dev),
-                          reinterpret_cast<const int*>(static_cast<char*>(S->filt_bufs.dev) + off_b),
+            HIP_TRY(hipMemcpyAsync(sl.filt.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
+            FilterBatch b{reinterpret_cast<const FilterParams*>(sl.filt.dev),
+                          reinterpret_cast<const int*>(static_cast<char*>(sl.filt.dev) + off_b),
                           n};
             HIP_TRY(launch_filter_batch(b, total, lds, S->stream));
             for (int i = 0; i < n; ++i) {
@@ -381,20 +552,14 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
             groups.push_back(std::move(grp));
             i = j;
         }
-        if ((rc = launch_bp_groups(S, groups))) return rc;
+        if ((rc = launch_bp_groups(S, groups, sl.bp))) return rc;
     }
-    // 5. advance; finished requests: fake-SIRT, output transform, copy out
+    // 5. advance; requests in their last round: fake-SIRT and output transform
+    // (the finisher copies results out once the round has completed)
     for (Req* r : prep) r->phase = 1;
     for (Req* r : iter) r->phase++;
     for (Req* r : act) {
-        if (r->state != kActive) continue;
-        if (r->kind == kSlab) {
-            Req* sr = S->series[r->series];
-            const int64_t per = static_cast<int64_t>(sr->d.out_nz) * sr->d.out_nx;
-            HIP_TRY(hipMemcpyAsync(r->out, r->slab, (r->y1 - r->y0) * per * 4,
-                                   hipMemcpyDeviceToHost, S->stream));
-            continue;
-        }
+        if (r->state != kActive || r->kind == kSlab) continue;
         tiltgpu_session* s = r->s;
         if (r->phase <= s->o.sirt) continue;  // more SIRT iterations to go
         s->sirt_done = s->o.sirt;
@@ -409,11 +574,7 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act) {
         if (s->o.sirt > 0 || s->o.fake_sirt > 0)
             HIP_TRY(launch_transform(s->vol, static_cast<int64_t>(ny) * onz * onx,
                                      s->o.densweight, s->o.outscale, s->o.outadd, S->stream));
-        HIP_TRY(hipMemcpyAsync(r->out, s->vol, static_cast<size_t>(ny) * onz * onx * 4,
-                               hipMemcpyDeviceToHost, S->stream));
     }
-    HIP_TRY(hipStreamSynchronize(S->stream));
-    S->rounds++;
     return TILTGPU_OK;
 }
 
@@ -422,18 +583,27 @@ bool round_finished(const Req* r) {
     return r->phase > r->s->o.sirt;
 }
 
-void release(tiltgpu_server* S, Req* r) {
-    if (r->kind == kSeries) return;
+// Return a finished request's device state to the cache (finisher thread, after
+// its last round and its result copy have completed).
+void release(tiltgpu_server* S, Req* r, bool keep_series) {
+    if (r->kind == kSeries && keep_series) return;
     if (r->s) {
         tiltgpu_destroy(r->s);
         r->s = nullptr;
     }
     if (r->slab) {
-        (void)hipFree(r->slab);
+        S->cache.put(r->slab, r->slab_bytes);
         r->slab = nullptr;
     }
-    r->proj.clear();
-    r->proj.shrink_to_fit();
+    r->proj = nullptr;
+}
+
+// Admission is possible when the head of the queue fits, or when nothing holds
+// memory that could still be freed (then it fails instead of waiting forever).
+bool admissible(const tiltgpu_server* S) {
+    if (S->queue.empty()) return false;
+    const Req* r = S->queue.front();
+    return S->used + r->need <= S->budget || (S->active.empty() && S->finishing == 0);
 }
 
 void worker_main(tiltgpu_server* S) {
@@ -442,13 +612,16 @@ void worker_main(tiltgpu_server* S) {
         std::vector<Req*> admitted;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->active.empty(); });
-            if (S->stop && S->active.empty() && S->queue.empty()) break;
-            // FIFO admission while the device-memory budget allows
-            while (!S->queue.empty()) {
+            const auto drained = [&] { return S->stop && S->queue.empty() && S->active.empty(); };
+            S->cv_work.wait(lk, [&] { return drained() || !S->active.empty() || admissible(S); });
+            if (drained()) break;
+            // FIFO admission while the device-memory budget allows; a few at a
+            // time, so that the first rounds start while later uploads proceed
+            int64_t admit_bytes = 0;
+            while (!S->queue.empty() && admitted.size() < kAdmitMax && admit_bytes < kAdmitBytes) {
                 Req* r = S->queue.front();
                 if (S->used + r->need > S->budget) {
-                    if (S->active.empty() && admitted.empty()) {
+                    if (S->active.empty() && admitted.empty() && S->finishing == 0) {
                         // nothing in flight will free memory: only resident series
                         // hold it, so this request can never be admitted
                         S->queue.pop_front();
@@ -463,16 +636,20 @@ void worker_main(tiltgpu_server* S) {
                 }
                 S->queue.pop_front();
                 S->used += r->need;
+                admit_bytes += static_cast<int64_t>(r->proj_n) * 4;
                 admitted.push_back(r);
             }
         }
+        // rounds that have completed free their returned buffers for reuse
+        for (Slot& sl : S->slots)
+            if (sl.seq && hipEventQuery(sl.done) == hipSuccess) S->cache.set_done(sl.seq);
         for (Req* r : admitted) {
             int rc = setup(S, r);
             std::lock_guard<std::mutex> lk(S->mu);
             if (rc) {
                 finish(S, r, rc, tiltgpu_last_error());
                 r->state = kFailed;
-                release(S, r);
+                release(S, r, false);
                 S->used -= r->need;
                 S->failed++;
                 S->cv_done.notify_all();
@@ -488,38 +665,103 @@ void worker_main(tiltgpu_server* S) {
             S->max_active = std::max<int>(S->max_active, static_cast<int>(act.size()));
         }
         if (act.empty()) continue;
-        int rc = run_round(S, act);
-        std::string msg = rc ? tiltgpu_last_error() : "";
-        if (rc == 0) {
-            for (Req* r : act)
-                if (r->s && (rc = check_kernel_error(r->s))) {
-                    msg = tiltgpu_last_error();
-                    break;
-                }
+        Slot& sl = S->slots[S->round_seq % kSlots];
+        if (sl.seq) {
+            (void)hipEventSynchronize(sl.done);
+            S->cache.set_done(sl.seq);
         }
-        std::lock_guard<std::mutex> lk(S->mu);
-        std::vector<Req*> keep;
+        const uint64_t seq = ++S->round_seq;
+        S->cache.set_enqueued(seq);
+        int rc = run_round(S, act, sl);
+        std::string msg = rc ? tiltgpu_last_error() : "";
+        if (hipEventRecord(sl.done, S->stream) == hipSuccess) sl.seq = seq;
+        std::vector<Req*> keep, done;
         for (Req* r : act) {
-            if (rc == 0 && !round_finished(r)) {
-                keep.push_back(r);
-                continue;
-            }
             // a failed round fails every request that was in it (the device
             // state of each is unknown); other queued requests are unaffected
-            finish(S, r, rc, msg.c_str());
-            r->state = rc ? kFailed : kDone;
-            release(S, r);
-            if (r->kind != kSeries) S->used -= r->need;
-            if (rc)
-                S->failed++;
-            else if (r->kind != kSeries) {
-                S->completed++;
-                S->lat_ms.push_back(
-                    std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
+            if (rc == 0 && !round_finished(r))
+                keep.push_back(r);
+            else
+                done.push_back(r);
+        }
+        std::shared_ptr<hipEvent_t> ev;
+        if (!done.empty()) {
+            ev = std::shared_ptr<hipEvent_t>(new hipEvent_t(take_event(S)), [S](hipEvent_t* e) {
+                give_event(S, *e);
+                delete e;
+            });
+            (void)hipEventRecord(*ev, S->stream);
+        }
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->active = keep;
+        S->rounds++;
+        for (Req* r : done) {
+            FinItem it;
+            it.ev = ev;
+            it.seq = seq;
+            it.rc = rc;
+            it.msg = msg;
+            it.r = r;
+            S->fin.push_back(std::move(it));
+        }
+        S->finishing += static_cast<int>(done.size());
+        if (!done.empty()) S->cv_fin.notify_all();
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fin_stop = true;
+    S->cv_fin.notify_all();
+}
+
+void finisher_main(tiltgpu_server* S, int k) {
+    (void)hipSetDevice(S->device);
+    for (;;) {
+        FinItem it;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_fin.wait(lk, [&] { return S->fin_stop || !S->fin.empty(); });
+            if (S->fin.empty()) break;
+            it = std::move(S->fin.front());
+            S->fin.pop_front();
+        }
+        Req* r = it.r;
+        int rc = it.rc;
+        std::string msg = it.msg;
+        hipError_t e = hipEventSynchronize(*it.ev);
+        S->cache.set_done(it.seq);
+        if (rc == 0 && e != hipSuccess) {
+            rc = TILTGPU_ERR_HIP;
+            msg = hipGetErrorString(e);
+        }
+        if (rc == 0 && r->s && (rc = check_kernel_error(r->s))) msg = tiltgpu_last_error();
+        if (rc == 0 && r->kind != kSeries) {
+            const void* src = r->kind == kSlab ? static_cast<const void*>(r->slab) : r->s->vol;
+            const size_t bytes = r->kind == kSlab ? r->slab_bytes
+                                                  : static_cast<size_t>(r->s->d.in_ny) *
+                                                        r->s->d.out_nz * r->s->d.out_nx * 4;
+            hipError_t c =
+                hipMemcpyAsync(r->out, src, bytes, hipMemcpyDeviceToHost, S->down_stream[k]);
+            if (c == hipSuccess) c = hipStreamSynchronize(S->down_stream[k]);
+            if (c != hipSuccess) {
+                rc = TILTGPU_ERR_HIP;
+                msg = hipGetErrorString(c);
             }
         }
-        S->active = keep;
+        release(S, r, rc == 0);
+        it.ev.reset();
+        std::lock_guard<std::mutex> lk(S->mu);
+        finish(S, r, rc, msg.c_str());
+        r->state = rc ? kFailed : kDone;
+        if (r->kind != kSeries || rc) S->used -= r->need;
+        if (rc)
+            S->failed++;
+        else if (r->kind != kSeries) {
+            S->completed++;
+            S->lat_ms.push_back(
+                std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
+        }
+        S->finishing--;
         S->cv_done.notify_all();
+        S->cv_work.notify_all();
     }
 }
 
@@ -557,8 +799,9 @@ int fill_recon(Req* r, const tiltgpu_dims* d, const tiltgpu_options* o, const fl
         tiltgpu_default_options(&r->o);
     if (r->o.storage < 0 || r->o.storage > 2) return fail(TILTGPU_ERR_ARG, "unknown storage");
     r->angles.assign(angles, angles + n_angles);
-    const size_t np = static_cast<size_t>(d->nz) * d->in_ny * d->in_nx;
-    r->proj.assign(projections, projections + np);
+    // borrowed: the caller keeps the projections until the request completes
+    r->proj = projections;
+    r->proj_n = static_cast<size_t>(d->nz) * d->in_ny * d->in_nx;
     r->need = recon_bytes(r->d, r->o);
     return TILTGPU_OK;
 }
@@ -579,11 +822,25 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out) {
     (void)hipGetDevice(&S->device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, S->device) == hipSuccess) S->n_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    S->cache.cap = budget_bytes + budget_bytes / 4;
+    bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&S->up_stream, hipStreamNonBlocking) == hipSuccess;
+    for (hipStream_t& st : S->down_stream)
+        ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    for (Slot& sl : S->slots)
+        ok = ok && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        for (Slot& sl : S->slots)
+            if (sl.done) (void)hipEventDestroy(sl.done);
+        for (hipStream_t st : {S->stream, S->up_stream})
+            if (st) (void)hipStreamDestroy(st);
+        for (hipStream_t st : S->down_stream)
+            if (st) (void)hipStreamDestroy(st);
         delete S;
-        return fail(TILTGPU_ERR_HIP, "server stream creation failed");
+        return fail(TILTGPU_ERR_HIP, "server stream/event creation failed");
     }
     S->worker = std::thread(worker_main, S);
+    for (int k = 0; k < kFinishers; ++k) S->finisher[k] = std::thread(finisher_main, S, k);
     *out = S;
     return TILTGPU_OK;
 }
@@ -761,18 +1018,29 @@ void tiltgpu_server_destroy(tiltgpu_server* S) {
         S->stop = true;
         S->cv_work.notify_all();
     }
+    // the scheduler drains the queue, then stops the finisher once it is empty
     if (S->worker.joinable()) S->worker.join();
+    for (std::thread& t : S->finisher)
+        if (t.joinable()) t.join();
+    (void)hipStreamSynchronize(S->stream);
     for (Req* r : S->series) {
         if (r->s) tiltgpu_destroy(r->s);
         r->s = nullptr;
     }
     for (auto& kv : S->reqs) delete kv.second;
-    for (Req* r : S->queue) (void)r;  // queued requests are owned by reqs
-    for (BatchBufs* b : {&S->bp_bufs, &S->filt_bufs, &S->fp_bufs}) {
-        if (b->dev) (void)hipFree(b->dev);
-        if (b->host) (void)hipHostFree(b->host);
+    for (Slot& sl : S->slots) {
+        for (BatchBufs* b : {&sl.bp, &sl.filt, &sl.fp}) {
+            if (b->dev) (void)hipFree(b->dev);
+            if (b->host) (void)hipHostFree(b->host);
+        }
+        if (sl.done) (void)hipEventDestroy(sl.done);
     }
-    if (S->stream) (void)hipStreamDestroy(S->stream);
+    for (hipEvent_t e : S->events) (void)hipEventDestroy(e);
+    S->cache.release_all();
+    for (hipStream_t st : {S->stream, S->up_stream})
+        if (st) (void)hipStreamDestroy(st);
+    for (hipStream_t st : S->down_stream)
+        if (st) (void)hipStreamDestroy(st);
     delete S;
 }
 

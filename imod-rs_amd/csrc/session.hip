@@ -475,12 +475,14 @@ int init_device(tiltgpu_session* s, hipStream_t external) {
 namespace tiltgpu_sess {
 
 int session_create(const tiltgpu_dims* dims, const tiltgpu_options* opts, hipStream_t stream,
-                   tiltgpu_session** out) {
+                   tiltgpu_session** out, BufferPool* pool) {
     if (!out) return fail(TILTGPU_ERR_ARG, "out is NULL");
     *out = nullptr;
     int rc = validate_dims(dims);
     if (rc) return rc;
     auto* s = new tiltgpu_session();
+    s->pool = pool;
+    s->timing = pool == nullptr;
     s->d = *dims;
     if (opts)
         s->o = *opts;
@@ -878,13 +880,19 @@ int64_t tiltgpu_device_bytes(const tiltgpu_session* s) { return s ? s->bytes : 0
 
 void tiltgpu_destroy(tiltgpu_session* s) {
     if (!s) return;
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    // a pooled (server) session is destroyed only after its work has completed
+    if (s->stream && !s->pool) (void)hipStreamSynchronize(s->stream);
     for (auto& r : s->pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
-    for (void* p : s->allocs) (void)hipFree(p);
+    for (size_t i = 0; i < s->allocs.size(); ++i) {
+        if (s->pool)
+            s->pool->put(s->allocs[i], s->alloc_bytes[i]);
+        else
+            (void)hipFree(s->allocs[i]);
+    }
     if (s->stream && s->own_stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }

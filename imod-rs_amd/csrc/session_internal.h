@@ -36,10 +36,23 @@ using namespace tiltgpu;
 
 enum { kClsPrep = 0, kClsFilter, kClsBp, kClsFp, kClsOther, kNumCls };
 
+namespace tiltgpu_sess {
+// Device-buffer provider a session may borrow (the server's buffer cache):
+// get() returns a buffer of at least `bytes`, put() takes one back.  Neither
+// synchronises the device, unlike hipFree.
+struct BufferPool {
+    virtual void* get(size_t bytes) = 0;
+    virtual void put(void* p, size_t bytes) = 0;
+    virtual ~BufferPool() = default;
+};
+}  // namespace tiltgpu_sess
+
 struct tiltgpu_session {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false when a server lends its stream
+    tiltgpu_sess::BufferPool* pool = nullptr;  // server sessions: buffers come from its cache
+    bool timing = true;      // per-stage event timing (off for server sessions)
     tiltgpu_dims d{};
     tiltgpu_options o{};
     bool seam = false;  // created by tiltgpu_bp_session_new
@@ -101,6 +114,7 @@ struct tiltgpu_session {
     int sirt_done = 0;
     int64_t bytes = 0;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // parallel to allocs
 
     std::vector<TimedRegion> pending;
     std::vector<hipEvent_t> event_pool;
@@ -117,11 +131,17 @@ inline int dev_alloc(tiltgpu_session* s, T** p, size_t count) {
     if (count == 0) count = 1;
     void* q = nullptr;
     const size_t bytes = count * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess)
-        return fail(e == hipErrorOutOfMemory ? TILTGPU_ERR_BUDGET : TILTGPU_ERR_HIP,
-                    "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    if (s->pool) {
+        q = s->pool->get(bytes);
+        if (!q) return fail(TILTGPU_ERR_BUDGET, "device buffer of %zu bytes unavailable", bytes);
+    } else {
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? TILTGPU_ERR_BUDGET : TILTGPU_ERR_HIP,
+                        "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+    }
     s->allocs.push_back(q);
+    s->alloc_bytes.push_back(bytes);
     s->bytes += static_cast<int64_t>(bytes);
     *p = static_cast<T*>(q);
     return TILTGPU_OK;
@@ -131,8 +151,18 @@ template <typename T>
 inline void dev_free(tiltgpu_session* s, T*& p) {
     if (!p) return;
     auto it = std::find(s->allocs.begin(), s->allocs.end(), static_cast<void*>(p));
-    if (it != s->allocs.end()) s->allocs.erase(it);
-    (void)hipFree(p);
+    size_t bytes = 0;
+    if (it != s->allocs.end()) {
+        const size_t k = static_cast<size_t>(it - s->allocs.begin());
+        bytes = s->alloc_bytes[k];
+        s->allocs.erase(it);
+        s->alloc_bytes.erase(s->alloc_bytes.begin() + static_cast<std::ptrdiff_t>(k));
+        s->bytes -= static_cast<int64_t>(bytes);
+    }
+    if (s->pool)
+        s->pool->put(p, bytes);  // stream order keeps earlier uses ahead of any reuse
+    else
+        (void)hipFree(p);
     p = nullptr;
 }
 
@@ -151,12 +181,14 @@ struct Scope {
     tiltgpu_session* s;
     TimedRegion r;
     Scope(tiltgpu_session* s_, int cls) : s(s_) {
+        if (!s->timing) return;
         r.a = take_event(s);
         r.b = take_event(s);
         r.cls = cls;
         (void)hipEventRecord(r.a, s->stream);
     }
     ~Scope() {
+        if (!s->timing) return;
         (void)hipEventRecord(r.b, s->stream);
         s->pending.push_back(r);
     }
@@ -190,8 +222,10 @@ int run_forward(tiltgpu_session* s);
 int run_means(tiltgpu_session* s);
 int validate_dims(const tiltgpu_dims* d);
 // Allocates the session's buffers on `stream` (nullptr: create its own stream).
+// pool: optional buffer provider (a server's cache); its sessions also skip
+// per-stage event timing and never synchronise on destruction.
 int session_create(const tiltgpu_dims* dims, const tiltgpu_options* opts, hipStream_t stream,
-                   tiltgpu_session** out);
+                   tiltgpu_session** out, BufferPool* pool = nullptr);
 int pick_pairs_per_wg(int n);
 
 }  // namespace tiltgpu_sess

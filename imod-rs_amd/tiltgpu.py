@@ -543,7 +543,7 @@ class Server:
         rid = C.c_int64()
         _check(lib().tiltgpu_server_submit(self.h, C.byref(d), C.byref(o), _f(ang), ang.size,
                                            _f(P), _f(out), C.byref(rid)))
-        self._pending[rid.value] = out
+        self._pending[rid.value] = (out, P)  # the server reads P until the request completes
         return rid.value
 
     def add_series(self, P, angles, opts: TiltOptions | None = None, xtilt=None, local=None):
@@ -577,7 +577,7 @@ class Server:
         out = np.empty((y1 - y0, onz, onx), np.float32)
         rid = C.c_int64()
         _check(lib().tiltgpu_server_submit_slab(self.h, series_id, y0, y1, _f(out), C.byref(rid)))
-        self._pending[rid.value] = out
+        self._pending[rid.value] = (out, None)
         return rid.value
 
     def poll(self, rid):
@@ -586,8 +586,11 @@ class Server:
     def wait(self, rid):
         """Blocks until done; returns (output array, latency in ms)."""
         lat = C.c_double()
-        out = self._pending.pop(rid)
-        _check(lib().tiltgpu_server_wait(self.h, rid, C.byref(lat)))
+        out, _ = self._pending[rid]
+        try:
+            _check(lib().tiltgpu_server_wait(self.h, rid, C.byref(lat)))
+        finally:
+            self._pending.pop(rid, None)
         return out, lat.value
 
     def stats(self):

@@ -214,9 +214,11 @@ typedef struct {
 } tiltgpu_server_stats;
 
 int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out);
-/* A whole reconstruction (the pipeline of tiltgpu_run); projections are copied at
- * submit; `out` receives [in_ny][out_nz][out_nx] and must stay valid until the
- * request is waited for. */
+/* A whole reconstruction (the pipeline of tiltgpu_run).  The projections are
+ * read when the request is admitted, not copied at submit: they, and `out`
+ * ([in_ny][out_nz][out_nx]), must stay valid and unchanged until the request
+ * has completed (tiltgpu_server_wait returns, or tiltgpu_server_poll reports it
+ * done or failed). */
 int tiltgpu_server_submit(tiltgpu_server* s, const tiltgpu_dims* dims,
                           const tiltgpu_options* opts, const float* angles, int32_t n_angles,
                           const float* projections, float* out, int64_t* id);
