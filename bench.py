@@ -269,8 +269,8 @@ def serve_stream(tg, jobs, arrivals, budget, slab_series=None):
 
 def plain_stream(tg, jobs, arrivals, slab_session=None):
     """Plain per-request FP32 launches: one request at a time in arrival order,
-    a session of its own per reconstruction, slabs from a backprojection session
-    over the filtered resident series; latency = arrival -> result."""
+    a session of its own per reconstruction, slabs as row-range WBP launches of a
+    session holding the resident series; latency = arrival -> result."""
     lat = []
     t0 = time.perf_counter()
     for job, ta in zip(jobs, arrivals):
@@ -278,7 +278,8 @@ def plain_stream(tg, jobs, arrivals, slab_session=None):
         if ta > now:
             time.sleep(ta - now)
         if job[0] == "slab":
-            slab_session.reconstruct_rows(job[1], job[1] + 64)
+            slab_session.run_rows(job[1], job[1] + 64)
+            slab_session.read_rows(job[1], job[1] + 64, stats=False)
         else:
             P, a, o, _ = job[1]
             plain_run(tg, P, a, o)
@@ -288,6 +289,18 @@ def plain_stream(tg, jobs, arrivals, slab_session=None):
 
 def pct(x, q):
     return float(np.percentile(x, q)) if len(x) else None
+
+
+def stream_jobs(reqs, with_slabs, seed):
+    """Reconstructions in order, with a slab request (64 output rows of the resident
+    2048^2 x 121 series, at a random 64-row offset) after every third one."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i, r in enumerate(reqs):
+        jobs.append(("recon", r))
+        if with_slabs and i % 3 == 2:
+            jobs.append(("slab", int(rng.integers(0, 32)) * 64))
+    return jobs
 
 
 def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30):
@@ -319,19 +332,43 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30):
                                    "gvoxel_view_per_s": round(work / dtp / 1e9, 2)}
     print(f"# plain sequential: {dtp:.2f} s", file=sys.stderr, flush=True)
     cap = out["batched_f32"]["requests_per_s"]
-    # served request streams at 20 / 60 / 90 % of batched capacity
+    # served request streams at 20 / 60 / 90 % of batched capacity, with slab
+    # requests of the resident C1 series mixed in when it is given
+    slab_session = None
+    if c1 is not None:
+        P1, ang1, xt1, loc1 = c1
+        slab_session = tg.Session(P1.shape[2], P1.shape[1], P1.shape[0],
+                                  tg.TiltOptions(thickness=512))
+        slab_session.set_angles(ang1)
+        slab_session.set_xtilt(xt1)
+        slab_session.set_local(*loc1)
+        slab_session.upload(P1)
+        slab_session.run_rows(0, 64)  # filters the resident series once
     stream = []
     for load in (0.2, 0.6, 0.9):
-        jobs = [("recon", r) for r in c3_requests(series, n_stream, "f32")]
-        arr = poisson_times(load * cap, len(jobs), seed=int(load * 100))
-        lat_s, st = serve_stream(tg, jobs, arr, budget)
-        lat_p = plain_stream(tg, jobs, arr)
-        stream.append({"load": load, "offered_requests_per_s": round(load * cap, 3),
-                       "served": {"p50_ms": round(pct(lat_s, 50), 2), "p99_ms": round(pct(lat_s, 99), 2),
-                                  "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2)},
-                       "plain_f32": {"p50_ms": round(pct(lat_p, 50), 2), "p99_ms": round(pct(lat_p, 99), 2)}})
+        jobs = stream_jobs(c3_requests(series, n_stream, "f32"), c1 is not None, seed=int(load * 100))
+        n_recon = sum(1 for j in jobs if j[0] == "recon")
+        # offered load counts reconstructions; slabs ride along
+        arr = poisson_times(load * cap * len(jobs) / n_recon, len(jobs), seed=int(load * 100))
+        lat_s, st = serve_stream(tg, jobs, arr, budget, slab_series=c1)
+        lat_p = plain_stream(tg, jobs, arr, slab_session=slab_session)
+        kinds = np.array([j[0] for j in jobs])
+        entry = {"load": load, "offered_requests_per_s": round(load * cap, 3),
+                 "jobs": {"reconstructions": n_recon, "slabs": int(np.sum(kinds == "slab"))},
+                 "served": {"p50_ms": round(pct(lat_s, 50), 2), "p99_ms": round(pct(lat_s, 99), 2),
+                            "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2)},
+                 "plain_f32": {"p50_ms": round(pct(lat_p, 50), 2), "p99_ms": round(pct(lat_p, 99), 2)}}
+        if c1 is not None:
+            sl = kinds == "slab"
+            entry["served"]["slab_p50_ms"] = round(pct(lat_s[sl], 50), 2)
+            entry["served"]["slab_p99_ms"] = round(pct(lat_s[sl], 99), 2)
+            entry["plain_f32"]["slab_p50_ms"] = round(pct(lat_p[sl], 50), 2)
+            entry["plain_f32"]["slab_p99_ms"] = round(pct(lat_p[sl], 99), 2)
+        stream.append(entry)
         print(f"# stream load {load}: served p50 {pct(lat_s, 50):.1f} p99 {pct(lat_s, 99):.1f} ms; "
               f"plain p50 {pct(lat_p, 50):.1f} p99 {pct(lat_p, 99):.1f} ms", file=sys.stderr, flush=True)
+    if slab_session is not None:
+        slab_session.close()
     out["stream"] = stream
     out["matched_accuracy"] = "served results are bit-identical to plain per-request FP32 runs"
     return out
@@ -347,7 +384,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-only", action="store_true", help="only the serving section")
     ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--stream-requests", type=int, default=24)
+    ap.add_argument("--stream-requests", type=int, default=48)
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
     ap.add_argument("--sirt-only", action="store_true", help="only the SIRT section")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -373,8 +410,9 @@ def main():
         print(json.dumps({"step_s": dt, **tm}), flush=True)
         return
     if a.serve_only:
-        print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.stream_requests)}),
-              flush=True)
+        c1 = c1_inputs(tg)
+        print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.stream_requests,
+                                                     c1=c1)}), flush=True)
         return
     if a.sirt_only:
         print(json.dumps({"sirt": sirt_section(tg, oracle=not a.no_cpu)}), flush=True)
@@ -437,12 +475,11 @@ def main():
         }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
-    if not a.no_sirt or not a.no_serve:
-        del P
     if not a.no_sirt:
         line["sirt"] = sirt_section(tg, oracle=not a.no_cpu)
     if not a.no_serve:
-        line["serving"] = serving_section(tg, a.serve_requests, a.stream_requests)
+        line["serving"] = serving_section(tg, a.serve_requests, a.stream_requests,
+                                          c1=(P, ang, xt, local))
     print(json.dumps(line), flush=True)
 
 

@@ -620,6 +620,7 @@ int tiltgpu_set_angles(tiltgpu_session* s, const float* a, int32_t n) {
     s->angles.assign(a, a + n);
     s->has_angles = true;
     s->geometry_ready = false;
+    s->filtered = false;
     return TILTGPU_OK;
 }
 
@@ -628,6 +629,7 @@ int tiltgpu_set_xtilt(tiltgpu_session* s, const float* a, int32_t n) {
     s->xtilt.assign(a, a + n);
     s->has_xtilt = true;
     s->geometry_ready = false;
+    s->filtered = false;
     return TILTGPU_OK;
 }
 
@@ -636,6 +638,7 @@ int tiltgpu_set_zfactor(tiltgpu_session* s, const float* a, int32_t n) {
     s->zfac.assign(a, a + n);
     s->has_zfac = true;
     s->geometry_ready = false;
+    s->filtered = false;
     return TILTGPU_OK;
 }
 
@@ -665,6 +668,7 @@ int tiltgpu_set_local_alignment(tiltgpu_session* s, const int32_t* nx, const int
     s->ldy.assign(dy, dy + tg);
     s->has_local = true;
     s->geometry_ready = false;
+    s->filtered = false;
     return TILTGPU_OK;
 }
 
@@ -678,7 +682,33 @@ int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t co
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded = true;
+    s->filtered = false;
     return TILTGPU_OK;
+}
+
+int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {
+    if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
+    if (s->o.sirt > 0 || s->o.fake_sirt > 0)
+        return fail(TILTGPU_ERR_ARG, "row ranges are reconstructed by WBP only (sirt and "
+                                     "fake_sirt couple the rows)");
+    if (y0 < 0 || y1 > s->d.in_ny || y0 >= y1) return fail(TILTGPU_ERR_ARG, "bad row range");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    if (!s->filtered) {
+        const float* O = s->o.log ? s->logd : s->raw;
+        if ((rc = run_means(s))) return rc;
+        if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
+        s->filtered = true;
+    }
+    const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
+                                                         : s->store;
+    const int64_t plane = static_cast<int64_t>(s->d.out_nz) * s->d.out_nx;
+    if ((rc = run_bp(s, proj, s->scale, y0, y1, 1, s->vol + y0 * plane))) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
+    s->ran = true;
+    return check_kernel_error(s);
 }
 
 int tiltgpu_run(tiltgpu_session* s) {
@@ -691,6 +721,7 @@ int tiltgpu_run(tiltgpu_session* s) {
     if ((rc = run_means(s))) return rc;
     // R-weighting + cosine stretch (+ storage conversion)
     if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
+    s->filtered = s->o.sirt == 0;  // SIRT filters its residuals into the same buffers
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
                                                          : s->store;
     const int out_ny = s->d.in_ny;

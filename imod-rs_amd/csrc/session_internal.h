@@ -110,6 +110,7 @@ struct tiltgpu_session {
     int radix[kMaxRadix];
     int nradix = 0;
     bool uploaded = false;
+    bool filtered = false;  // filt/store hold this upload's R-weighted, stretched views
     bool ran = false;
     int sirt_done = 0;
     int64_t bytes = 0;

@@ -110,6 +110,7 @@ SIGNATURES = [
     ("tiltgpu_set_local_alignment", C.c_int, [_vp, _ip, _ip, _fp, _fp, _fp, _fp]),
     ("tiltgpu_upload_projections", C.c_int, [_vp, _fp, C.c_int64]),
     ("tiltgpu_run", C.c_int, [_vp]),
+    ("tiltgpu_run_rows", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_read_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
     ("tiltgpu_sirt_residuals", C.c_int, [_vp, _dp, C.c_int32]),
     ("tiltgpu_reproject", C.c_int, [_vp, _fp, _fp]),
@@ -301,6 +302,10 @@ class Session:
 
     def run(self):
         _check(lib().tiltgpu_run(self.h))
+
+    def run_rows(self, y0, y1):
+        """WBP of output rows [y0, y1) (filtering once per upload); read them with read_rows."""
+        _check(lib().tiltgpu_run_rows(self.h, y0, y1))
 
     def read_rows(self, y0=0, y1=None, stats=True):
         y1 = self.dims.in_ny if y1 is None else y1

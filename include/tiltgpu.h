@@ -142,6 +142,12 @@ int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t co
  * volume stays resident in HBM.  Replaces main.rs:607-1090 minus file I/O. */
 int tiltgpu_run(tiltgpu_session* s);
 
+/* WBP of output rows [y0, y1) only (then tiltgpu_read_rows(s, y0, y1, ...)): the
+ * row-by-row form of the reference's GPU dispatch (main.rs:898-918 calls
+ * GpuSession::reconstruct_row per row), with the filtering done on the device
+ * once per upload and reused by later row ranges.  WBP only (sirt = fake_sirt = 0). */
+int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1);
+
 /* Copy output Y rows [y0, y1) to host as [y][out_nz][out_nx] (the MRC section
  * order of main.rs:1077-1085) with per-row statistics min, max, mean
  * (min_max_mean of each section; stats may be NULL). */
