@@ -62,7 +62,10 @@ def load_traffic():
         return None
 
 
-def run_gpu(tg, P, ang, xt, local, storage, steps, warmup):
+ERR_ROWS = (992, 1056)  # the 64-slice subset the FP64-oracle error is measured on
+
+
+def run_gpu(tg, P, ang, xt, local, storage, steps, warmup, keep_rows=None):
     opts = tg.TiltOptions(thickness=512, radial=(0.35, 0.05), storage=storage)
     s = tg.Session(P.shape[2], P.shape[1], P.shape[0], opts)
     s.set_angles(ang)
@@ -80,8 +83,43 @@ def run_gpu(tg, P, ang, xt, local, storage, steps, warmup):
     s.synchronize()
     dt = time.perf_counter() - t0
     tm = s.timing()
+    rows = s.read_rows(*keep_rows, stats=False)[0] if keep_rows else None
     s.close()
+    if keep_rows:
+        return dt, tm, rows
     return dt, tm
+
+
+def oracle_rows_f64(P, ang, xt, local, rows=ERR_ROWS, halo=24):
+    """The FP64 oracle's output rows [rows) of the WBP workload (the checker for
+    the error figures), from the projection rows they can touch."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402 -- checker only
+    y0, y1 = rows
+    p0, p1 = y0 - halo, y1 + halo
+    sub = np.ascontiguousarray(P[:, p0:p1, :])
+    nv = P.shape[0]
+    patches = [(v, float(local[2][v]), float(local[3][v] - p0), float(local[4][v]),
+                float(local[5][v])) for v in range(nv)]
+    grids = O.build_local_grids(patches, nv)
+    opts = O.TiltOptions(thickness=512, radial=(0.35, 0.05))
+    vol, _ = O.tilt_run(sub, ang, opts, xtilt=xt, local=grids, prec="f64",
+                        nthreads=min(16, os.cpu_count() or 1), rows=(halo, halo + (y1 - y0)),
+                        means=view_means(P))
+    return vol
+
+
+def view_means(P):
+    """Per-view means of the whole images (the fill values), in f64."""
+    return np.array([np.mean(P[v], dtype=np.float64) for v in range(P.shape[0])])
+
+
+def err_vs(x, ref):
+    ref = ref.astype(np.float64)
+    d = x.astype(np.float64) - ref
+    rms = float(np.sqrt(np.mean(ref ** 2)))
+    return {"max_abs_over_rms": float(f"{np.max(np.abs(d)) / rms:.3g}"),
+            "rms_over_rms": float(f"{np.sqrt(np.mean(d ** 2)) / rms:.3g}")}
 
 
 def cpu_baseline(P, ang, xt, local, rows=32):
@@ -101,9 +139,10 @@ def cpu_baseline(P, ang, xt, local, rows=32):
                 float(local[5][v])) for v in range(nv)]
     grids = O.build_local_grids(patches, nv)
     opts = O.TiltOptions(thickness=512, radial=(0.35, 0.05))
+    means = view_means(P)
     t0 = time.perf_counter()
     O.tilt_run(sub, ang, opts, xtilt=xt, local=grids, prec="f32", nthreads=nthreads,
-               rows=(halo, halo + rows))
+               rows=(halo, halo + rows), means=means)
     dt = time.perf_counter() - t0
     vv = rows * 512 * P.shape[2] * nv
     return {"value": vv / dt / 1e9, "unit": "Gvoxel·view/s", "cores": nthreads,
@@ -423,7 +462,7 @@ def main():
     nx, ny, nv = P.shape[2], P.shape[1], P.shape[0]
     vv = float(nx) * ny * 512 * nv
 
-    dt, tm = run_gpu(tg, P, ang, xt, local, "f32", a.steps, a.warmup)
+    dt, tm, rows32 = run_gpu(tg, P, ang, xt, local, "f32", a.steps, a.warmup, keep_rows=ERR_ROWS)
     ms_step = dt / a.steps * 1e3
     value = vv / (dt / a.steps) / 1e9
     bp_ms = tm["bp_ms"] / max(1, tm["bp_launches"])
@@ -466,13 +505,26 @@ def main():
             "launch_ms": round(bp_ms, 3),
         },
     }
+    rows16 = None
     if not a.no_fp16:
-        dt16, tm16 = run_gpu(tg, P, ang, xt, local, "f16", a.steps, a.warmup)
+        dt16, tm16, rows16 = run_gpu(tg, P, ang, xt, local, "f16", a.steps, a.warmup,
+                                     keep_rows=ERR_ROWS)
         line["fp16_storage"] = {
             "value": round(vv / (dt16 / a.steps) / 1e9, 3),
             "ms_per_step": round(dt16 / a.steps * 1e3, 3),
             "bp_launch_ms": round(tm16["bp_ms"] / max(1, tm16["bp_launches"]), 3),
         }
+    if not a.no_cpu:
+        t1 = time.perf_counter()
+        ref64 = oracle_rows_f64(P, ang, xt, local)
+        err = {"subset": f"output rows [{ERR_ROWS[0]}, {ERR_ROWS[1]}) of 2048 (64 slices), "
+                         f"FP64 oracle {time.perf_counter() - t1:.1f} s",
+               "f32_storage": err_vs(rows32, ref64)}
+        if rows16 is not None:
+            err["f16_storage"] = err_vs(rows16, ref64)
+            err["f16_vs_f32_gpu"] = err_vs(rows16, rows32)
+        line["error_vs_f64_oracle"] = err
+        print(f"# error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
     if not a.no_sirt:

@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["csrc/kernels_bp.hip", "csrc/kernels_filter.hip", "csrc/kernels_fp.hip",
                "csrc/kernels_misc.hip", "csrc/session.hip", "csrc/server.hip"]
-HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp"]
+HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp", "csrc/tuning.cpp"]
 CLI_SOURCES = ["host/tilt_main.cpp"]
 
 # -ffp-contract=off: coordinate arithmetic must round like the reference (which

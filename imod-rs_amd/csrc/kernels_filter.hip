@@ -80,13 +80,15 @@ __device__ __forceinline__ void dft_small<5>(float2* x) {
 template <int R>
 __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in,
                                               float2* __restrict__ out, int n, int pairs,
-                                              int Ns, const float2* __restrict__ tw) {
+                                              int Ns, const float2* __restrict__ tw,
+                                              const FastDiv& mdiv, const FastDiv& nsdiv) {
     const int m = n / R;
     const int tstep = n / (Ns * R);
     for (int idx = threadIdx.x; idx < pairs * m; idx += blockDim.x) {
-        const int q = idx / m;
+        const int q = static_cast<int>(mdiv.div(static_cast<uint32_t>(idx)));
         const int j = idx - q * m;
-        const int k = j % Ns;
+        const int jq = static_cast<int>(nsdiv.div(static_cast<uint32_t>(j)));
+        const int k = j - jq * Ns;
         const float2* src = in + q * n;
         float2 x[R];
 #pragma unroll
@@ -96,7 +98,7 @@ __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in,
             for (int r = 1; r < R; ++r) x[r] = cmul(x[r], tw[r * k * tstep]);
         }
         dft_small<R>(x);
-        float2* dst = out + q * n + (j / Ns) * Ns * R + k;
+        float2* dst = out + q * n + jq * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[r * Ns] = x[r];
     }
@@ -106,16 +108,18 @@ __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in,
 // twiddle table (W_p^e = tw[e * n/p]).
 __device__ __noinline__ void stockham_pass_generic(const float2* __restrict__ in,
                                                    float2* __restrict__ out, int n, int pairs,
-                                                   int Ns, int R, const float2* __restrict__ tw) {
+                                                   int Ns, int R, const float2* __restrict__ tw,
+                                                   FastDiv mdiv, FastDiv nsdiv) {
     const int m = n / R;
     const int tstep = n / (Ns * R);
     const int pstep = n / R;
     for (int idx = threadIdx.x; idx < pairs * m; idx += blockDim.x) {
-        const int q = idx / m;
+        const int q = static_cast<int>(mdiv.div(static_cast<uint32_t>(idx)));
         const int j = idx - q * m;
-        const int k = j % Ns;
+        const int jq = static_cast<int>(nsdiv.div(static_cast<uint32_t>(j)));
+        const int k = j - jq * Ns;
         const float2* src = in + q * n;
-        float2* dst = out + q * n + (j / Ns) * Ns * R + k;
+        float2* dst = out + q * n + jq * Ns * R + k;
         for (int qq = 0; qq < R; ++qq) {
             float2 s = make_float2(0.f, 0.f);
             for (int r = 0; r < R; ++r) {
@@ -137,16 +141,18 @@ __device__ float2* fft_forward(float2* b0, float2* b1, const FilterParams& P, in
     int Ns = 1;
     for (int s = 0; s < P.nradix; ++s) {
         const int R = P.radix[s];
+        const FastDiv& md = P.m_div[s];
+        const FastDiv& nd = P.ns_div[s];
         if (R == 4)
-            stockham_pass<4>(in, out, P.n, pairs, Ns, P.tw);
+            stockham_pass<4>(in, out, P.n, pairs, Ns, P.tw, md, nd);
         else if (R == 2)
-            stockham_pass<2>(in, out, P.n, pairs, Ns, P.tw);
+            stockham_pass<2>(in, out, P.n, pairs, Ns, P.tw, md, nd);
         else if (R == 3)
-            stockham_pass<3>(in, out, P.n, pairs, Ns, P.tw);
+            stockham_pass<3>(in, out, P.n, pairs, Ns, P.tw, md, nd);
         else if (R == 5)
-            stockham_pass<5>(in, out, P.n, pairs, Ns, P.tw);
+            stockham_pass<5>(in, out, P.n, pairs, Ns, P.tw, md, nd);
         else
-            stockham_pass_generic(in, out, P.n, pairs, Ns, R, P.tw);
+            stockham_pass_generic(in, out, P.n, pairs, Ns, R, P.tw, md, nd);
         __syncthreads();
         Ns *= R;
         float2* t = in;
@@ -213,7 +219,7 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
     // ---- load two rows per pair (row a -> re, row b -> im), zero past the end
     double sq = 0.0;
     for (int idx = tid; idx < pairs * n; idx += blockDim.x) {
-        const int q = idx / n;
+        const int q = static_cast<int>(P.n_div.div(static_cast<uint32_t>(idx)));
         const int i = idx - q * n;
         const int ra = 2 * (pair0 + q), rb = ra + 1;
         float va = P.src[static_cast<long>(ra) * n + i];
@@ -244,7 +250,7 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
         float2* X = fft_forward(b0, b1, P, pairs);
         // Z' = conj(W * X)
         for (int idx = tid; idx < pairs * n; idx += blockDim.x) {
-            const int i = idx % n;
+            const int i = idx - n * static_cast<int>(P.n_div.div(static_cast<uint32_t>(idx)));
             const float w = P.weights[i];
             const float2 v = X[idx];
             X[idx] = make_float2(v.x * w, -(v.y * w));
@@ -266,7 +272,7 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
 
     // ---- cosine stretch (per row: its own view) and coalesced write
     for (int idx = tid; idx < pairs * n; idx += blockDim.x) {
-        const int q = idx / n;
+        const int q = static_cast<int>(P.n_div.div(static_cast<uint32_t>(idx)));
         const int ox = idx - q * n;
         const float2* row = filt + q * n;
 #pragma unroll

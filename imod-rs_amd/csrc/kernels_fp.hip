@@ -440,12 +440,6 @@ size_t fp_lds_bytes(const FpParams& p) {
     return b;
 }
 
-int fp_bins_per_lane(int in_nx) {
-    if (in_nx <= 256) return 4;
-    if (in_nx <= 512) return 8;
-    return 16;
-}
-
 int fp_gather_blocks(const FpParams& p) { return fp_blocks(p); }
 
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "session_internal.h"
+#include "tuning.h"
 
 namespace tiltgpu_sess {
 
@@ -97,7 +98,20 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
         return pad * (1.0 + 16.0 / tz);
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
-    if (s->tile_z_override > 0) order = {s->tile_z_override};
+    s->tile_source = kTuneHeuristic;
+    const int tuned = tuned_tile_z(s->two_row ? 1 : 0, s->has_local ? 1 : 0,
+                                   s->o.storage != TILTGPU_STORE_F32 ? 1 : 0, s->d.in_nx, out_nz0);
+    if (tuned > 0) {
+        // the measured depth first; the others remain as LDS fallbacks
+        order.erase(std::remove(order.begin(), order.end(), tuned), order.end());
+        order.insert(order.begin(), tuned);
+        s->tile_source = kTuneTable;
+    }
+    const int forced = s->tune_tile_z > 0 ? s->tune_tile_z : s->tile_z_override;
+    if (forced > 0) {
+        order = {forced};
+        s->tile_source = kTuneForced;
+    }
     for (int tz : order) {
         int cap = 1;
         for (int v = 0; v < nv; ++v) {
@@ -328,7 +342,14 @@ FilterParams make_filter(tiltgpu_session* s, const float* src, const float* sub,
     f.pairs_per_wg = pick_pairs_per_wg(n);
     f.cubic = s->o.cosine_cubic;
     f.nradix = s->nradix;
-    for (int i = 0; i < s->nradix; ++i) f.radix[i] = s->radix[i];
+    f.n_div = make_fastdiv(static_cast<uint32_t>(n));
+    int ns = 1;
+    for (int i = 0; i < s->nradix; ++i) {
+        f.radix[i] = s->radix[i];
+        f.m_div[i] = make_fastdiv(static_cast<uint32_t>(n / s->radix[i]));
+        f.ns_div[i] = make_fastdiv(static_cast<uint32_t>(ns));
+        ns *= s->radix[i];
+    }
     return f;
 }
 
@@ -382,7 +403,7 @@ FpParams make_fp(tiltgpu_session* s) {
     p.nz_pad = s->nz_pad;
     p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
     p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
-    p.bins_per_lane = fp_bins_per_lane(s->d.in_nx);
+    p.bins_per_lane = s->tune_fp_bins > 0 ? s->tune_fp_bins : tuned_fp_bins(s->d.in_nx, nullptr);
     return p;
 }
 
@@ -804,6 +825,33 @@ int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n) {
     const int k = std::min<int>(n, s->sirt_done);
     if (k > 0) HIP_TRY(hipMemcpy(out, s->resid, k * sizeof(double), hipMemcpyDeviceToHost));
     return k;
+}
+
+int tiltgpu_set_tuning(tiltgpu_session* s, int32_t bp_tile_z, int32_t fp_bins_per_lane) {
+    if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    if (bp_tile_z != 0 && bp_tile_z != 32 && bp_tile_z != 64 && bp_tile_z != 128)
+        return fail(TILTGPU_ERR_ARG, "tile depth %d is not 0, 32, 64 or 128", bp_tile_z);
+    if (fp_bins_per_lane != 0 && fp_bins_per_lane != 4 && fp_bins_per_lane != 8 &&
+        fp_bins_per_lane != 16)
+        return fail(TILTGPU_ERR_ARG, "bins per lane %d is not 0, 4, 8 or 16", fp_bins_per_lane);
+    s->tune_tile_z = bp_tile_z;
+    s->tune_fp_bins = fp_bins_per_lane;
+    s->geometry_ready = false;  // the tile depth sets the padded depth of the view tables
+    return TILTGPU_OK;
+}
+
+int tiltgpu_get_tuning(tiltgpu_session* s, int32_t* out4) {
+    if (!s || s->seam || !out4) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    int fp_src = kTuneForced;
+    const int bins =
+        s->tune_fp_bins > 0 ? s->tune_fp_bins : tuned_fp_bins(s->d.in_nx, &fp_src);
+    out4[0] = s->tile_z;
+    out4[1] = s->tile_source;
+    out4[2] = bins;
+    out4[3] = fp_src;
+    return TILTGPU_OK;
 }
 
 int tiltgpu_set_sirt_trace(tiltgpu_session* s, int32_t y0, int32_t y1) {

@@ -105,6 +105,9 @@ struct tiltgpu_session {
     int nz_pad = 0;
     int tile_z = 64;
     int tile_z_override = 0;  // TILTGPU_TILE_Z (tuning): force the backprojection tile depth
+    int tune_tile_z = 0;      // tiltgpu_set_tuning: forced tile depth (0: table / heuristic)
+    int tune_fp_bins = 0;     // tiltgpu_set_tuning: forced FP bins per lane (0: table / rule)
+    int tile_source = 0;      // kTuneHeuristic / kTuneTable / kTuneForced
     int lds_cap = 0;
     bool two_row = false;
     int radix[kMaxRadix];

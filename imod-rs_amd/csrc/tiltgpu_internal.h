@@ -86,6 +86,27 @@ struct BpBatch {
 };
 
 // R-weighting filter + cosine stretch of rows of length n.
+// Division by a runtime constant d >= 1 as a multiply-high and shifts
+// (Granlund-Montgomery, exact for every 32-bit dividend).
+struct FastDiv {
+    uint32_t mul, shift, d;
+    __host__ __device__ uint32_t div(uint32_t x) const {
+        if (d == 1) return x;
+        const uint32_t t = static_cast<uint32_t>((static_cast<uint64_t>(x) * mul) >> 32);
+        return (t + ((x - t) >> 1)) >> (shift - 1);
+    }
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{0, 0, d};
+    if (d > 1) {
+        uint32_t l = 0;
+        while ((1ull << l) < d) ++l;
+        f.shift = l;
+        f.mul = static_cast<uint32_t>((((1ull << l) - d) << 32) / d + 1);
+    }
+    return f;
+}
+
 struct FilterParams {
     const float* src;       // [nrows][n]
     const float* src_sub;   // optional: rows are src - src_sub (SIRT residual)
@@ -99,6 +120,9 @@ struct FilterParams {
     int cubic;
     int nradix;
     int radix[kMaxRadix];
+    FastDiv n_div;                  // by n
+    FastDiv m_div[kMaxRadix];       // by n / radix[s] (butterflies per row of pass s)
+    FastDiv ns_div[kMaxRadix];      // by radix[0] * ... * radix[s-1] (pass s stride)
 };
 
 // Ordered-walk forward projection (no x-axis tilt, no local alignment).
@@ -110,7 +134,7 @@ struct FpParams {
     const float* zc_tab;
     int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
     float cx, ocx;
-    int bins_per_lane;    // 4, 8 or 16: fp_bins_per_lane(in_nx)
+    int bins_per_lane;    // 4, 8 or 16 (tuned_fp_bins)
 };
 
 // Scatter-form forward projection (x-axis tilt and/or local alignment).
@@ -169,7 +193,6 @@ hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_
                                        hipStream_t st);
 size_t fp_lds_bytes(const FpParams& p);  // dynamic LDS of one series' workgroups
 int fp_gather_blocks(const FpParams& p);
-int fp_bins_per_lane(int in_nx);
 hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st);
 hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
                                hipStream_t st);

@@ -114,6 +114,8 @@ SIGNATURES = [
     ("tiltgpu_read_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
     ("tiltgpu_sirt_residuals", C.c_int, [_vp, _dp, C.c_int32]),
     ("tiltgpu_reproject", C.c_int, [_vp, _fp, _fp]),
+    ("tiltgpu_set_tuning", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("tiltgpu_get_tuning", C.c_int, [_vp, _ip]),
     ("tiltgpu_set_sirt_trace", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_read_sirt_trace", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_read_filtered", C.c_int, [_vp, _fp]),
@@ -318,6 +320,16 @@ class Session:
         out = np.zeros(max(1, self.opts.sirt), np.float64)
         k = _check(lib().tiltgpu_sirt_residuals(self.h, out.ctypes.data_as(_dp), out.size))
         return out[:k]
+
+    def set_tuning(self, bp_tile_z=0, fp_bins_per_lane=0):
+        """Force launch shapes (0 = tuned table / analytic rule)."""
+        _check(lib().tiltgpu_set_tuning(self.h, bp_tile_z, fp_bins_per_lane))
+
+    def tuning(self):
+        """{'tile_z', 'tile_source', 'fp_bins', 'fp_source'}; source 0 rule, 1 table, 2 forced."""
+        out = np.zeros(4, np.int32)
+        _check(lib().tiltgpu_get_tuning(self.h, out.ctypes.data_as(_ip)))
+        return dict(zip(("tile_z", "tile_source", "fp_bins", "fp_source"), map(int, out)))
 
     def set_sirt_trace(self, y0, y1):
         """Record rows [y0, y1) after the WBP and every SIRT iteration of run()."""

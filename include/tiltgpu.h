@@ -158,6 +158,18 @@ int tiltgpu_read_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out,
  * device (a build addition: the reference computes none, main.rs:928-953). */
 int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n);
 
+/* Launch shapes.  No counterpart in the reference (its CPU loops have none);
+ * for imod-rs_amd/autotune.py, which measures them, and for callers pinning a
+ * shape.  bp_tile_z: backprojection tile depth 32, 64 or 128; fp_bins_per_lane:
+ * forward-projection bins per lane 4, 8 or 16; 0 = the tuned table's choice (or
+ * the analytic rule where the table has no entry).  Results do not depend on
+ * either (they change only the work split), except that the tile depth must fit
+ * the geometry's LDS need. */
+int tiltgpu_set_tuning(tiltgpu_session* s, int32_t bp_tile_z, int32_t fp_bins_per_lane);
+/* out4 = {tile depth, its source, FP bins per lane, its source}; sources:
+ * 0 analytic rule, 1 tuned table, 2 forced by tiltgpu_set_tuning. */
+int tiltgpu_get_tuning(tiltgpu_session* s, int32_t* out4);
+
 /* Keep output rows [y0, y1) of the volume after the initial WBP and after every
  * SIRT iteration of the next tiltgpu_run (before the output transform), for
  * per-iteration monitoring of the iteration of apps/tilt/src/main.rs:927-953,

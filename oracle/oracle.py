@@ -69,7 +69,7 @@ def _declare(L):
             _ip, _ip, _fp, _fp, _fp, _fp, _fp, _fp]
         getattr(L, f"or_tilt_run_rows_{suf}").argtypes = [
             C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
-            _ip, _ip, _fp, _fp, _fp, _fp, C.c_int, C.c_int, _fp, _fp]
+            _ip, _ip, _fp, _fp, _fp, _fp, C.c_int, C.c_int, _fp, _fp, _dp]
         getattr(L, f"or_sirt_trace_{suf}").argtypes = [
             C.POINTER(OrOpts), _fp, _fp, C.c_int, _fp, C.c_int, C.c_int, C.c_int, _fp, _dp]
         getattr(L, f"or_backproject_simple_{suf}").argtypes = [
@@ -385,10 +385,11 @@ def sirt_trace(P, angles, opts: TiltOptions, rows, zfactor=None, prec="f64", nth
 
 
 def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
-             prec="f32", nthreads=8, rows=None):
+             prec="f32", nthreads=8, rows=None, means=None):
     """The whole reconstruction (main.rs:601-1093 after reading).  P: [nz][ny][nx]
     raw projections.  Returns (volume [ny][out_nz][out_nx] float32, (amin, amax, amean)).
-    rows=(y0, y1) reconstructs (and returns) only those output rows."""
+    rows=(y0, y1) reconstructs (and returns) only those output rows.  means: the
+    per-view means (f64) of the whole images when P is a band of their rows."""
     P = np.array(P, dtype=np.float32, copy=True)
     nz, ny, nx = P.shape
     ang = _f32c(angles)
@@ -410,5 +411,6 @@ def tilt_run(P, angles, opts: TiltOptions, xtilt=None, zfactor=None, local=None,
         lp = (None,) * 6
     getattr(lib(), f"or_tilt_run_rows_{prec}")(
         C.byref(o), _f(P), _f(ang), ang.size, _f(xt), 0 if xt is None else xt.size,
-        _f(zf), 0 if zf is None else zf.size, *lp, y0, y1, _f(vol), _f(stats))
+        _f(zf), 0 if zf is None else zf.size, *lp, y0, y1, _f(vol), _f(stats),
+        None if means is None else np.ascontiguousarray(means, np.float64).ctypes.data_as(_dp))
     return vol, (float(stats[0]), float(stats[1]), float(stats[2]))

@@ -574,7 +574,7 @@ void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_an
                     const int* lnx, const int* lny, const float* lxs, const float* lys,
                     const float* ldx, const float* ldy, float* vol, float* stats) {
     F(or_tilt_run_rows)(o, P_in, angles, n_angles, xtilt, n_xtilt, zfac, n_zfac, lnx, lny,
-                        lxs, lys, ldx, ldy, 0, o->in_ny, vol, stats);
+                        lxs, lys, ldx, ldy, 0, o->in_ny, vol, stats, NULL);
 }
 
 /* Per-view means of the (logged) projections, main.rs:615-623 (min_max_mean's
@@ -602,7 +602,7 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
                          const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
                          const int* lnx, const int* lny, const float* lxs, const float* lys,
                          const float* ldx, const float* ldy, int y0, int y1, float* vol,
-                         float* stats) {
+                         float* stats, const double* means_in) {
     long nxy = (long)o->in_nx * o->in_ny;
     long total = nxy * o->nz;
     if (o->log_)
@@ -610,7 +610,11 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
             float v = P_in[i] > 0.001f ? P_in[i] : 0.001f;
             P_in[i] = log10f(v);
         }
+    /* means_in: the per-view means of the whole images when P_in holds only a
+     * band of their rows (a row-subset run); NULL computes them from P_in */
     REAL* means = F(view_means)(o, P_in);
+    if (means_in)
+        for (int p = 0; p < o->nz; p++) means[p] = (REAL)means_in[p];
     REAL* P = (REAL*)malloc(sizeof(REAL) * (size_t)total);
     for (long i = 0; i < total; i++) P[i] = (REAL)P_in[i];
     REAL* O = NULL;

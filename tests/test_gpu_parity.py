@@ -245,6 +245,34 @@ def test_run_rows_equals_full_run(gpu):
             s.run_rows(0, 4)
 
 
+@pytest.mark.parametrize("xtilt", [None, 0.5])
+def test_launch_shapes_do_not_change_results(gpu, xtilt):
+    """Every tile depth and FP bins-per-lane choice gives the same bits (WBP and
+    SIRT); the default comes from the tuned table."""
+    xt = None if xtilt is None else np.full(21, xtilt, np.float32)
+    P, ang = case(gpu, nx=160, ny=10, nviews=21, thick=72, seed=41, xtilt=xt)
+    outs = []
+    for tz, bins in [(0, 0), (32, 4), (64, 8), (128, 16)]:
+        with gpu.Session(160, 10, 21, gpu.TiltOptions(thickness=72, sirt=2)) as s:
+            s.set_angles(ang)
+            if xt is not None:
+                s.set_xtilt(xt)
+            s.set_tuning(tz, bins)
+            s.upload(P)
+            s.run()
+            outs.append(s.read_rows()[0])
+            t = s.tuning()
+            if tz:
+                assert t["tile_z"] == tz and t["tile_source"] == 2 and t["fp_bins"] == bins
+            else:
+                assert t["tile_source"] == 1 and t["fp_source"] == 1  # the tuned table
+    for o in outs[1:]:
+        if xt is None:
+            assert np.array_equal(o, outs[0])
+        else:  # the two-row FP scatters with float atomics: equal to rounding
+            assert np.max(np.abs(o - outs[0])) <= 1e-5 * np.sqrt(np.mean(outs[0] ** 2))
+
+
 def test_sirt_xtilt_scatter(gpu, rel):
     P, ang = case(gpu, nx=96, ny=10, nviews=21, thick=32, seed=10)
     xt = np.full(21, 1.0, np.float32)

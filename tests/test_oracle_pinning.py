@@ -335,6 +335,36 @@ def test_sirt_trace_rows_equal_whole_run():
         assert np.all(np.diff(res) < 0)
 
 
+def test_row_band_with_whole_image_means_equals_whole_run():
+    """A band of projection rows (with a halo for x-axis tilt) and the whole
+    images' means reconstructs those output rows as the whole run: exactly in
+    FP64 (the form the FP64-oracle error of the large benchmark is computed in);
+    in FP32 to the rounding of iy + dx*sin(xt) at a shifted iy."""
+    rng = np.random.default_rng(16)
+    nz, ny, nx = 11, 30, 48
+    P = (rng.random((nz, ny, nx)) + 0.5).astype(np.float32)
+    ang = np.linspace(-50, 50, nz).astype(np.float32)
+    xt = np.full(nz, 2.0, np.float32)
+    sx, sy = rng.normal(0, 1, nz), rng.normal(0, 1, nz)
+    opts = O.TiltOptions(thickness=16)
+    full_grids = O.build_local_grids([(v, 24.0, 15.0, sx[v], sy[v]) for v in range(nz)], nz)
+    whole, _ = O.tilt_run(P, ang, opts, xtilt=xt, local=full_grids)
+    p0, p1 = 4, 22
+    band_grids = O.build_local_grids([(v, 24.0, 15.0 - p0, sx[v], sy[v]) for v in range(nz)], nz)
+    means = np.array([np.mean(P[v], dtype=np.float64) for v in range(nz)])
+    sub = np.ascontiguousarray(P[:, p0:p1])
+    whole64, _ = O.tilt_run(P, ang, opts, xtilt=xt, local=full_grids, prec="f64")
+    band64, _ = O.tilt_run(sub, ang, opts, xtilt=xt, local=band_grids, rows=(6, 12),
+                           means=means, prec="f64")
+    assert np.array_equal(band64, whole64[10:16])
+    band, _ = O.tilt_run(sub, ang, opts, xtilt=xt, local=band_grids, rows=(6, 12), means=means)
+    ref = whole[10:16].astype(np.float64)
+    assert np.max(np.abs(band - ref)) <= 1e-5 * np.sqrt(np.mean(ref ** 2))
+    # the band's own means (fill values) would be wrong by percents
+    own, _ = O.tilt_run(sub, ang, opts, xtilt=xt, local=band_grids, rows=(6, 12))
+    assert np.max(np.abs(own - ref)) > 1e-3 * np.sqrt(np.mean(ref ** 2))
+
+
 def test_fp_threads_do_not_change_bits():
     rng = np.random.default_rng(15)
     V = rng.normal(size=(3, 12, 30)).astype(np.float32)
