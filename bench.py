@@ -342,7 +342,7 @@ def stream_jobs(reqs, with_slabs, seed):
     return jobs
 
 
-def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30):
+def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True):
     t0 = time.perf_counter()
     series = c3_series(tg)
     print(f"# serving inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
@@ -370,6 +370,8 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30):
     out["plain_f32_sequential"] = {"seconds": round(dtp, 3), "requests_per_s": round(n_req / dtp, 3),
                                    "gvoxel_view_per_s": round(work / dtp / 1e9, 2)}
     print(f"# plain sequential: {dtp:.2f} s", file=sys.stderr, flush=True)
+    if not streams:
+        return out
     cap = out["batched_f32"]["requests_per_s"]
     # served request streams at 20 / 60 / 90 % of batched capacity, with slab
     # requests of the resident C1 series mixed in when it is given
@@ -422,6 +424,8 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-only", action="store_true", help="only the serving section")
+    ap.add_argument("--serve-batch-only", action="store_true",
+                    help="only the batched and plain runs of the serving section (for profiles)")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--stream-requests", type=int, default=48)
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
@@ -448,6 +452,10 @@ def main():
         dt, tm = run_gpu(tg, P, ang, xt, local, "f32", 1, 0)
         print(json.dumps({"step_s": dt, **tm}), flush=True)
         return
+    if a.serve_batch_only:
+        print(json.dumps({"serving": serving_section(tg, a.serve_requests, 0, streams=False)}),
+              flush=True)
+        return
     if a.serve_only:
         c1 = c1_inputs(tg)
         print(json.dumps({"serving": serving_section(tg, a.serve_requests, a.stream_requests,
@@ -467,6 +475,7 @@ def main():
     value = vv / (dt / a.steps) / 1e9
     bp_ms = tm["bp_ms"] / max(1, tm["bp_launches"])
     achieved = BP_FLOP_PER_VV * vv / (bp_ms * 1e-3) / 1e12
+    fma_tflops = tg.fma_calibration()  # attainable non-packed FP32 rate, measured
     traffic = load_traffic()
     line = {
         "metric": METRIC,
@@ -503,6 +512,8 @@ def main():
             "traffic": traffic,
             "flop_per_voxel_view": BP_FLOP_PER_VV,
             "launch_ms": round(bp_ms, 3),
+            "fp32_fma_attainable_tflops": round(fma_tflops, 1),
+            "frac_of_attainable": round(achieved / fma_tflops, 4),
         },
     }
     rows16 = None

@@ -295,6 +295,31 @@ __global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ sr
         dst[i] = src[i];
 }
 
+// Independent FP32 FMA chains (eight per lane, one v_fma_f32 each, no packed
+// forms): the attainable non-packed FP32 VALU rate of the device.
+__global__ void __launch_bounds__(256) fma_rate_kernel(float* out, int iters, float a, float b) {
+    float x0 = static_cast<float>(threadIdx.x), x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
+    float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
+    for (int i = 0; i < iters; ++i) {
+        x0 = fmaf(x0, a, b);
+        x1 = fmaf(x1, a, b);
+        x2 = fmaf(x2, a, b);
+        x3 = fmaf(x3, a, b);
+        x4 = fmaf(x4, a, b);
+        x5 = fmaf(x5, a, b);
+        x6 = fmaf(x6, a, b);
+        x7 = fmaf(x7, a, b);
+    }
+    const float s = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+    if (s == 0.123f) out[blockIdx.x * blockDim.x + threadIdx.x] = s;  // keeps the chains live
+}
+
+hipError_t launch_fma_rate(float* out, int blocks, int iters, hipStream_t st) {
+    hipLaunchKernelGGL(fma_rate_kernel, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f,
+                       0.001f);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
     const int64_t n4 = bytes / 16;
     if (n4 <= 0) return hipSuccess;

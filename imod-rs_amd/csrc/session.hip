@@ -1037,6 +1037,41 @@ int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out) {
     return tiltgpu_reconstruct_rows(s, iy, iy + 1, out);
 }
 
+int tiltgpu_fma_calibration(int32_t iters, double* tflops) {
+    if (iters < 1 || !tflops) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");
+    int dev = 0, cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    const int blocks = cus * 32;  // 8 waves per SIMD
+    float* out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TILTGPU_OK;
+    if (hipMalloc(&out, static_cast<size_t>(blocks) * 256 * 4) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(TILTGPU_ERR_HIP, "FMA calibration: allocation failed");
+    } else {
+        float ms = 0.f;
+        if (launch_fma_rate(out, blocks, iters, st) != hipSuccess ||  // warm-up
+            hipEventRecord(e0, st) != hipSuccess || launch_fma_rate(out, blocks, iters, st) != hipSuccess ||
+            hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(TILTGPU_ERR_HIP, "FMA calibration: launch failed");
+        const double fma = static_cast<double>(blocks) * 256 * 8 * iters;
+        *tflops = 2.0 * fma / (static_cast<double>(ms) * 1e-3) / 1e12;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy) {
     if (bytes < 16 || iters < 1 || !ms_per_copy) return fail(TILTGPU_ERR_ARG, "bad arguments");
     int count = 0;

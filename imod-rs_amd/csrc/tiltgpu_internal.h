@@ -208,5 +208,6 @@ hipError_t launch_slice_stats(const float* v, int rows, int64_t per_row, float* 
                               hipStream_t st);
 hipError_t launch_fill_f32(float* p, int64_t n, float value, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+hipError_t launch_fma_rate(float* out, int blocks, int iters, hipStream_t st);
 
 }  // namespace tiltgpu

@@ -144,6 +144,7 @@ SIGNATURES = [
     ("tiltgpu_server_reset_stats", C.c_int, [_vp]),
     ("tiltgpu_server_destroy", None, [_vp]),
     ("tiltgpu_copy_calibration", C.c_int, [C.c_int64, C.c_int32, _dp]),
+    ("tiltgpu_fma_calibration", C.c_int, [C.c_int32, _dp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_parse_local_file", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _fp, _fp,
@@ -532,6 +533,13 @@ def copy_calibration(nbytes, iters=1):
     ms = C.c_double()
     _check(lib().tiltgpu_copy_calibration(int(nbytes), int(iters), C.byref(ms)))
     return ms.value
+
+
+def fma_calibration(iters=1 << 14):
+    """Attainable non-packed FP32 FMA rate of the device, TFLOP/s."""
+    t = C.c_double()
+    _check(lib().tiltgpu_fma_calibration(iters, C.byref(t)))
+    return t.value
 
 
 class Server:

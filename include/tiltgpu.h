@@ -264,6 +264,10 @@ void tiltgpu_server_destroy(tiltgpu_server* s);
  * byte count calibrates hardware traffic counters and gives the attainable
  * HBM bandwidth of the device. */
 int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy);
+/* Diagnostic (no counterpart in the reference): the attainable FP32 VALU rate of
+ * the device with non-packed FMAs (the form every kernel here is built with),
+ * from `iters` iterations of eight independent FMA chains per lane; TFLOP/s. */
+int tiltgpu_fma_calibration(int32_t iters, double* tflops);
 
 /* -------------------------------------------------------- host utilities */
 
