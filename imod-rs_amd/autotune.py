@@ -2,7 +2,7 @@
 
 Measures, for every backprojection kernel variant (single-row; two-row with
 x-axis tilt; two-row with local shifts) x projection storage (f32, f16) x width
-x thickness class, the backprojection time at each tile depth (32, 64, 128),
+x thickness class, the backprojection time at each tile depth (32, 64, 128, 256),
 and, per width, the forward-projection time at 4, 8 and 16 bins per lane.  The
 measurements are written as DIR/autotune.json and turned into the C++ table
 csrc/tile_table.inc (compiled into the library; sessions and the server's
@@ -33,7 +33,7 @@ VARIANTS = [  # (name, two_row, local)
 ]
 WIDTHS = [256, 512, 1024, 2048]
 THICK = [64, 128, 256, 512]
-TILES = [32, 64, 128]
+TILES = [32, 64, 128, 256]
 FP_WIDTHS = [256, 384, 512, 768, 1024, 1536, 2048]
 FP_BINS = [4, 8, 16]
 
@@ -106,8 +106,8 @@ def emit(meas, out):
           "forward projection: median ms of one reprojection of a w x w x 128 volume",
           "at 61 views.", "",
           "## Backprojection tile depth", "",
-          "| variant | storage | width | thickness | tz=32 | tz=64 | tz=128 | fastest |",
-          "|---|---|---|---|---|---|---|---|"]
+          "| variant | storage | width | thickness | tz=32 | tz=64 | tz=128 | tz=256 | fastest |",
+          "|---|---|---|---|---|---|---|---|---|"]
     rows = []
     groups = {}
     for m in meas["bp"]:
@@ -119,7 +119,7 @@ def emit(meas, out):
         md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | {m['thickness']} | "
                   + " | ".join(cells) + f" | {best} |")
     md += ["", "Per-padded-slice cost k(tz), relative to k(128) = 1000:", "",
-           "| two_row | local | half | width | k32 | k64 | k128 |", "|---|---|---|---|---|---|---|"]
+           "| two_row | local | half | width | k32 | k64 | k128 | k256 |", "|---|---|---|---|---|---|---|---|"]
     for key, ms in sorted(groups.items()):
         k = {}
         for tz in TILES:
@@ -128,7 +128,7 @@ def emit(meas, out):
         base = k[128] if k[128] else min(v for v in k.values() if v)
         kk = [int(round(1000 * k[tz] / base)) if k[tz] else 0 for tz in TILES]
         rows.append((*key, *kk))
-        md.append("| %d | %d | %d | %d | %d | %d | %d |" % (*key, *kk))
+        md.append("| %d | %d | %d | %d | %d | %d | %d | %d |" % (*key, *kk))
     md += ["", "## Forward-projection bins per lane", "",
            "| width | 4 | 8 | 16 | chosen |", "|---|---|---|---|---|"]
     fp_rows = []
@@ -139,12 +139,12 @@ def emit(meas, out):
         md.append(f"| {m['width']} | " + " | ".join(f"{ms[b]:.3f}" for b in FP_BINS) + f" | {best} |")
     with open(os.path.join(out, "tile_table.inc"), "w") as fh:
         fh.write("// Measured launch shapes for gfx950, written by imod-rs_amd/autotune.py.\n")
-        fh.write("// Backprojection: per-padded-slice cost of tile depth 32, 64, 128 (relative,\n")
-        fh.write("// k128 = 1000) per kernel variant, storage and width.\n")
-        fh.write("// {two_row, local, half, in_nx, k32, k64, k128}\n")
+        fh.write("// Backprojection: per-padded-slice cost of tile depth 32, 64, 128, 256 (relative,\n")
+        fh.write("// k128 = 1000; 0 = not measured) per kernel variant, storage and width.\n")
+        fh.write("// {two_row, local, half, in_nx, k32, k64, k128, k256}\n")
         fh.write("static const BpTune kBpTable[] = {\n")
         for r in rows:
-            fh.write("    {%d, %d, %d, %d, %d, %d, %d},\n" % r)
+            fh.write("    {%d, %d, %d, %d, %d, %d, %d, %d},\n" % r)
         fh.write("};\n// Forward projection: fastest bins per lane per width.  {in_nx, bins_per_lane}\n")
         fh.write("static const FpTune kFpTable[] = {\n")
         for r in fp_rows:

@@ -353,8 +353,11 @@ __device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int 
 // XCD; a workgroup drains its own XCD's chunk (L2 locality: consecutive tiles
 // share projection rows), then steals from the others.  Every workgroup visits
 // every chunk once and exits when all are drained.
+// Register budget: 4 waves per SIMD (<= 128 VGPRs) up to 32 z per lane; the
+// 64-z tile keeps 3 (<= 168).
 template <int ZPT, bool TWO, bool LOCAL, int ST, bool BATCH>
-__global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P1, BpBatch B) {
+__global__ void __launch_bounds__(kBpThreads)
+    __attribute__((amdgpu_waves_per_eu(ZPT > 32 ? 3 : 4, 8))) bp_kernel(BpParams P1, BpBatch B) {
     extern __shared__ float4 lds4[];
     __shared__ int s_tile;
     constexpr int TZ = 4 * ZPT;
@@ -443,6 +446,8 @@ __global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P1, BpBatch B) 
                     if (cur.row >= 0) {
                         const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
                         const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
+                        // the fences keep the scheduler from hoisting every voxel's
+                        // 16-byte LDS read at once (registers, not issue, limit it)
                         if (LOCAL && cur.shift) {
     #pragma unroll
                             for (int k = 0; k < ZPT; ++k) {
@@ -450,6 +455,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P1, BpBatch B) 
                                 const float4 q = E[cvt_floor(px)];
                                 const float f = __builtin_amdgcn_fractf(px);
                                 acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
                             }
                         } else {
     #pragma unroll
@@ -458,6 +464,7 @@ __global__ void __launch_bounds__(kBpThreads) bp_kernel(BpParams P1, BpBatch B) 
                                 const float4 q = E[cvt_floor(px)];
                                 const float f = __builtin_amdgcn_fractf(px);
                                 acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
                             }
                         }
                     }
@@ -566,6 +573,7 @@ hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, 
         case 32: return launch_bp_batch_z<8>(b, storage, two_row, local, grid, lds, st);
         case 64: return launch_bp_batch_z<16>(b, storage, two_row, local, grid, lds, st);
         case 128: return launch_bp_batch_z<32>(b, storage, two_row, local, grid, lds, st);
+        case 256: return launch_bp_batch_z<64>(b, storage, two_row, local, grid, lds, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -576,6 +584,7 @@ hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int t
         case 32: return launch_bp_z<8>(p, storage, two_row, st);
         case 64: return launch_bp_z<16>(p, storage, two_row, st);
         case 128: return launch_bp_z<32>(p, storage, two_row, st);
+        case 256: return launch_bp_z<64>(p, storage, two_row, st);
         default: return hipErrorInvalidValue;
     }
 }

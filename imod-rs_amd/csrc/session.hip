@@ -92,10 +92,10 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
     const int max_lds_bytes = 64 * 1024;
     int chosen = 0;
     const int out_nz0 = s->d.out_nz;
-    std::vector<int> order = {128, 64, 32};
+    std::vector<int> order = {256, 128, 64, 32};
     auto cost = [&](int tz) {
         const double pad = static_cast<double>((out_nz0 + tz - 1) / tz * tz);
-        return pad * (1.0 + 16.0 / tz);
+        return pad * (1.0 + 32.0 / tz);
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
     s->tile_source = kTuneHeuristic;
@@ -473,7 +473,7 @@ int validate_dims(const tiltgpu_dims* d) {
 int init_device(tiltgpu_session* s, hipStream_t external) {
     if (const char* tz = std::getenv("TILTGPU_TILE_Z")) {
         const int v = std::atoi(tz);
-        if (v == 32 || v == 64 || v == 128) s->tile_z_override = v;
+        if (v == 32 || v == 64 || v == 128 || v == 256) s->tile_z_override = v;
     }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -829,8 +829,9 @@ int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n) {
 
 int tiltgpu_set_tuning(tiltgpu_session* s, int32_t bp_tile_z, int32_t fp_bins_per_lane) {
     if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
-    if (bp_tile_z != 0 && bp_tile_z != 32 && bp_tile_z != 64 && bp_tile_z != 128)
-        return fail(TILTGPU_ERR_ARG, "tile depth %d is not 0, 32, 64 or 128", bp_tile_z);
+    if (bp_tile_z != 0 && bp_tile_z != 32 && bp_tile_z != 64 && bp_tile_z != 128 &&
+        bp_tile_z != 256)
+        return fail(TILTGPU_ERR_ARG, "tile depth %d is not 0, 32, 64, 128 or 256", bp_tile_z);
     if (fp_bins_per_lane != 0 && fp_bins_per_lane != 4 && fp_bins_per_lane != 8 &&
         fp_bins_per_lane != 16)
         return fail(TILTGPU_ERR_ARG, "bins per lane %d is not 0, 4, 8 or 16", fp_bins_per_lane);

@@ -7,7 +7,7 @@ namespace tiltgpu {
 namespace {
 
 struct BpTune {
-    int two_row, local, half, in_nx, k32, k64, k128;  // per-padded-slice cost of each depth
+    int two_row, local, half, in_nx, k32, k64, k128, k256;  // per-padded-slice cost of each depth
 };
 struct FpTune {
     int in_nx, bins;
@@ -32,11 +32,11 @@ int tuned_tile_z(int two_row, int local, int half, int in_nx, int out_nz) {
     }
     if (!best) return 0;
     // time ~ padded depth x per-slice cost of the depth
-    const int tz[3] = {32, 64, 128};
-    const int k[3] = {best->k32, best->k64, best->k128};
+    const int tz[4] = {32, 64, 128, 256};
+    const int k[4] = {best->k32, best->k64, best->k128, best->k256};
     int choice = 0;
     double bt = 1e300;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         if (k[i] <= 0) continue;
         const double t = static_cast<double>((out_nz + tz[i] - 1) / tz[i] * tz[i]) * k[i];
         if (t < bt) {

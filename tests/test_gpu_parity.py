@@ -252,7 +252,7 @@ def test_launch_shapes_do_not_change_results(gpu, xtilt):
     xt = None if xtilt is None else np.full(21, xtilt, np.float32)
     P, ang = case(gpu, nx=160, ny=10, nviews=21, thick=72, seed=41, xtilt=xt)
     outs = []
-    for tz, bins in [(0, 0), (32, 4), (64, 8), (128, 16)]:
+    for tz, bins in [(0, 0), (32, 4), (64, 8), (128, 16), (256, 8)]:
         with gpu.Session(160, 10, 21, gpu.TiltOptions(thickness=72, sirt=2)) as s:
             s.set_angles(ang)
             if xt is not None:
@@ -348,7 +348,7 @@ def test_c0_config(gpu, rel):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("tile_z", [32, 64, 128])
+@pytest.mark.parametrize("tile_z", [32, 64, 128, 256])
 @pytest.mark.parametrize("geom", ["plain", "xtilt", "local"])
 def test_tile_depths(gpu, rel, monkeypatch, tile_z, geom):
     """Every backprojection tile depth gives the oracle's result (one-row,
