@@ -7,8 +7,8 @@
 // (filtered a) + i*(filtered b) and one complex transform pair filters both
 // rows exactly.  The inverse transform uses IDFT(Z) = conj(DFT(conj(Z))), so
 // only the forward self-sorting Stockham kernel exists (radix 4, 2, 3, 5 and a
-// generic small-prime radix), run entirely in LDS with twiddles from a table
-// computed in double on the host.
+// generic radix for any other prime), run entirely in LDS with twiddles from a
+// table computed in double on the host.
 //
 // The stretch then reads the filtered rows from LDS at
 // src_x = (ox - c) * (1/cos theta) + c with the reference's rounding
@@ -104,32 +104,35 @@ __device__ __forceinline__ void stockham_pass(const float2* __restrict__ in,
     }
 }
 
-// Generic radix for a small prime p (p <= 31): O(p^2) butterfly with the
-// twiddle table (W_p^e = tw[e * n/p]).
+// Generic radix: any prime R the unrolled passes do not cover.  Each output of
+// each butterfly is one work item (an O(R) sum with W_R^e = tw[e * n/R]), so a
+// large prime spreads over the threads.
 __device__ __noinline__ void stockham_pass_generic(const float2* __restrict__ in,
                                                    float2* __restrict__ out, int n, int pairs,
                                                    int Ns, int R, const float2* __restrict__ tw,
-                                                   FastDiv mdiv, FastDiv nsdiv) {
+                                                   FastDiv mdiv, FastDiv nsdiv, FastDiv rdiv) {
     const int m = n / R;
     const int tstep = n / (Ns * R);
     const int pstep = n / R;
-    for (int idx = threadIdx.x; idx < pairs * m; idx += blockDim.x) {
+    const int items = pairs * m * R;
+    for (int w = threadIdx.x; w < items; w += blockDim.x) {
+        const int idx = static_cast<int>(rdiv.div(static_cast<uint32_t>(w)));
+        const int qq = w - idx * R;
         const int q = static_cast<int>(mdiv.div(static_cast<uint32_t>(idx)));
         const int j = idx - q * m;
         const int jq = static_cast<int>(nsdiv.div(static_cast<uint32_t>(j)));
         const int k = j - jq * Ns;
         const float2* src = in + q * n;
-        float2* dst = out + q * n + jq * Ns * R + k;
-        for (int qq = 0; qq < R; ++qq) {
-            float2 s = make_float2(0.f, 0.f);
-            for (int r = 0; r < R; ++r) {
-                float2 v = src[j + r * m];
-                if (Ns > 1 && r > 0) v = cmul(v, tw[r * k * tstep]);
-                const int e = (r * qq) % R;
-                s = cadd(s, cmul(v, tw[e * pstep]));
-            }
-            dst[qq * Ns] = s;
+        float2 acc = make_float2(0.f, 0.f);
+        int e = 0;  // (r * qq) mod R
+        for (int r = 0; r < R; ++r) {
+            float2 v = src[j + r * m];
+            if (Ns > 1 && r > 0) v = cmul(v, tw[r * k * tstep]);
+            acc = cadd(acc, cmul(v, tw[e * pstep]));
+            e += qq;
+            if (e >= R) e -= R;
         }
+        out[q * n + jq * Ns * R + k + qq * Ns] = acc;
     }
 }
 
@@ -152,7 +155,7 @@ __device__ float2* fft_forward(float2* b0, float2* b1, const FilterParams& P, in
         else if (R == 5)
             stockham_pass<5>(in, out, P.n, pairs, Ns, P.tw, md, nd);
         else
-            stockham_pass_generic(in, out, P.n, pairs, Ns, R, P.tw, md, nd);
+            stockham_pass_generic(in, out, P.n, pairs, Ns, R, P.tw, md, nd, P.r_div[s]);
         __syncthreads();
         Ns *= R;
         float2* t = in;
@@ -328,10 +331,11 @@ int filter_plan(int n, int* radix, int* nradix) {
     if (n < 2) return -1;
     while (m % 4 == 0 && k < kMaxRadix) { radix[k++] = 4; m /= 4; }
     while (m % 2 == 0 && k < kMaxRadix) { radix[k++] = 2; m /= 2; }
-    for (int p = 3; p <= 31 && m > 1; p += 2) {
+    for (int p = 3; p * p <= m && k < kMaxRadix; p += 2) {
         while (m % p == 0 && k < kMaxRadix) { radix[k++] = p; m /= p; }
     }
-    if (m != 1) return -1;  // a prime factor above 31: unsupported width
+    if (m > 1 && k < kMaxRadix) { radix[k++] = m; m = 1; }  // the remaining prime factor
+    if (m != 1) return -1;
     *nradix = k;
     return 0;
 }

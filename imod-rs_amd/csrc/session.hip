@@ -236,10 +236,7 @@ int prepare_pipeline_geometry(tiltgpu_session* s) {
 int prepare_filter(tiltgpu_session* s) {
     const int n = s->d.in_nx;
     if (filter_plan(n, s->radix, &s->nradix) != 0)
-        return fail(TILTGPU_ERR_ARG,
-                    "projection width %d has a prime factor above 31; the LDS FFT "
-                    "supports widths whose prime factors are all <= 31",
-                    n);
+        return fail(TILTGPU_ERR_ARG, "projection width %d is not supported by the LDS FFT", n);
     if (filter_lds_bytes(n, pick_pairs_per_wg(n)) > 160 * 1024)
         return fail(TILTGPU_ERR_ARG, "projection width %d too large for the LDS FFT", n);
     std::vector<float> w(n);
@@ -348,6 +345,7 @@ FilterParams make_filter(tiltgpu_session* s, const float* src, const float* sub,
         f.radix[i] = s->radix[i];
         f.m_div[i] = make_fastdiv(static_cast<uint32_t>(n / s->radix[i]));
         f.ns_div[i] = make_fastdiv(static_cast<uint32_t>(ns));
+        f.r_div[i] = make_fastdiv(static_cast<uint32_t>(s->radix[i]));
         ns *= s->radix[i];
     }
     return f;

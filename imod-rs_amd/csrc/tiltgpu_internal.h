@@ -123,6 +123,7 @@ struct FilterParams {
     FastDiv n_div;                  // by n
     FastDiv m_div[kMaxRadix];       // by n / radix[s] (butterflies per row of pass s)
     FastDiv ns_div[kMaxRadix];      // by radix[0] * ... * radix[s-1] (pass s stride)
+    FastDiv r_div[kMaxRadix];       // by radix[s] (outputs per butterfly, generic passes)
 };
 
 // Ordered-walk forward projection (no x-axis tilt, no local alignment).
