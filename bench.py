@@ -65,8 +65,8 @@ def load_traffic():
 ERR_ROWS = (992, 1056)  # the 64-slice subset the FP64-oracle error is measured on
 
 
-def run_gpu(tg, P, ang, xt, local, storage, steps, warmup, keep_rows=None):
-    opts = tg.TiltOptions(thickness=512, radial=(0.35, 0.05), storage=storage)
+def run_gpu(tg, P, ang, xt, local, storage, steps, warmup, keep_rows=None, geometry="f32"):
+    opts = tg.TiltOptions(thickness=512, radial=(0.35, 0.05), storage=storage, geometry=geometry)
     s = tg.Session(P.shape[2], P.shape[1], P.shape[0], opts)
     s.set_angles(ang)
     s.set_xtilt(xt)
@@ -534,6 +534,10 @@ def main():
         if rows16 is not None:
             err["f16_storage"] = err_vs(rows16, ref64)
             err["f16_vs_f32_gpu"] = err_vs(rows16, rows32)
+        # the per-view tables computed in FP64 on the GPU (one step each)
+        for st in ("f32",) + (() if a.no_fp16 else ("f16",)):
+            r = run_gpu(tg, P, ang, xt, local, st, 1, 0, keep_rows=ERR_ROWS, geometry="f64")[2]
+            err[f"{st}_storage_f64_geometry"] = err_vs(r, ref64)
         line["error_vs_f64_oracle"] = err
         print(f"# error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
     if not a.no_cpu:

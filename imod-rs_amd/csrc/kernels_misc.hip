@@ -295,6 +295,68 @@ __global__ void __launch_bounds__(256) copy_kernel(const float4* __restrict__ sr
         dst[i] = src[i];
 }
 
+// ---------------------------------------------------------- FP64 geometry
+// The per-view tables of ProjParams (main.rs:682-700) and the stretch factors
+// (main.rs:449-459) computed in FP64 and rounded once to FP32, for the precision
+// form (the reference computes them in FP32).  view_out: 5 floats per view
+// {cos t, sin t, cos xt, sin xt, zfactor}; stretch_out: one per section.
+__global__ void geometry_f64_kernel(const float* angles, int n_angles, const float* xtilt,
+                                    int n_xtilt, const float* zfac, int n_zfac, int nviews,
+                                    int nsec, float max_tilt, int stretch, float* view_out,
+                                    float* stretch_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double d2r = 3.14159265358979323846 / 180.0;
+    if (i < nviews) {
+        const double t = static_cast<double>(angles[i]) * d2r;
+        const double xt = (xtilt && i < n_xtilt) ? static_cast<double>(xtilt[i]) * d2r : 0.0;
+        view_out[5 * i + 0] = static_cast<float>(cos(t));
+        view_out[5 * i + 1] = static_cast<float>(sin(t));
+        view_out[5 * i + 2] = static_cast<float>(cos(xt));
+        view_out[5 * i + 3] = static_cast<float>(sin(xt));
+        view_out[5 * i + 4] = (zfac && i < n_zfac) ? zfac[i] : 1.0f;
+    }
+    if (i < nsec) {
+        float st = 0.f;
+        if (stretch && i < n_angles && fabsf(angles[i]) <= max_tilt) {
+            const double c = cos(static_cast<double>(angles[i]) * d2r);
+            if (fabs(c) >= 1e-6) st = static_cast<float>(1.0 / c);
+        }
+        stretch_out[i] = st;
+    }
+}
+
+// zc[v][oz] = (dz * sin t) * zfactor with dz = oz - out_nz/2, in FP64.
+__global__ void zc_f64_kernel(const float* angles, const float* zfac, int n_zfac, int nviews,
+                              int nz_pad, int out_nz, float* zc) {
+    const int oz = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (oz >= nz_pad || v >= nviews) return;
+    const double s = sin(static_cast<double>(angles[v]) * (3.14159265358979323846 / 180.0));
+    const double zf = (zfac && v < n_zfac) ? static_cast<double>(zfac[v]) : 1.0;
+    const double dz = static_cast<double>(oz) - static_cast<double>(out_nz) / 2.0;
+    zc[static_cast<int64_t>(v) * nz_pad + oz] = static_cast<float>(dz * s * zf);
+}
+
+hipError_t launch_geometry_f64(const float* angles, int n_angles, const float* xtilt,
+                               int n_xtilt, const float* zfac, int n_zfac, int nviews, int nsec,
+                               float max_tilt, int stretch, float* view_out, float* stretch_out,
+                               hipStream_t st) {
+    const int n = nviews > nsec ? nviews : nsec;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geometry_f64_kernel, dim3((n + 63) / 64), dim3(64), 0, st, angles,
+                       n_angles, xtilt, n_xtilt, zfac, n_zfac, nviews, nsec, max_tilt, stretch,
+                       view_out, stretch_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_f64(const float* angles, const float* zfac, int n_zfac, int nviews,
+                         int nz_pad, int out_nz, float* zc, hipStream_t st) {
+    if (nviews <= 0 || nz_pad <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zc_f64_kernel, dim3((nz_pad + 255) / 256, nviews), dim3(256), 0, st,
+                       angles, zfac, n_zfac, nviews, nz_pad, out_nz, zc);
+    return hipGetLastError();
+}
+
 // Independent FP32 FMA chains (eight per lane, one v_fma_f32 each, no packed
 // forms): the attainable non-packed FP32 VALU rate of the device.
 __global__ void __launch_bounds__(256) fma_rate_kernel(float* out, int iters, float a, float b) {

@@ -203,30 +203,87 @@ int upload_local(tiltgpu_session* s) {
     return TILTGPU_OK;
 }
 
+// The view tables and stretch factors in FP64 on the device, rounded once to
+// FP32 (TILTGPU_GEOMETRY_F64).  Leaves the angles and z factors on the device
+// for the z-term table (dev_angles, dev_zfac).
+static int geometry_f64_tables(tiltgpu_session* s, int nv, std::vector<tiltgpu_view_params>& vp,
+                               std::vector<float>& st, float** dev_angles, float** dev_zfac) {
+    const int nz = s->d.nz;
+    const int na = static_cast<int>(s->angles.size());
+    const int nxt = s->has_xtilt ? static_cast<int>(s->xtilt.size()) : 0;
+    const int nzf = s->has_zfac ? static_cast<int>(s->zfac.size()) : 0;
+    float *d_xt = nullptr, *d_view = nullptr, *d_st = nullptr;
+    int rc;
+    if ((rc = dev_alloc(s, dev_angles, std::max(na, 1))) ||
+        (rc = dev_alloc(s, &d_xt, std::max(nxt, 1))) ||
+        (rc = dev_alloc(s, dev_zfac, std::max(nzf, 1))) ||
+        (rc = dev_alloc(s, &d_view, 5 * std::max(nv, 1))) ||
+        (rc = dev_alloc(s, &d_st, std::max(nz, 1))))
+        return rc;
+    HIP_TRY(hipMemcpy(*dev_angles, s->angles.data(), na * 4, hipMemcpyHostToDevice));
+    if (nxt) HIP_TRY(hipMemcpy(d_xt, s->xtilt.data(), nxt * 4, hipMemcpyHostToDevice));
+    if (nzf) HIP_TRY(hipMemcpy(*dev_zfac, s->zfac.data(), nzf * 4, hipMemcpyHostToDevice));
+    HIP_TRY(launch_geometry_f64(*dev_angles, na, nxt ? d_xt : nullptr, nxt, nzf ? *dev_zfac : nullptr,
+                                nzf, nv, nz, s->o.max_cosine_tilt, s->o.no_cosine_stretch ? 0 : 1,
+                                d_view, d_st, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    static_assert(sizeof(tiltgpu_view_params) == 5 * sizeof(float), "view params layout");
+    HIP_TRY(hipMemcpy(vp.data(), d_view, nv * sizeof(tiltgpu_view_params), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(st.data(), d_st, nz * 4, hipMemcpyDeviceToHost));
+    dev_free(s, d_xt);
+    dev_free(s, d_view);
+    dev_free(s, d_st);
+    return TILTGPU_OK;
+}
+
 // Geometry of the full pipeline from the angle / xtilt / zfactor inputs.
 int prepare_pipeline_geometry(tiltgpu_session* s) {
     if (!s->has_angles) return fail(TILTGPU_ERR_STATE, "tilt angles not set");
     const int nz = s->d.nz;
     const int nv = std::min(nz, static_cast<int>(s->angles.size()));
+    if (!s->o.no_cosine_stretch && static_cast<int>(s->angles.size()) < nz)
+        return fail(TILTGPU_ERR_ARG,
+                    "cosine stretch needs one tilt angle per section (%zu angles, %d "
+                    "sections); the reference panics here (main.rs:450)",
+                    s->angles.size(), nz);
     std::vector<tiltgpu_view_params> vp(std::max(nv, 1));
-    int rc = tiltgpu_proj_params(s->angles.data(), nv, s->has_xtilt ? s->xtilt.data() : nullptr,
+    // stretch factors for every section (main.rs:449 indexes tilt_angles[p], p < nz)
+    std::vector<float> st(std::max(nz, 1), 0.f);
+    const bool f64 = s->o.geometry == TILTGPU_GEOMETRY_F64;
+    float *d_ang = nullptr, *d_zf = nullptr;
+    int rc;
+    if (f64) {
+        if ((rc = geometry_f64_tables(s, nv, vp, st, &d_ang, &d_zf))) return rc;
+    } else {
+        rc = tiltgpu_proj_params(s->angles.data(), nv, s->has_xtilt ? s->xtilt.data() : nullptr,
                                  static_cast<int32_t>(s->xtilt.size()),
                                  s->has_zfac ? s->zfac.data() : nullptr,
                                  static_cast<int32_t>(s->zfac.size()), vp.data());
-    if (rc) return rc;
-    vp.resize(nv);
-    if ((rc = setup_geometry(s, vp))) return rc;
-    if ((rc = upload_local(s))) return rc;
-    // stretch factors for every section (main.rs:449 indexes tilt_angles[p], p < nz)
-    std::vector<float> st(std::max(nz, 1), 0.f);
-    if (!s->o.no_cosine_stretch) {
-        if (static_cast<int>(s->angles.size()) < nz)
-            return fail(TILTGPU_ERR_ARG,
-                        "cosine stretch needs one tilt angle per section (%zu angles, %d "
-                        "sections); the reference panics here (main.rs:450)",
-                        s->angles.size(), nz);
-        for (int p = 0; p < nz; ++p) st[p] = stretch_factor(s->angles[p], s->o.max_cosine_tilt);
+        if (rc) return rc;
+        if (!s->o.no_cosine_stretch)
+            for (int p = 0; p < nz; ++p) st[p] = stretch_factor(s->angles[p], s->o.max_cosine_tilt);
     }
+    vp.resize(nv);
+    s->vp_host = vp;
+    rc = setup_geometry(s, vp);
+    if (f64) {
+        // the z terms in FP64 as well (setup_geometry wrote the FP32 ones)
+        hipError_t e = hipSuccess;
+        if (rc == TILTGPU_OK) {
+            e = launch_zc_f64(d_ang, s->has_zfac ? d_zf : nullptr,
+                              s->has_zfac ? static_cast<int>(s->zfac.size()) : 0, nv, s->nz_pad,
+                              s->d.out_nz, s->zc, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        }
+        dev_free(s, d_ang);
+        dev_free(s, d_zf);
+        if (e != hipSuccess) {
+            s->geometry_ready = false;
+            return fail(TILTGPU_ERR_HIP, "FP64 z-term table: %s", hipGetErrorString(e));
+        }
+    }
+    if (rc) return rc;
+    if ((rc = upload_local(s))) return rc;
     dev_free(s, s->stretch);
     if ((rc = dev_alloc(s, &s->stretch, st.size()))) return rc;
     HIP_TRY(hipMemcpy(s->stretch, st.data(), st.size() * 4, hipMemcpyHostToDevice));
@@ -511,6 +568,10 @@ int session_create(const tiltgpu_dims* dims, const tiltgpu_options* opts, hipStr
         delete s;
         return fail(TILTGPU_ERR_ARG, "unknown storage %d", s->o.storage);
     }
+    if (s->o.geometry != TILTGPU_GEOMETRY_F32 && s->o.geometry != TILTGPU_GEOMETRY_F64) {
+        delete s;
+        return fail(TILTGPU_ERR_ARG, "unknown geometry precision %d", s->o.geometry);
+    }
     if (s->o.fill_mode < 0 || s->o.fill_mode > 2) s->o.fill_mode = TILTGPU_FILL_MEAN;
     if ((rc = init_device(s, stream)) || (rc = prepare_filter(s))) {
         tiltgpu_destroy(s);
@@ -578,6 +639,20 @@ void tiltgpu_default_options(tiltgpu_options* o) {
     o->outadd = 0.0f;
     o->sirt_relax = 1.0f;
     o->storage = TILTGPU_STORE_F32;
+}
+
+int tiltgpu_geometry_tables(tiltgpu_session* s, tiltgpu_view_params* views, float* stretch,
+                            float* zc) {
+    if (!s) return fail(TILTGPU_ERR_ARG, "null session");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    const int nv = s->nviews, out_nz = s->d.out_nz;
+    if (views && nv) std::memcpy(views, s->vp_host.data(), nv * sizeof(tiltgpu_view_params));
+    if (stretch) HIP_TRY(hipMemcpy(stretch, s->stretch, s->d.nz * sizeof(float), hipMemcpyDeviceToHost));
+    if (zc && nv && out_nz)
+        HIP_TRY(hipMemcpy2D(zc, out_nz * sizeof(float), s->zc, s->nz_pad * sizeof(float),
+                            out_nz * sizeof(float), nv, hipMemcpyDeviceToHost));
+    return TILTGPU_OK;
 }
 
 int tiltgpu_proj_params(const float* angles, int32_t n, const float* xtilt, int32_t n_xtilt,

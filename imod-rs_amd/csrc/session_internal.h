@@ -61,6 +61,7 @@ struct tiltgpu_session {
     bool has_xtilt = false, has_zfac = false, has_angles = false;
     int nviews = 0;
     bool geometry_ready = false;
+    std::vector<tiltgpu_view_params> vp_host;  // the view tables of the prepared geometry
 
     // local alignment (host copy)
     bool has_local = false;

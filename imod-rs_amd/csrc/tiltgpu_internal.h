@@ -210,5 +210,11 @@ hipError_t launch_slice_stats(const float* v, int rows, int64_t per_row, float* 
 hipError_t launch_fill_f32(float* p, int64_t n, float value, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 hipError_t launch_fma_rate(float* out, int blocks, int iters, hipStream_t st);
+hipError_t launch_geometry_f64(const float* angles, int n_angles, const float* xtilt,
+                               int n_xtilt, const float* zfac, int n_zfac, int nviews, int nsec,
+                               float max_tilt, int stretch, float* view_out, float* stretch_out,
+                               hipStream_t st);
+hipError_t launch_zc_f64(const float* angles, const float* zfac, int n_zfac, int nviews,
+                         int nz_pad, int out_nz, float* zc, hipStream_t st);
 
 }  // namespace tiltgpu

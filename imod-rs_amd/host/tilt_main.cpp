@@ -37,6 +37,7 @@ struct Args {
     bool hamming = false;
     int fake_sirt = 0;
     std::string storage = "f32";
+    std::string geometry = "f32";
     int slab_rows = 64;
 };
 
@@ -69,6 +70,8 @@ struct Args {
                  "      --hamming              Hamming window instead of Gaussian roll-off\n"
                  "      --fake-sirt <n>        low-pass passes after reconstruction [0]\n"
                  "      --storage <s>          f32 | f16 | bf16 projection storage in HBM [f32]\n"
+                 "      --geometry <g>         f32 (the reference's tables) | f64 (computed in\n"
+                 "                             FP64 on the GPU, rounded once) [f32]\n"
                  "      --slab-rows <n>        output rows per device->file transfer [64]\n");
     std::exit(2);
 }
@@ -119,6 +122,7 @@ Args parse_args(int argc, char** argv) {
         {"hamming", 0, 0, &a.hamming},
         {"fake-sirt", 0, 2, &a.fake_sirt},
         {"storage", 0, 1, &a.storage},
+        {"geometry", 0, 1, &a.geometry},
         {"slab-rows", 0, 2, &a.slab_rows},
     };
     for (int i = 1; i < argc; ++i) {
@@ -250,6 +254,7 @@ int main(int argc, char** argv) {
     o.fake_sirt = a.fake_sirt;
     const std::string st = lower(a.storage);
     o.storage = st == "f16" ? TILTGPU_STORE_F16 : st == "bf16" ? TILTGPU_STORE_BF16 : TILTGPU_STORE_F32;
+    o.geometry = lower(a.geometry) == "f64" ? TILTGPU_GEOMETRY_F64 : TILTGPU_GEOMETRY_F32;
 
     // projections (main.rs:601-605)
     std::vector<float> proj(static_cast<size_t>(nz) * in_nx * in_ny);

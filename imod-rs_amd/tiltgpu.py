@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libtiltgpu.so")
 OK, ERR_ARG, ERR_HIP, ERR_NODEV, ERR_BUDGET, ERR_STATE, ERR_IO, ERR_KERNEL = 0, -1, -2, -3, -4, -5, -6, -7
 FILL = {"mean": 0, "zero": 1, "edge": 2}
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+GEOMETRY = {"f32": 0, "f64": 1}
 
 
 class TiltGpuError(RuntimeError):
@@ -48,7 +49,7 @@ class Options(C.Structure):
                 ("log", C.c_int32), ("fill_mode", C.c_int32),
                 ("densweight", C.c_float), ("outscale", C.c_float), ("outadd", C.c_float),
                 ("sirt", C.c_int32), ("sirt_relax", C.c_float), ("fake_sirt", C.c_int32),
-                ("storage", C.c_int32)]
+                ("storage", C.c_int32), ("geometry", C.c_int32)]
 
 
 class ViewParams(C.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = [
     ("tiltgpu_default_options", None, [C.POINTER(Options)]),
     ("tiltgpu_proj_params", C.c_int, [_fp, C.c_int32, _fp, C.c_int32, _fp, C.c_int32,
                                       C.POINTER(ViewParams)]),
+    ("tiltgpu_geometry_tables", C.c_int, [_vp, C.POINTER(ViewParams), _fp, _fp]),
     ("tiltgpu_radial_weights", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_float,
                                          C.c_int32, _fp]),
     ("tiltgpu_create", C.c_int, [C.POINTER(Dims), C.POINTER(Options), C.POINTER(_vp)]),
@@ -214,6 +216,7 @@ class TiltOptions:
     hamming: bool = False
     fake_sirt: int = 0
     storage: str = "f32"
+    geometry: str = "f32"  # per-view tables: "f32" (the reference's) or "f64" (GPU, rounded once)
 
     def to_c(self) -> Options:
         o = Options()
@@ -230,6 +233,7 @@ class TiltOptions:
         o.densweight, o.outscale, o.outadd = self.densweight, self.outscale, self.outadd
         o.sirt, o.sirt_relax, o.fake_sirt = int(self.sirt), float(self.sirt_relax), int(self.fake_sirt)
         o.storage = STORAGE[self.storage]
+        o.geometry = GEOMETRY[self.geometry]
         return o
 
 
@@ -258,6 +262,7 @@ class Session:
         self.out_nz = self.opts.thickness if self.opts.thickness > 0 else in_nx
         self.dims = Dims(in_nx, in_ny, nz, self.out_nx, self.out_nz)
         self._trace_rows = (0, 0)
+        self._n_angles = 0
         self._copts = self.opts.to_c()
         h = _vp()
         _check(lib().tiltgpu_create(C.byref(self.dims), C.byref(self._copts), C.byref(h)))
@@ -283,6 +288,7 @@ class Session:
     def set_angles(self, a):
         a = _f32(a)
         _check(lib().tiltgpu_set_angles(self.h, _f(a), a.size))
+        self._n_angles = a.size
 
     def set_xtilt(self, a):
         a = _f32(a)
@@ -321,6 +327,18 @@ class Session:
         out = np.zeros(max(1, self.opts.sirt), np.float64)
         k = _check(lib().tiltgpu_sirt_residuals(self.h, out.ctypes.data_as(_dp), out.size))
         return out[:k]
+
+    def geometry_tables(self):
+        """(views [nviews][5] cos_t, sin_t, cos_xt, sin_xt, zfactor; stretch [nz];
+        zc [nviews][out_nz]) the session reconstructs with."""
+        nv = min(self.dims.nz, self._n_angles)
+        vp = (ViewParams * max(1, nv))()
+        st = np.zeros(self.dims.nz, np.float32)
+        zc = np.zeros((max(1, nv), self.out_nz), np.float32)
+        _check(lib().tiltgpu_geometry_tables(self.h, vp, _f(st), _f(zc)))
+        views = np.array([[v.cos_t, v.sin_t, v.cos_xt, v.sin_xt, v.zfactor] for v in vp[:nv]],
+                         np.float32).reshape(nv, 5)
+        return views, st, zc[:nv]
 
     def set_tuning(self, bp_tile_z=0, fp_bins_per_lane=0):
         """Force launch shapes (0 = tuned table / analytic rule)."""

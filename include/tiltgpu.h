@@ -47,6 +47,12 @@ extern "C" {
 #define TILTGPU_STORE_F16 1
 #define TILTGPU_STORE_BF16 2
 
+/* Per-view geometry tables (cos/sin of the tilt and x-axis tilt, the z terms
+ * (dz*sin t)*zfactor, the stretch factors 1/cos t): FP32 as the reference
+ * computes them, or FP64 on the GPU rounded once to FP32 (precision form). */
+#define TILTGPU_GEOMETRY_F32 0
+#define TILTGPU_GEOMETRY_F64 1
+
 /* Sizes of one reconstruction.  in_* come from the MRC header of the aligned
  * stack (main.rs:560-571): nz = number of sections; out_ny = in_ny. */
 typedef struct {
@@ -71,7 +77,8 @@ typedef struct {
     int32_t sirt;                        /* --sirt N                           */
     float sirt_relax;                    /* --sirt-relax                       */
     int32_t fake_sirt;                   /* --fake-sirt N                      */
-    int32_t storage;                     /* TILTGPU_STORE_* (build addition)   */
+    int32_t storage;                     /* TILTGPU_STORE_* (no reference flag) */
+    int32_t geometry;                    /* TILTGPU_GEOMETRY_* (no ref. flag)   */
 } tiltgpu_options;
 
 /* Per-view geometry, the ProjParams of main.rs:130-140,682-700, in FP32 as the
@@ -107,6 +114,16 @@ void tiltgpu_default_options(tiltgpu_options* opts);
 int tiltgpu_proj_params(const float* angles_deg, int32_t n, const float* xtilt_deg,
                         int32_t n_xtilt, const float* zfactor, int32_t n_zfactor,
                         tiltgpu_view_params* out);
+
+/* The per-view tables a session reconstructs with (after tilt angles are set):
+ * views [nviews] (nviews = min(nz, angle count)), stretch [nz] (1/cos t, 0 for
+ * skipped views) and zc [nviews][out_nz] (the z terms (dz*sin t)*zfactor,
+ * dz = oz - out_nz/2).  Any pointer may be NULL.  FP32 tables follow
+ * main.rs:449-459,688-697,1015-1016; TILTGPU_GEOMETRY_F64 tables are computed
+ * in FP64 on the GPU and rounded once.  No counterpart in the reference:
+ * exposes the tables for checking. */
+int tiltgpu_geometry_tables(tiltgpu_session* s, tiltgpu_view_params* views, float* stretch,
+                            float* zc);
 
 /* Radial filter weights W[0..n) (main.rs:348-369). */
 int tiltgpu_radial_weights(int32_t n, int32_t exact_filter_size, float cutoff,
@@ -155,7 +172,7 @@ int tiltgpu_read_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out,
                       float* stats);
 
 /* SIRT residual norms ||O - FP(V)||_2 per iteration, reduced in FP64 on the
- * device (a build addition: the reference computes none, main.rs:928-953). */
+ * device (the reference computes none, main.rs:928-953). */
 int tiltgpu_sirt_residuals(tiltgpu_session* s, double* out, int32_t n);
 
 /* Launch shapes.  No counterpart in the reference (its CPU loops have none);
@@ -222,7 +239,7 @@ int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* 
  * every round batches the in-flight requests' filter, forward-projection and
  * backprojection work into shared launches (persistent per-XCD tile queues for
  * the backprojection).  A request's result is bit-identical alone and in any
- * mix.  No counterpart in the reference: a build addition for serving. */
+ * mix.  No counterpart in the reference: added for serving. */
 typedef struct tiltgpu_server tiltgpu_server;
 typedef struct {
     int64_t completed, failed, rounds, bp_launches, bp_batched_requests;

@@ -31,7 +31,7 @@ def case(tg, nx=128, ny=20, nviews=31, thick=48, seed=0, span=60.0, xtilt=None,
 def both(tg, P, angles, gopts, xtilt=None, zfactor=None, local=None, prec="f32"):
     print(f"case: shape={P.shape} opts={gopts}", flush=True)
     vol_g, st_g = tg.reconstruct(P, angles, gopts, xtilt=xtilt, zfactor=zfactor, local=local)
-    oopts = O.TiltOptions(**{k: v for k, v in vars(gopts).items() if k != "storage"})
+    oopts = O.TiltOptions(**{k: v for k, v in vars(gopts).items() if k not in ("storage", "geometry")})
     if oopts.exact_filter_size is None:
         oopts.exact_filter_size = 0
     vol_o, st_o = O.tilt_run(P, angles, oopts, xtilt=xtilt, zfactor=zfactor, local=local,
@@ -331,6 +331,99 @@ def test_half_storage(gpu, rel, storage, tol_max, tol_rms):
     assert rel(g, g32) <= tol_max
     assert rms_err(g, g32) <= tol_rms
     assert rms_err(g, o64) <= rms_err(g32, o64) + tol_rms
+
+
+def _ulps(a, b):
+    """Distance in FP32 units in the last place (values of one sign or zero)."""
+    a = np.asarray(a, np.float32).ravel()
+    b = np.asarray(b, np.float32).ravel()
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+def _geometry_case(gpu, geometry):
+    rng = np.random.default_rng(21)
+    nv, out_nz = 41, 37
+    ang = np.sort(rng.uniform(-70.0, 70.0, nv)).astype(np.float32)
+    ang[3] = -84.0  # beyond --max-cosine-tilt 80: stretch factor 0
+    xt = rng.uniform(-3.0, 3.0, nv).astype(np.float32)
+    zf = rng.uniform(0.9, 1.1, nv).astype(np.float32)
+    s = gpu.Session(64, 4, nv, gpu.TiltOptions(thickness=out_nz, geometry=geometry))
+    try:
+        s.set_angles(ang)
+        s.set_xtilt(xt)
+        s.set_zfactor(zf)
+        views, st, zc = s.geometry_tables()
+    finally:
+        s.close()
+    return ang, xt, zf, out_nz, views, st, zc
+
+
+def test_geometry_tables_f32_are_the_references():
+    """The default (FP32) tables: cos/sin equal the oracle's FP32 ProjParams
+    (main.rs:688-697) bit for bit, and the z terms are (dz * sin t) * zfactor in
+    FP32 (main.rs:1015-1016), checked by recomputing those two IEEE products."""
+    import tiltgpu as gpu
+    ang, xt, zf, out_nz, views, st, zc = _geometry_case(gpu, "f32")
+    tab = np.ascontiguousarray(O.proj_params(ang, xtilt=xt, zfactor=zf, prec="f32").T)
+    assert np.array_equal(views.view(np.uint32), tab.view(np.uint32))
+    dz = (np.arange(out_nz, dtype=np.float32) - np.float32(out_nz / 2.0))
+    exp = (dz[None, :] * views[:, 1:2]) * views[:, 4:5]
+    assert np.array_equal(zc.view(np.uint32), exp.astype(np.float32).view(np.uint32))
+    assert st[3] == 0.0 and np.all(st[np.abs(ang) <= 80.0] > 1.0 - 1e-7)
+
+
+def test_geometry_tables_f64_closed_form():
+    """TILTGPU_GEOMETRY_F64: cos/sin of the tilt and x-axis tilt, the stretch
+    factors 1/cos t and the z terms dz * sin t * zfactor computed in FP64 on the
+    GPU and rounded once to FP32, against the same closed forms evaluated by
+    numpy in float64 and rounded once.  Within 1 ulp: the device's double cos/sin
+    and the host libm's may differ in the last double bit, which moves the FP32
+    rounding only at a near-tie.  A table computed in FP32 (the default) sits
+    several ulps away for some views, which this bound catches."""
+    import tiltgpu as gpu
+    ang, xt, zf, out_nz, views, st, zc = _geometry_case(gpu, "f64")
+    d = np.deg2rad(ang.astype(np.float64))
+    dx = np.deg2rad(xt.astype(np.float64))
+    exp_views = np.stack([np.cos(d), np.sin(d), np.cos(dx), np.sin(dx), zf.astype(np.float64)], 1)
+    assert _ulps(views, exp_views).max() <= 1
+    exp_st = np.where(np.abs(ang) <= 80.0, 1.0 / np.cos(d), 0.0)
+    assert st[3] == 0.0 and _ulps(st, exp_st).max() <= 1
+    dz = np.arange(out_nz, dtype=np.float64) - out_nz / 2.0
+    exp_zc = dz[None, :] * np.sin(d)[:, None] * zf.astype(np.float64)[:, None]
+    assert _ulps(zc, exp_zc).max() <= 1
+    _, _, _, _, v32, _, zc32 = _geometry_case(gpu, "f32")
+    print(f"FP32 tables vs FP64 closed form: views {_ulps(v32, exp_views).max()} ulp, "
+          f"z terms {_ulps(zc32, exp_zc).max()} ulp")
+    assert _ulps(v32, exp_views).max() > 1
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_geometry_f64_reconstruction(gpu, rel, storage):
+    """Reconstruction with FP64 geometry tables (x-axis tilt and z factors
+    included).  Against the FP32-geometry path most voxels move only by the
+    tables' rounding; the rest are voxels whose fill decision (a coordinate at
+    the edge of the image) flips with a last-bit change of the tables, as it
+    does between FP32 and FP64.  The error against the FP64 oracle is stated and
+    is no larger than the FP32-geometry path's in RMS."""
+    nv = 31
+    xt = np.full(nv, 1.5, np.float32)
+    P, ang = case(gpu, seed=23, ny=8, nviews=nv, xtilt=xt)
+    zf = np.linspace(0.95, 1.05, nv).astype(np.float32)
+    print(f"case: geometry f64, storage {storage}", flush=True)
+    g32, _ = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=48, storage=storage),
+                             xtilt=xt, zfactor=zf)
+    g64, _ = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=48, storage=storage, geometry="f64"),
+                             xtilt=xt, zfactor=zf)
+    o64, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=48), xtilt=xt, zfactor=zf, prec="f64")
+    rms64 = np.sqrt(np.mean(o64.astype(np.float64) ** 2))
+    rms_err = lambda a, b: float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)) / rms64)
+    print(f"{storage} geometry f64: vs FP64 oracle max {rel(g64, o64):.3e} rms {rms_err(g64, o64):.3e}; "
+          f"geometry f32: max {rel(g32, o64):.3e} rms {rms_err(g32, o64):.3e}; "
+          f"f64 vs f32 geometry rms {rms_err(g64, g32):.3e}")
+    d = np.abs(g64.astype(np.float64) - g32) / rms64
+    print(f"voxels moved by more than 1e-4 x RMS: {np.mean(d > 1e-4):.2%}")
+    assert np.median(d) <= 1e-5
+    assert rms_err(g64, o64) <= rms_err(g32, o64) * 1.05 + 1e-6
 
 
 def test_c0_config(gpu, rel):
