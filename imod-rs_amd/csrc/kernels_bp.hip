@@ -137,15 +137,19 @@ __device__ __forceinline__ void grid_shift(const BpParams& P, int view, float x,
 
 constexpr int kSlots = 2;  // staged entries prefetched per thread
 
-// Per-view state of one lane and of its wave (the wave-level values are equal in
-// all four waves: they hold the same 64 ox positions).
+// Per-view state of one lane and of its wave.  a, ldx, fy and row are per lane;
+// the staging geometry (seg_lo .. total, inv_L) is wave-uniform and equal in all
+// four waves (they hold the same 64 ox positions), so it lives in SGPRs.
 struct Vs {
     float a;      // (dx*cos_t)*cos_xt
     float ldx;    // x shift (0 when none)
     float fy;     // weight of row r+1 (0: one row)
     int row;      // row pair start r, or -1 (no contribution)
     int seg_lo, L, r_lo, NR, total, shift;
+    float inv_L;  // 1/L (row of a linear entry index, two-row geometry)
 };
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <bool TWO, bool LOCAL>
 __device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int ox, int iy,
@@ -168,23 +172,27 @@ __device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int
     s.ldx = ldx;
     const TILTGPU_CONSTANT float* zc = cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad;
     const float z_first = zc[zt0], z_last = zc[zt0 + TZ - 1];
+    // with local shifts the shift is always added: ldx = +0 without one, and
+    // x + 0 == x for every x but -0 (same floor and fraction)
     float pa = (s.a + z_first) + P.cx;
     float pb = (s.a + z_last) + P.cx;
-    if (mode) {
+    if (LOCAL) {
         pa = pa + ldx;
         pb = pb + ldx;
     }
-    float lo = fminf(pa, pb), hi = fmaxf(pa, pb);
+    // floor is monotonic, so the floors of the extremes bound the segment
+    const int flo = cvt_floor(fminf(pa, pb)), fhi = cvt_floor(fmaxf(pa, pb));
+    int seg_lo, seg_hi;
     if (mode == 2) {
-        lo = wave_min(lo);
-        hi = wave_max(hi);
+        seg_lo = uni(wave_min_i(flo));
+        seg_hi = uni(wave_max_i(fhi));
     } else {
         // proj_x is monotonic in ox (a constant shift keeps it so): extremes at lanes 0, 63
-        lo = fminf(lane_f(lo, 0), lane_f(lo, 63));
-        hi = fmaxf(lane_f(hi, 0), lane_f(hi, 63));
+        seg_lo = min(__builtin_amdgcn_readlane(flo, 0), __builtin_amdgcn_readlane(flo, 63));
+        seg_hi = max(__builtin_amdgcn_readlane(fhi, 0), __builtin_amdgcn_readlane(fhi, 63));
     }
-    s.seg_lo = cvt_floor(lo);
-    s.L = cvt_floor(hi) - s.seg_lo + 1;
+    s.seg_lo = seg_lo;
+    s.L = seg_hi - seg_lo + 1;
     s.row = iy;
     s.fy = 0.f;
     s.r_lo = iy;
@@ -192,28 +200,28 @@ __device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int
     if constexpr (TWO) {
         const float yo = dx * sin_xt;
         float sy = static_cast<float>(iy) + yo;
-        if (mode) sy = sy + ldy;
+        if (LOCAL) sy = sy + ldy;
         const float fl = floorf(sy);
+        const int ifl = static_cast<int>(fl);
         const float fny = static_cast<float>(P.in_ny);
         const bool two = LOCAL || fabsf(yo) > 1e-6f;
         s.row = -1;
         if (two && fl >= 0.f && fl + 1.f < fny) {
-            s.row = static_cast<int>(fl);
+            s.row = ifl;
             s.fy = sy - fl;
         } else if (fl >= 0.f && fl < fny) {
-            s.row = static_cast<int>(fl);
+            s.row = ifl;
         }
         int rmin, rmax;
         if (mode == 2) {
-            rmin = wave_min_i(s.row >= 0 ? s.row : INT_MAX);
-            rmax = wave_max_i(s.row);
+            rmin = uni(wave_min_i(s.row >= 0 ? s.row : INT_MAX));
+            rmax = uni(wave_max_i(s.row));
         } else {
             // floor(sy) is monotonic in ox; clamp the extremes into the valid rows
-            const float f0 = lane_f(fl, 0), f1 = lane_f(fl, 63);
-            const float lo_r = fmaxf(fminf(f0, f1), 0.f);
-            const float hi_r = fminf(fmaxf(f0, f1), fny - 1.f);
-            rmin = lo_r <= hi_r ? static_cast<int>(lo_r) : INT_MAX;
-            rmax = lo_r <= hi_r ? static_cast<int>(hi_r) : -1;
+            const int f0 = __builtin_amdgcn_readlane(ifl, 0), f1 = __builtin_amdgcn_readlane(ifl, 63);
+            const int lo_r = max(min(f0, f1), 0), hi_r = min(max(f0, f1), P.in_ny - 1);
+            rmin = lo_r <= hi_r ? lo_r : INT_MAX;
+            rmax = lo_r <= hi_r ? hi_r : -1;
         }
         if (rmax < 0) {
             NR = 0;
@@ -225,59 +233,106 @@ __device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int
     }
     s.NR = NR;
     s.total = NR * s.L;
+    s.inv_L = __builtin_amdgcn_rcpf(static_cast<float>(max(s.L, 1)));
     return s;
 }
 
-// Global loads for staged entry e of view p: v[x], v[x+1] of row r (and r+1);
-// a padded entry loads the edge value instead (used by the Edge fill).
+// Staged entry e (0 <= e < total) of a view is row ri = e / L of the staged rows
+// and x = seg_lo + e % L; it lands at LDS index e.  The row comes from a float
+// product: with e < 2^16 the error of (e + 0.5) * rcp(L) is below 1e-3 / L,
+// far inside the 0.5 / L margin to the nearest integer.
+template <bool TWO>
+__device__ __forceinline__ void entry_pos(const Vs& s, int e, int& ri, int& x) {
+    if constexpr (TWO) {
+        ri = static_cast<int>(fmaf(static_cast<float>(e), s.inv_L, 0.5f * s.inv_L));
+        x = s.seg_lo + (e - __mul24(ri, s.L));
+    } else {
+        ri = 0;
+        x = s.seg_lo + e;
+    }
+}
+
+// Raw samples of entry e: v[i0], v[i1] of row r (and of row r+1) with
+// i0 = clamp(x, 0, n-1) and i1 = clamp(x+1, 0, n-1).  Outside [0, n-2] i0 == i1
+// is the edge sample, so the difference of the pair is 0 there.  Row bases are
+// wave-uniform; per lane only a 32-bit element offset is formed.  A row r+1
+// past the last row (weight 0) is read as row r.
+// min(max(x, 0), hi) in one v_med3_i32 (hi wave-uniform).
+__device__ __forceinline__ int clamp_i(int x, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(x), "s"(hi));
+    return r;
+}
+
+// Buffer resource over the samples of view p (range-checked to the view: an
+// access outside it would read 0, never another allocation).  Word 3 is the
+// gfx9-family raw-buffer format (32-bit data, no swizzle).
+template <int ST>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const BpParams& P, int p) {
+    constexpr long esz = ST == 0 ? 4 : 2;
+    const long nxy = static_cast<long>(P.in_nx) * P.in_ny;
+    const char* base = static_cast<const char*>(P.proj) + static_cast<long>(p) * nxy * esz;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), 0,
+                                             static_cast<int>(nxy * esz), 0x00020000);
+}
+
+template <int ST>
+__device__ __forceinline__ unsigned load_at(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+    if constexpr (ST == 0)
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    else
+        return static_cast<unsigned>(__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+
+// Raw samples of entry e: v[i0], v[i1] of row r (and of row r+1) with
+// i0 = clamp(x, 0, n-1) and i1 = clamp(x+1, 0, n-1).  Outside [0, n-2] i0 == i1
+// is the edge sample, so the difference of the pair is 0 there.  The row
+// offset is wave-uniform (scalar); per lane only a 32-bit byte offset is
+// formed.  A row r+1 past the last row (weight 0) is read as row r.
 template <bool TWO, int ST>
-__device__ __forceinline__ void load_entry(const BpParams& P, const Vs& s, int p, int iy,
-                                           int ri, int xi, unsigned (&r)[4]) {
+__device__ __forceinline__ void load_entry(const BpParams& P, const Vs& s, int p, int e,
+                                           unsigned (&r)[4], int& x) {
+    constexpr unsigned esz = ST == 0 ? 4u : 2u;
     const int n = P.in_nx;
-    const long nxy = static_cast<long>(n) * P.in_ny;
-    const int x = s.seg_lo + xi;
-    const int row = TWO ? s.r_lo + ri : iy;
-    const long off = static_cast<long>(p) * nxy + static_cast<long>(row) * n;
-    const int i0 = x < 0 ? 0 : (x > n - 2 ? n - 1 : x);
-    const int i1 = (x >= 0 && x <= n - 2) ? x + 1 : i0;
-    r[0] = load_raw<ST>(P.proj, off + i0);
-    r[1] = load_raw<ST>(P.proj, off + i1);
+    int ri;
+    entry_pos<TWO>(s, e, ri, x);
+    const int i0 = clamp_i(x, n - 1);
+    const int i1 = clamp_i(x + 1, n - 1);
+    const __amdgpu_buffer_rsrc_t rs = view_rsrc<ST>(P, p);
+    const int srow = s.r_lo * n * static_cast<int>(esz);
+    const int off = __mul24(ri, n);
+    r[0] = load_at<ST>(rs, static_cast<unsigned>(off + i0) * esz, srow);
+    r[1] = load_at<ST>(rs, static_cast<unsigned>(off + i1) * esz, srow);
     r[2] = r[3] = 0u;
     if constexpr (TWO) {
-        if (row + 1 < P.in_ny) {  // a row past the end is only used with weight 0
-            r[2] = load_raw<ST>(P.proj, off + n + i0);
-            r[3] = load_raw<ST>(P.proj, off + n + i1);
+        if (s.r_lo + s.NR < P.in_ny) {  // every staged row has a row below it
+            const int srow2 = srow + n * static_cast<int>(esz);
+            r[2] = load_at<ST>(rs, static_cast<unsigned>(off + i0) * esz, srow2);
+            r[3] = load_at<ST>(rs, static_cast<unsigned>(off + i1) * esz, srow2);
+        } else {
+            const int off2 = __mul24(min(ri + 1, P.in_ny - 1 - s.r_lo), n);
+            r[2] = load_at<ST>(rs, static_cast<unsigned>(off2 + i0) * esz, srow);
+            r[3] = load_at<ST>(rs, static_cast<unsigned>(off2 + i1) * esz, srow);
         }
     }
 }
 
+// Entry (v0, d0[, dv, dd]) from the raw samples: inside [0, n-2] the lerp
+// coefficients of sample_row_with_fill, outside the fill value with slope 0
+// (Edge fill: the edge sample itself, which the clamped loads delivered).
 template <bool TWO, int ST>
-__device__ __forceinline__ float4 make_entry(const BpParams& P, const Vs& s, int ri, int xi,
-                                             const unsigned (&r)[4], float scale, float mean) {
-    const int x = s.seg_lo + xi;
-    const bool inside = x >= 0 && x <= P.in_nx - 2;
+__device__ __forceinline__ float4 make_entry(const BpParams& P, int x, const unsigned (&r)[4],
+                                             float scale, float cfill) {
+    const bool keep = P.fill_mode == 2 || static_cast<unsigned>(x) <= static_cast<unsigned>(P.in_nx - 2);
     const float a0 = decode<ST>(r[0], scale), a1 = decode<ST>(r[1], scale);
-    float v0, d0;
-    if (inside) {
-        v0 = a0;
-        d0 = a1 - a0;
-    } else {
-        v0 = P.fill_mode == 0 ? mean : (P.fill_mode == 1 ? 0.f : a0);
-        d0 = 0.f;
-    }
+    const float v0 = keep ? a0 : cfill;
+    const float d0 = a1 - a0;
     if constexpr (!TWO) {
         return make_float4(v0, d0, 0.f, 0.f);
     } else {
-        float v1 = 0.f, d1 = 0.f;
-        if (s.r_lo + ri + 1 < P.in_ny) {
-            const float b0 = decode<ST>(r[2], scale), b1 = decode<ST>(r[3], scale);
-            if (inside) {
-                v1 = b0;
-                d1 = b1 - b0;
-            } else {
-                v1 = P.fill_mode == 0 ? mean : (P.fill_mode == 1 ? 0.f : b0);
-            }
-        }
+        const float b0 = decode<ST>(r[2], scale), b1 = decode<ST>(r[3], scale);
+        const float v1 = keep ? b0 : cfill;
+        const float d1 = b1 - b0;
         return make_float4(v0, d0, v1 - v0, d1 - d0);
     }
 }
@@ -290,56 +345,34 @@ __device__ __forceinline__ void put_entry(void* lds, int idx, float4 q) {
         static_cast<float2*>(lds)[idx] = make_float2(q.x, q.y);
 }
 
-// Staging work split: slot k of a thread is entry x = seg_lo + tid of row pair k
-// (two-row geometry) or entry x = seg_lo + tid + 256*k (one-row geometry).
-// Entries beyond the slots (segments wider than 256, more than kSlots row
-// pairs) are loaded and stored directly in stage_store.
-template <bool TWO>
-__device__ __forceinline__ bool slot_entry(const Vs& s, int k, int& ri, int& xi) {
-    if constexpr (TWO) {
-        ri = k;
-        xi = threadIdx.x;
-        return ri < s.NR && xi < s.L;
-    } else {
-        ri = 0;
-        xi = threadIdx.x + k * kBpThreads;
-        return xi < s.L;
-    }
-}
-
+// Slot k of a thread is entry e = tid + 256*k; entries past the slots (more
+// than 512 per view) are loaded and stored directly in stage_store.
 template <bool TWO, int ST>
-__device__ __forceinline__ void stage_load(const BpParams& P, const Vs& s, int p, int iy,
-                                           unsigned (&raw)[kSlots][4]) {
+__device__ __forceinline__ void stage_load(const BpParams& P, const Vs& s, int p,
+                                           unsigned (&raw)[kSlots][4], int (&xs)[kSlots]) {
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) {
-        int ri, xi;
-        if (slot_entry<TWO>(s, k, ri, xi)) load_entry<TWO, ST>(P, s, p, iy, ri, xi, raw[k]);
+        const int e = threadIdx.x + k * kBpThreads;
+        if (e < s.total) load_entry<TWO, ST>(P, s, p, e, raw[k], xs[k]);
     }
 }
 
 template <bool TWO, int ST>
-__device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int p, int iy,
-                                            const unsigned (&raw)[kSlots][4], void* lds,
-                                            int buf) {
+__device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int p,
+                                            const unsigned (&raw)[kSlots][4],
+                                            const int (&xs)[kSlots], void* lds, int buf) {
     const float scale = (ST == 0 || P.view_scale == nullptr) ? 1.f : cptr(P.view_scale)[p];
-    const float mean = cptr(P.means)[p];
+    const float cfill = P.fill_mode == 0 ? cptr(P.means)[p] : 0.f;
 #pragma unroll
     for (int k = 0; k < kSlots; ++k) {
-        int ri, xi;
-        if (slot_entry<TWO>(s, k, ri, xi))
-            put_entry<TWO, ST>(lds, buf + ri * s.L + xi,
-                               make_entry<TWO, ST>(P, s, ri, xi, raw[k], scale, mean));
+        const int e = threadIdx.x + k * kBpThreads;
+        if (e < s.total) put_entry<TWO, ST>(lds, buf + e, make_entry<TWO, ST>(P, xs[k], raw[k], scale, cfill));
     }
-    // entries beyond the prefetch slots: load and store directly
-    for (int ri = 0; ri < s.NR; ++ri) {
-        int xi0 = threadIdx.x;
-        if (TWO ? ri < kSlots : true) xi0 += TWO ? kBpThreads : kSlots * kBpThreads;
-        for (int xi = xi0; xi < s.L; xi += kBpThreads) {
-            unsigned one[4];
-            load_entry<TWO, ST>(P, s, p, iy, ri, xi, one);
-            put_entry<TWO, ST>(lds, buf + ri * s.L + xi,
-                               make_entry<TWO, ST>(P, s, ri, xi, one, scale, mean));
-        }
+    for (int e = threadIdx.x + kSlots * kBpThreads; e < s.total; e += kBpThreads) {
+        unsigned one[4];
+        int x;
+        load_entry<TWO, ST>(P, s, p, e, one, x);
+        put_entry<TWO, ST>(lds, buf + e, make_entry<TWO, ST>(P, x, one, scale, cfill));
     }
 }
 
@@ -418,8 +451,9 @@ __global__ void __launch_bounds__(kBpThreads)
 
         if (P.nviews > 0) {
             unsigned raw[kSlots][4];
+            int xs[kSlots];
             Vs cur = view_setup<TWO, LOCAL>(P, 0, dx, ox, iy, zt0, TZ);
-            stage_load<TWO, ST>(P, cur, 0, iy, raw);
+            stage_load<TWO, ST>(P, cur, 0, raw, xs);
             for (int p = 0; p < P.nviews; ++p) {
                 if (cur.total > P.lds_cap) {
                     // host bound violated: report it (the host fails the call) and skip
@@ -433,12 +467,12 @@ __global__ void __launch_bounds__(kBpThreads)
                 // buffer (p & 1) was last read by the compute of view p-2; every thread
                 // has passed the barrier of view p-1 since, so it is free.
                 const int buf = (p & 1) * P.lds_cap;
-                stage_store<TWO, ST>(P, cur, p, iy, raw, lds4, buf);
+                stage_store<TWO, ST>(P, cur, p, raw, xs, lds4, buf);
                 __syncthreads();
                 Vs nxt = cur;
                 if (p + 1 < P.nviews) {
                     nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
-                    stage_load<TWO, ST>(P, nxt, p + 1, iy, raw);
+                    stage_load<TWO, ST>(P, nxt, p + 1, raw, xs);
                 }
                 const TILTGPU_CONSTANT float* zc =
                     cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
@@ -447,25 +481,18 @@ __global__ void __launch_bounds__(kBpThreads)
                         const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
                         const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
                         // the fences keep the scheduler from hoisting every voxel's
-                        // 16-byte LDS read at once (registers, not issue, limit it)
-                        if (LOCAL && cur.shift) {
+                        // 16-byte LDS read at once (registers, not issue, limit it).
+                        // With local shifts every view takes the shifted chain: a view
+                        // without one has ldx = +0, and px + 0 == px for every px but
+                        // -0 (which becomes +0: same floor, same fraction).
     #pragma unroll
-                            for (int k = 0; k < ZPT; ++k) {
-                                const float px = ((a + zc[k]) + P.cx) + ldx;
-                                const float4 q = E[cvt_floor(px)];
-                                const float f = __builtin_amdgcn_fractf(px);
-                                acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
-                                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
-                            }
-                        } else {
-    #pragma unroll
-                            for (int k = 0; k < ZPT; ++k) {
-                                const float px = (a + zc[k]) + P.cx;
-                                const float4 q = E[cvt_floor(px)];
-                                const float f = __builtin_amdgcn_fractf(px);
-                                acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
-                                if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
-                            }
+                        for (int k = 0; k < ZPT; ++k) {
+                            float px = (a + zc[k]) + P.cx;
+                            if constexpr (LOCAL) px = px + ldx;
+                            const float4 q = E[cvt_floor(px)];
+                            const float f = __builtin_amdgcn_fractf(px);
+                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                            if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
                         }
                     }
                 } else if (cur.total > 0) {
