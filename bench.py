@@ -434,9 +434,8 @@ def main():
                     help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the WBP Y-slab sharding is described in DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the multi-GPU path (WBP Y-slab "
+              "sharding) is not built; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import tiltgpu as tg
