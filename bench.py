@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -120,6 +121,57 @@ def err_vs(x, ref):
     rms = float(np.sqrt(np.mean(ref ** 2)))
     return {"max_abs_over_rms": float(f"{np.max(np.abs(d)) / rms:.3g}"),
             "rms_over_rms": float(f"{np.sqrt(np.mean(d ** 2)) / rms:.3g}")}
+
+
+def end_to_end_section(tg, P, ang, xt, local):
+    """The `tilt` command line on the C1 workload from files: MRC read in chunks
+    overlapped with upload, WBP, and read-back overlapped with the MRC write
+    (page-locked buffers both ways).  Wall times as the command reports them;
+    the files live in a temporary directory that is removed afterwards."""
+    import shutil
+    import subprocess
+    import tempfile
+    tilt = os.path.join(REPO, "imod-rs_amd", "_build", "tilt")
+    if not os.path.exists(tilt):
+        return {"skipped": "tilt executable not built"}
+    d = tempfile.mkdtemp(prefix="tilt_e2e_")
+    try:
+        need = P.nbytes + P.shape[1] * 512 * P.shape[2] * 4 + (1 << 30)
+        free = shutil.disk_usage(d).free
+        if free < need:
+            return {"skipped": f"{free / 1e9:.1f} GB free in {d}, {need / 1e9:.1f} GB needed"}
+        inp, out = os.path.join(d, "c1.mrc"), os.path.join(d, "c1.rec")
+        tg.mrc_write(inp, P)
+        nv = P.shape[0]
+        with open(os.path.join(d, "c1.tlt"), "w") as fh:
+            fh.write("".join(f"{a:.9g}\n" for a in ang))
+        with open(os.path.join(d, "xtilt.txt"), "w") as fh:
+            fh.write("".join(f"{v:.9g}\n" for v in xt))
+        with open(os.path.join(d, "local.txt"), "w") as fh:
+            fh.write("".join(f"{v} {local[2][v]:.9g} {local[3][v]:.9g} {local[4][v]:.9g} "
+                             f"{local[5][v]:.9g}\n" for v in range(nv)))
+        cmd = [tilt, "-i", inp, "-o", out, "-t", os.path.join(d, "c1.tlt"), "-z", "512",
+               "--radial", "0.35,0.05", "--xtilt-file", os.path.join(d, "xtilt.txt"),
+               "--local-file", os.path.join(d, "local.txt")]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        line = [l for l in r.stderr.splitlines() if l.startswith("tilt: wall time")]
+        vals = [float(x) for x in re.findall(r"([0-9.]+) s", line[0])] if line else []
+        res = {"command": "tilt -i c1.mrc -o c1.rec -t c1.tlt -z 512 --radial 0.35,0.05 "
+                          "--xtilt-file xtilt.txt --local-file local.txt",
+               "process_wall_s": round(wall, 3),
+               "input_bytes": int(os.path.getsize(inp)), "output_bytes": int(os.path.getsize(out))}
+        if len(vals) == 4:
+            res.update({"read_upload_s": vals[0], "reconstruction_s": vals[1],
+                        "readback_write_s": vals[2], "total_s": vals[3],
+                        "gvoxel_view_per_s": round(float(P.shape[2]) * P.shape[1] * 512 * nv
+                                                   / vals[3] / 1e9, 1)})
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def cpu_baseline(P, ang, xt, local, rows=32):
@@ -429,6 +481,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--stream-requests", type=int, default=48)
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end run of the tilt command on files")
     ap.add_argument("--sirt-only", action="store_true", help="only the SIRT section")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
@@ -541,6 +595,12 @@ def main():
         print(f"# error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
+    if not a.no_e2e:
+        try:
+            line["end_to_end"] = end_to_end_section(tg, P, ang, xt, local)
+        except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
+            line["end_to_end"] = {"error": str(e)[:300]}
+        print(f"# end to end: {line['end_to_end']}", file=sys.stderr, flush=True)
     if not a.no_sirt:
         line["sirt"] = sirt_section(tg, oracle=not a.no_cpu)
     if not a.no_serve:

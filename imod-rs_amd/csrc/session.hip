@@ -776,8 +776,52 @@ int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t co
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded = true;
+    s->view_up.assign(s->d.nz, 1);
+    s->views_up = s->d.nz;
     s->filtered = false;
     return TILTGPU_OK;
+}
+
+int tiltgpu_upload_views(tiltgpu_session* s, int32_t v0, int32_t nv, const float* host) {
+    if (!s || !host || s->seam) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    if (v0 < 0 || nv < 1 || v0 + nv > s->d.nz)
+        return fail(TILTGPU_ERR_ARG, "views [%d, %d) outside the %d sections", v0, v0 + nv,
+                    s->d.nz);
+    const int64_t plane = static_cast<int64_t>(s->d.in_ny) * s->d.in_nx;
+    HIP_TRY(hipMemcpyAsync(s->raw + v0 * plane, host, nv * plane * sizeof(float),
+                           hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->view_up.size() != static_cast<size_t>(s->d.nz)) {
+        s->view_up.assign(s->d.nz, 0);
+        s->views_up = 0;
+    }
+    for (int v = v0; v < v0 + nv; ++v)
+        if (!s->view_up[v]) {
+            s->view_up[v] = 1;
+            ++s->views_up;
+        }
+    s->uploaded = s->views_up == s->d.nz;
+    s->filtered = false;
+    return TILTGPU_OK;
+}
+
+void* tiltgpu_host_alloc(int64_t bytes) {
+    if (bytes <= 0) {
+        fail(TILTGPU_ERR_ARG, "bad size");
+        return nullptr;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, static_cast<size_t>(bytes), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        fail(TILTGPU_ERR_HIP, "hipHostMalloc(%lld bytes) failed: %s", static_cast<long long>(bytes),
+             hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+
+void tiltgpu_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {

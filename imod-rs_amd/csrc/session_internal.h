@@ -114,6 +114,8 @@ struct tiltgpu_session {
     int radix[kMaxRadix];
     int nradix = 0;
     bool uploaded = false;
+    std::vector<uint8_t> view_up;  // views present after tiltgpu_upload_views
+    int views_up = 0;
     bool filtered = false;  // filt/store hold this upload's R-weighted, stretched views
     bool ran = false;
     int sirt_done = 0;

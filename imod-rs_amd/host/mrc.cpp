@@ -173,6 +173,37 @@ bool MrcReader::open(const std::string& path, std::string* err) {
     return true;
 }
 
+bool MrcReader::read_slice_f32_into(int z, float* dst, size_t cap, size_t* n, std::string* err) {
+    if (z < 0 || z >= h_.nz) {
+        *err = "slice index out of range";
+        return false;
+    }
+    if (h_.mode == 2 && !swapped_) {
+        const size_t npix = static_cast<size_t>(h_.nx) * h_.ny;
+        if (npix > cap) {
+            *err = "slice larger than the buffer";
+            return false;
+        }
+        const uint64_t off = h_.data_offset() + static_cast<uint64_t>(z) * h_.slice_bytes();
+        if (fseeko(f_, static_cast<off_t>(off), SEEK_SET) != 0 ||
+            std::fread(dst, sizeof(float), npix, f_) != npix) {
+            *err = "short read of slice " + std::to_string(z);
+            return false;
+        }
+        *n = npix;
+        return true;
+    }
+    std::vector<float> s;
+    if (!read_slice_f32(z, &s, err)) return false;
+    if (s.size() > cap) {
+        *err = "slice larger than the buffer";
+        return false;
+    }
+    std::memcpy(dst, s.data(), s.size() * sizeof(float));
+    *n = s.size();
+    return true;
+}
+
 bool MrcReader::read_slice_f32(int z, std::vector<float>* out, std::string* err) {
     if (z < 0 || z >= h_.nz) {
         *err = "slice index out of range";

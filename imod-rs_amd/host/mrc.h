@@ -60,6 +60,9 @@ class MrcReader {
     bool swapped() const { return swapped_; }
     // read_slice_f32 (read.rs:127-231): any supported mode -> f32
     bool read_slice_f32(int z, std::vector<float>* out, std::string* err);
+    // The same conversion into dst[0 .. cap): mode 2 in host byte order is read
+    // straight into dst.  Returns the value count through *n.
+    bool read_slice_f32_into(int z, float* dst, size_t cap, size_t* n, std::string* err);
     ~MrcReader();
 
   private:

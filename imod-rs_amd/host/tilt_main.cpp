@@ -1,11 +1,19 @@
 // `tilt` -- the drop-in front end: the option surface and file handling of
 // imod-rs apps/tilt/src/main.rs:18-120,540-1095, with the reconstruction run on
-// the GPU through libtiltgpu (include/tiltgpu.h).  Output rows are streamed to
-// the MRC file in slabs straight from HBM.
+// the GPU through libtiltgpu (include/tiltgpu.h).  The stack is read in chunks
+// of sections while the previous chunk uploads, and output rows come back in
+// slabs while the previous slab is written: two page-locked buffers each way,
+// a worker thread on one side of each pair.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tiltgpu.h"
@@ -15,6 +23,71 @@
 using namespace tiltgpu_host;
 
 namespace {
+
+// Two-buffer pipeline: produce(k, buf) runs on a worker thread, consume(k, buf)
+// on the caller's, chunk k in buffer k % 2.  Stops at the first failure (its
+// message in *err) and joins the worker before returning.
+bool pipeline2(int chunks, float* const bufs[2],
+               const std::function<bool(int, float*, std::string*)>& produce,
+               const std::function<bool(int, float*, std::string*)>& consume, std::string* err) {
+    std::mutex m;
+    std::condition_variable cv;
+    int full[2] = {0, 0};
+    bool failed = false;
+    std::string werr;
+    std::thread worker([&] {
+        for (int k = 0; k < chunks; ++k) {
+            const int b = k & 1;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return full[b] == 0 || failed; });
+                if (failed) return;
+            }
+            std::string e;
+            const bool ok = produce(k, bufs[b], &e);
+            std::lock_guard<std::mutex> lk(m);
+            if (!ok) {
+                failed = true;
+                werr = e;
+            } else {
+                full[b] = 1;
+            }
+            cv.notify_all();
+            if (!ok) return;
+        }
+    });
+    bool ok = true;
+    for (int k = 0; k < chunks && ok; ++k) {
+        const int b = k & 1;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return full[b] == 1 || failed; });
+            if (failed) {
+                ok = false;
+                break;
+            }
+        }
+        std::string e;
+        ok = consume(k, bufs[b], &e);
+        std::lock_guard<std::mutex> lk(m);
+        if (!ok) {
+            failed = true;
+            werr = e;
+        }
+        full[b] = 0;
+        cv.notify_all();
+    }
+    worker.join();
+    if (failed || !ok) {
+        *err = werr;
+        return false;
+    }
+    return true;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 struct Args {
     std::string input, output, tilt_file;
@@ -39,6 +112,7 @@ struct Args {
     std::string storage = "f32";
     std::string geometry = "f32";
     int slab_rows = 64;
+    int upload_views = 0;
 };
 
 [[noreturn]] void usage(const char* msg) {
@@ -72,7 +146,8 @@ struct Args {
                  "      --storage <s>          f32 | f16 | bf16 projection storage in HBM [f32]\n"
                  "      --geometry <g>         f32 (the reference's tables) | f64 (computed in\n"
                  "                             FP64 on the GPU, rounded once) [f32]\n"
-                 "      --slab-rows <n>        output rows per device->file transfer [64]\n");
+                 "      --slab-rows <n>        output rows per device->file transfer [64]\n"
+                 "      --upload-views <n>     sections per file->device transfer [about 128 MB]\n");
     std::exit(2);
 }
 
@@ -124,6 +199,7 @@ Args parse_args(int argc, char** argv) {
         {"storage", 0, 1, &a.storage},
         {"geometry", 0, 1, &a.geometry},
         {"slab-rows", 0, 2, &a.slab_rows},
+        {"upload-views", 0, 2, &a.upload_views},
     };
     for (int i = 1; i < argc; ++i) {
         std::string arg = argv[i];
@@ -256,19 +332,6 @@ int main(int argc, char** argv) {
     o.storage = st == "f16" ? TILTGPU_STORE_F16 : st == "bf16" ? TILTGPU_STORE_BF16 : TILTGPU_STORE_F32;
     o.geometry = lower(a.geometry) == "f64" ? TILTGPU_GEOMETRY_F64 : TILTGPU_GEOMETRY_F32;
 
-    // projections (main.rs:601-605)
-    std::vector<float> proj(static_cast<size_t>(nz) * in_nx * in_ny);
-    {
-        std::vector<float> s;
-        for (int z = 0; z < nz; ++z) {
-            if (!reader.read_slice_f32(z, &s, &err) || s.size() != static_cast<size_t>(in_nx) * in_ny) {
-                std::fprintf(stderr, "Error: %s\n", err.empty() ? "unsupported MRC mode" : err.c_str());
-                return 1;
-            }
-            std::memcpy(proj.data() + static_cast<size_t>(z) * in_nx * in_ny, s.data(), s.size() * 4);
-        }
-    }
-
     tiltgpu_dims d{in_nx, in_ny, nz, out_nx, out_nz};
     tiltgpu_session* S = nullptr;
     check(tiltgpu_create(&d, &o, &S), "session creation");
@@ -292,10 +355,56 @@ int main(int argc, char** argv) {
         const auto zf = read_values(a.zfactor_file, true, "zfactor file");
         check(tiltgpu_set_zfactor(S, zf.data(), static_cast<int32_t>(zf.size())), "zfactor");
     }
-    check(tiltgpu_upload_projections(S, proj.data(), static_cast<int64_t>(proj.size())), "upload");
-    proj.clear();
-    proj.shrink_to_fit();
+    // projections (main.rs:601-605): chunks of sections read into page-locked
+    // buffers on a worker thread while the previous chunk uploads
+    const double t_start = now_s();
+    const size_t plane = static_cast<size_t>(in_nx) * in_ny;
+    const int chunk_views = std::max(
+        1, std::min(nz, a.upload_views > 0 ? a.upload_views
+                                           : static_cast<int>((128u << 20) / (plane * 4) + 1)));
+    const int in_chunks = (nz + chunk_views - 1) / chunk_views;
+    float* inbuf[2] = {nullptr, nullptr};
+    for (auto& b : inbuf) {
+        b = static_cast<float*>(tiltgpu_host_alloc(static_cast<int64_t>(chunk_views * plane * 4)));
+        if (!b) {
+            std::fprintf(stderr, "Error: %s\n", tiltgpu_last_error());
+            return 1;
+        }
+    }
+    {
+        std::string perr;
+        const bool ok = pipeline2(
+            in_chunks, inbuf,
+            [&](int k, float* buf, std::string* e) {
+                const int v0 = k * chunk_views, v1 = std::min(nz, v0 + chunk_views);
+                for (int z = v0; z < v1; ++z) {
+                    size_t n = 0;
+                    if (!reader.read_slice_f32_into(z, buf + (z - v0) * plane, plane, &n, e) ||
+                        n != plane) {
+                        if (e->empty()) *e = "unsupported MRC mode";
+                        return false;
+                    }
+                }
+                return true;
+            },
+            [&](int k, float* buf, std::string* e) {
+                const int v0 = k * chunk_views, v1 = std::min(nz, v0 + chunk_views);
+                if (tiltgpu_upload_views(S, v0, v1 - v0, buf) != 0) {
+                    *e = std::string("upload: ") + tiltgpu_last_error();
+                    return false;
+                }
+                return true;
+            },
+            &perr);
+        for (auto& b : inbuf) tiltgpu_host_free(b);
+        if (!ok) {
+            std::fprintf(stderr, "Error: %s\n", perr.c_str());
+            return 1;
+        }
+    }
+    const double t_read = now_s();
     check(tiltgpu_run(S), "reconstruction");
+    const double t_run = now_s();
 
     // output header and sections (main.rs:710-719, 1077-1093)
     MrcHeader oh = MrcHeader::make(out_nx, out_nz, out_ny, 2);
@@ -311,36 +420,67 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "Error: %s\n", err.c_str());
         return 1;
     }
+    // output rows come back in slabs into page-locked buffers on a worker
+    // thread while the previous slab is written (main.rs:1077-1093)
     const size_t per = static_cast<size_t>(out_nx) * out_nz;
-    const int slab = a.slab_rows > 0 ? a.slab_rows : 64;
-    std::vector<float> buf(per * static_cast<size_t>(slab));
-    std::vector<float> stats(3 * static_cast<size_t>(slab));
+    const int slab = std::max(1, std::min(out_ny, a.slab_rows > 0 ? a.slab_rows : 64));
+    const int out_chunks = (out_ny + slab - 1) / slab;
+    float* outbuf[2] = {nullptr, nullptr};
+    for (auto& b : outbuf) {
+        b = static_cast<float*>(tiltgpu_host_alloc(static_cast<int64_t>((per + 3) * slab * 4)));
+        if (!b) {
+            std::fprintf(stderr, "Error: %s\n", tiltgpu_last_error());
+            return 1;
+        }
+    }
     float gmin = 3.40282347e38f, gmax = -3.40282347e38f;
     double gsum = 0.0;
-    for (int y0 = 0; y0 < out_ny; y0 += slab) {
-        const int y1 = std::min(out_ny, y0 + slab);
-        check(tiltgpu_read_rows(S, y0, y1, buf.data(), stats.data()), "read rows");
-        for (int r = 0; r < y1 - y0; ++r) {
-            const float smin = stats[3 * r], smax = stats[3 * r + 1], smean = stats[3 * r + 2];
-            if (smin < gmin) gmin = smin;
-            if (smax > gmax) gmax = smax;
-            gsum += static_cast<double>(smean) * static_cast<double>(per);
-            if (!writer.write_slice_f32(buf.data() + r * per, per, &err)) {
-                std::fprintf(stderr, "Error: %s\n", err.c_str());
-                return 1;
-            }
+    {
+        std::string perr;
+        const bool ok = pipeline2(
+            out_chunks, outbuf,
+            [&](int k, float* buf, std::string* e) {
+                const int y0 = k * slab, y1 = std::min(out_ny, y0 + slab);
+                // the slab's per-row statistics follow its rows in the buffer
+                if (tiltgpu_read_rows(S, y0, y1, buf, buf + per * slab) != 0) {
+                    *e = std::string("read rows: ") + tiltgpu_last_error();
+                    return false;
+                }
+                return true;
+            },
+            [&](int k, float* buf, std::string* e) {
+                const int y0 = k * slab, y1 = std::min(out_ny, y0 + slab);
+                const float* stats = buf + per * slab;
+                for (int r = 0; r < y1 - y0; ++r) {
+                    const float smin = stats[3 * r], smax = stats[3 * r + 1], smean = stats[3 * r + 2];
+                    if (smin < gmin) gmin = smin;
+                    if (smax > gmax) gmax = smax;
+                    gsum += static_cast<double>(smean) * static_cast<double>(per);
+                    if (!writer.write_slice_f32(buf + r * per, per, e)) return false;
+                }
+                std::fprintf(stderr, "  %d/%d rows\n", y1, out_ny);
+                return true;
+            },
+            &perr);
+        for (auto& b : outbuf) tiltgpu_host_free(b);
+        if (!ok) {
+            std::fprintf(stderr, "Error: %s\n", perr.c_str());
+            return 1;
         }
-        std::fprintf(stderr, "  %d/%d rows\n", y1, out_ny);
     }
     const float gmean = static_cast<float>(gsum / (static_cast<double>(per) * out_ny));
     if (!writer.finish(gmin, gmax, gmean, &err)) {
         std::fprintf(stderr, "Error: %s\n", err.c_str());
         return 1;
     }
+    const double t_end = now_s();
     tiltgpu_timing t;
     if (tiltgpu_get_timing(S, &t) == 0)
         std::fprintf(stderr, "tilt: device time filter %.2f ms, backprojection %.2f ms, "
                              "forward projection %.2f ms\n", t.filter_ms, t.bp_ms, t.fp_ms);
+    std::fprintf(stderr, "tilt: wall time read+upload %.3f s, reconstruction %.3f s, "
+                         "read-back+write %.3f s, total %.3f s\n",
+                 t_read - t_start, t_run - t_read, t_end - t_run, t_end - t_start);
     tiltgpu_destroy(S);
     std::fprintf(stderr, "tilt: reconstruction complete -> %s\n", a.output.c_str());
     return 0;

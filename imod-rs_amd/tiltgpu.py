@@ -111,6 +111,9 @@ SIGNATURES = [
     ("tiltgpu_set_zfactor", C.c_int, [_vp, _fp, C.c_int32]),
     ("tiltgpu_set_local_alignment", C.c_int, [_vp, _ip, _ip, _fp, _fp, _fp, _fp]),
     ("tiltgpu_upload_projections", C.c_int, [_vp, _fp, C.c_int64]),
+    ("tiltgpu_upload_views", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_host_alloc", C.c_void_p, [C.c_int64]),
+    ("tiltgpu_host_free", None, [C.c_void_p]),
     ("tiltgpu_run", C.c_int, [_vp]),
     ("tiltgpu_run_rows", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_read_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
@@ -253,6 +256,29 @@ def radial_weights(n, cutoff=0.35, falloff=0.05, exact_filter_size=0, hamming=Fa
     return out
 
 
+class PinnedArray:
+    """A float32 numpy array in page-locked host memory (tiltgpu_host_alloc)."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        self.ptr = lib().tiltgpu_host_alloc(max(4, 4 * n))
+        if not self.ptr:
+            raise TiltGpuError(-2, lib().tiltgpu_last_error().decode())
+        self.array = np.ctypeslib.as_array((C.c_float * n).from_address(self.ptr)).reshape(shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().tiltgpu_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Session:
     """One reconstruction resident on the GPU (tiltgpu_create ... tiltgpu_destroy)."""
 
@@ -308,6 +334,11 @@ class Session:
     def upload(self, P):
         P = _f32(P)
         _check(lib().tiltgpu_upload_projections(self.h, _f(P), P.size))
+
+    def upload_views(self, v0, P):
+        """Raw projections of sections [v0, v0 + len(P)) (a stack uploaded in chunks)."""
+        P = _f32(P)
+        _check(lib().tiltgpu_upload_views(self.h, v0, P.shape[0], _f(P)))
 
     def run(self):
         _check(lib().tiltgpu_run(self.h))

@@ -154,6 +154,19 @@ int tiltgpu_set_local_alignment(tiltgpu_session* s, const int32_t* nx, const int
 /* Raw projections [nz][in_ny][in_nx] (main.rs:601-605), before --log. */
 int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t count);
 
+/* Raw projections of sections [v0, v0+nv) ([nv][in_ny][in_nx]), for a stack
+ * read in chunks (main.rs:601-605 reads it whole).  The session is uploaded
+ * once every section has arrived; a later call replaces those sections.
+ * Returns after the copy; from tiltgpu_host_alloc memory it runs at the
+ * pinned-host rate.  No counterpart in the reference: added for streaming. */
+int tiltgpu_upload_views(tiltgpu_session* s, int32_t v0, int32_t nv, const float* host);
+
+/* Page-locked host memory for upload and read-back buffers (NULL on failure,
+ * tiltgpu_last_error says why); free with tiltgpu_host_free.  No counterpart
+ * in the reference. */
+void* tiltgpu_host_alloc(int64_t bytes);
+void tiltgpu_host_free(void* p);
+
 /* Run the reconstruction on the device: log, per-view means, R-weighting,
  * cosine stretch, then WBP, SIRT or fake-SIRT, and the output transform.  The
  * volume stays resident in HBM.  Replaces main.rs:607-1090 minus file I/O. */

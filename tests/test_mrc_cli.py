@@ -132,7 +132,7 @@ def test_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
     (tmp_path / "loc").write_text("".join(f"{v} 48 7 {sx[v]} {sy[v]}\n" for v in range(25)))
     cmd = [TILT, "-i", inp, "-o", out, "-t", str(tmp_path / "a.tlt"), "-z", "40",
            "--xtilt-file", str(tmp_path / "xt"), "--local-file", str(tmp_path / "loc"),
-           "--densweight", "2", "--outadd=1", "--slab-rows", "5"]
+           "--densweight", "2", "--outadd=1", "--slab-rows", "5", "--upload-views", "4"]
     print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
