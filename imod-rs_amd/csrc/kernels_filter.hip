@@ -15,9 +15,27 @@
 // (-ffp-contract=off) and writes the row coalesced.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "tiltgpu_internal.h"
 
 namespace tiltgpu {
+
+constexpr int kFilterThreads = 256;
+
+__shared__ double s_wave_sum[kFilterThreads / 64];
+
+// Sum over the 256 threads of a workgroup in a fixed order (waves by
+// butterfly, then the four wave sums); all threads call it.
+__device__ __forceinline__ double wg_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s_wave_sum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = (s_wave_sum[0] + s_wave_sum[1]) + (s_wave_sum[2] + s_wave_sum[3]);
+    __syncthreads();
+    return t;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -207,8 +225,274 @@ __device__ float stretch_sample(const float2* row, int c, int n, float s, int cu
     return 0.f;
 }
 
-// Workgroup `blk` of one filter launch: row pairs [blk*ppw, (blk+1)*ppw).
-__device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
+// ---------------------------------------------------------------- power of two
+//
+// n = 2^L, 64 <= n <= 2048: every thread holds 8 complex values of one row pair
+// (n/8 threads per pair, 2048/n pairs per workgroup).  Radix-8 Stockham passes
+// with the butterfly in registers, then one radix-4 or radix-2 pass; sizes are
+// compile-time, so indices are shifts and masks.  The weight multiply is fused
+// into the last forward pass and the 1/n scale into the last inverse pass.
+
+// DFT8 by decimation in frequency: b_m = x_m + x_{m+4}, c_m = (x_m - x_{m+4}) W8^m,
+// X[2m] = DFT4(b)[m], X[2m+1] = DFT4(c)[m].
+__device__ __forceinline__ void dft8(float2 (&x)[8]) {
+    const float h = 0.70710678118654752440f;  // 1/sqrt(2)
+    float2 b[4], c[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        b[m] = cadd(x[m], x[m + 4]);
+        c[m] = csub(x[m], x[m + 4]);
+    }
+    // W8 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = -(1 + i)/sqrt2
+    c[1] = make_float2((c[1].x + c[1].y) * h, (c[1].y - c[1].x) * h);
+    c[2] = cmul_mi(c[2]);
+    c[3] = make_float2((c[3].y - c[3].x) * h, -(c[3].x + c[3].y) * h);
+    dft_small<4>(b);
+    dft_small<4>(c);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        x[2 * m] = b[m];
+        x[2 * m + 1] = c[m];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_r(float2* x) {
+    if constexpr (R == 8)
+        dft8(*reinterpret_cast<float2(*)[8]>(x));
+    else
+        dft_small<R>(x);
+}
+
+// Fused operation on a pass's outputs before they are stored.
+enum { kP2None = 0, kP2Weight = 1, kP2Scale = 2 };
+
+// LDS slot of element i: one pad slot every 8 elements, which spreads the
+// strided stores of the first passes over the banks.
+__device__ __forceinline__ int p2_slot(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int p2_row_slots(int n) { return n + n / 8; }
+
+// Radix, span and per-thread butterflies of pass S, and the thread's twiddle
+// count before pass S (compile time).
+template <int N>
+struct P2 {
+    static constexpr int T = N / 8;
+    static constexpr int radix(int s) {
+        int ns = 1;
+        for (int i = 0; i < s; ++i) ns *= (N / ns >= 8 ? 8 : N / ns);
+        return N / ns >= 8 ? 8 : N / ns;
+    }
+    static constexpr int span(int s) {
+        int ns = 1;
+        for (int i = 0; i < s; ++i) ns *= radix(i);
+        return ns;
+    }
+    static constexpr int passes() {
+        int s = 0;
+        while (span(s) < N) ++s;
+        return s;
+    }
+    static constexpr int per(int s) { return (N / radix(s)) / T; }
+    static constexpr int tw_regs(int s) { return span(s) > 1 ? per(s) * (radix(s) - 1) : 0; }
+    static constexpr int tw_before(int s) {
+        int c = 0;
+        for (int i = 0; i < s; ++i) c += tw_regs(i);
+        return c;
+    }
+    static constexpr int tw_total() { return tw_before(passes()); }
+    static constexpr int tab_off(int s) {
+        int c = 0;
+        for (int i = 0; i < s; ++i) c += span(i) * radix(i);
+        return c;
+    }
+};
+
+// The thread's twiddles of every pass, loaded once (the same for the forward
+// and the inverse transform).
+template <int N, int S>
+__device__ __forceinline__ void p2_load_tw(float2* twr, int j, const float2* __restrict__ tw8) {
+    if constexpr (S < P2<N>::passes()) {
+        constexpr int R = P2<N>::radix(S), NS = P2<N>::span(S), PER = P2<N>::per(S);
+        if constexpr (NS > 1) {
+            float2* w = twr + P2<N>::tw_before(S);
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int k = (j + u * P2<N>::T) & (NS - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    w[u * (R - 1) + r - 1] = tw8[P2<N>::tab_off(S) + k * R + r];
+            }
+        }
+        p2_load_tw<N, S + 1>(twr, j, tw8);
+    }
+}
+
+// One in-place Stockham pass of radix R and span NS over a pair's padded row:
+// thread j does the butterflies b = j + u*T (T = N/8, u < N/(R*T)); inputs
+// b + r*N/R, outputs (b - k)*R + k + r*NS with k = b mod NS.  All reads finish
+// (barrier) before any write.
+template <int N, int S, int FUSE>
+__device__ __forceinline__ void p2_pass(float2* row, int j, const float2* twr,
+                                        const float* __restrict__ weights, float inv_n) {
+    constexpr int R = P2<N>::radix(S), NS = P2<N>::span(S), PER = P2<N>::per(S);
+    constexpr int T = P2<N>::T, BF = N / R;
+    float2 x[PER][R];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int b = j + u * T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[u][r] = row[p2_slot(b + r * BF)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int b = j + u * T;
+        const int k = b & (NS - 1);
+        if constexpr (NS > 1) {
+            const float2* w = twr + P2<N>::tw_before(S) + u * (R - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) x[u][r] = cmul(x[u][r], w[r - 1]);
+        }
+        dft_r<R>(x[u]);
+        const int base = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 v = x[u][r];
+            if constexpr (FUSE == kP2Weight) {
+                // Z' = conj(W * X), the input of the inverse transform
+                const float wt = weights[base + r * NS];
+                v = make_float2(v.x * wt, -(v.y * wt));
+            } else if constexpr (FUSE == kP2Scale) {
+                // a = Re(Y) / n, b = -Im(Y) / n  (main.rs:376-379: re * (1/n))
+                v = make_float2(v.x * inv_n, -v.y * inv_n);
+            }
+            row[p2_slot(base + r * NS)] = v;
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int S, int FUSE>
+__device__ __forceinline__ void p2_fft(float2* row, int j, const float2* twr,
+                                       const float* __restrict__ weights, float inv_n) {
+    if constexpr (S < P2<N>::passes()) {
+        constexpr bool LAST = S + 1 == P2<N>::passes();
+        p2_pass<N, S, LAST ? FUSE : kP2None>(row, j, twr, weights, inv_n);
+        p2_fft<N, S + 1, FUSE>(row, j, twr, weights, inv_n);
+    }
+}
+
+__device__ __forceinline__ float p2_at(const float2* row, int i, int c) {
+    const float2 v = row[p2_slot(i)];
+    return c ? v.y : v.x;
+}
+
+// stretch_sample over a padded LDS row (same arithmetic, main.rs:408-488).
+__device__ float p2_stretch_sample(const float2* row, int c, int n, float s, int cubic, int ox) {
+    const float center = static_cast<float>(n) / 2.0f;
+    const float dxs = static_cast<float>(ox) - center;
+    const float src_x = dxs * s + center;
+    const float fl = floorf(src_x);
+    const float fn = static_cast<float>(n);
+    if (cubic) {
+        const float frac = src_x - fl;
+        if (fl >= 1.f && fl + 2.f < fn) {
+            const int i = static_cast<int>(fl);
+            const float p0 = p2_at(row, i - 1, c), p1 = p2_at(row, i, c);
+            const float p2 = p2_at(row, i + 1, c), p3 = p2_at(row, i + 2, c);
+            const float t = frac, t2 = t * t, t3 = t2 * t;
+            const float a = -0.5f * p0 + 1.5f * p1 - 1.5f * p2 + 0.5f * p3;
+            const float b = p0 - 2.5f * p1 + 2.0f * p2 - 0.5f * p3;
+            const float cc = -0.5f * p0 + 0.5f * p2;
+            return a * t3 + b * t2 + cc * t + p1;
+        } else if (fl >= 0.f && fl + 1.f < fn) {
+            const int i = static_cast<int>(fl);
+            return p2_at(row, i, c) * (1.0f - frac) + p2_at(row, i + 1, c) * frac;
+        } else if (fl >= 0.f && fl < fn) {
+            return p2_at(row, static_cast<int>(fl), c);
+        }
+        return 0.f;
+    }
+    if (fl >= 0.f && fl + 1.f < fn) {
+        const int i = static_cast<int>(fl);
+        const float frac = src_x - fl;
+        return p2_at(row, i, c) * (1.0f - frac) + p2_at(row, i + 1, c) * frac;
+    } else if (fl >= 0.f && fl < fn) {
+        return p2_at(row, static_cast<int>(fl), c);
+    }
+    return 0.f;
+}
+
+template <int N>
+__device__ void filter_body_p2(const FilterParams& P, int blk) {
+    extern __shared__ float2 sm[];
+    constexpr int T = P2<N>::T;
+    constexpr int PPW = 2048 / N;
+    constexpr int RS = p2_row_slots(N);
+    constexpr int LOGN = (N == 64) ? 6 : (N == 128) ? 7 : (N == 256) ? 8 : (N == 512) ? 9
+                       : (N == 1024) ? 10 : 11;
+    const int tid = threadIdx.x;
+    const int q = tid / T;  // pair of this thread within the workgroup
+    const int j = tid & (T - 1);
+    const int pair0 = blk * PPW;
+    const int total_pairs = (P.nrows + 1) / 2;
+    const int pairs = min(PPW, total_pairs - pair0);
+    float2 twr[P2<N>::tw_total() > 0 ? P2<N>::tw_total() : 1];
+    if (P.weights) p2_load_tw<N, 0>(twr, j, P.tw8);
+
+    // ---- load two rows per pair (row a -> re, row b -> im), zero past the end
+    double sq = 0.0;
+    for (int idx = tid; idx < PPW * N; idx += kFilterThreads) {
+        const int qq = idx >> LOGN;
+        const int i = idx & (N - 1);
+        float va = 0.f, vb = 0.f;
+        if (qq < pairs) {
+            const int ra = 2 * (pair0 + qq), rb = ra + 1;
+            va = P.src[static_cast<long>(ra) * N + i];
+            if (P.src_sub) va = va - P.src_sub[static_cast<long>(ra) * N + i];
+            if (rb < P.nrows) {
+                vb = P.src[static_cast<long>(rb) * N + i];
+                if (P.src_sub) vb = vb - P.src_sub[static_cast<long>(rb) * N + i];
+            }
+        }
+        sq += static_cast<double>(va) * va + static_cast<double>(vb) * vb;
+        sm[qq * RS + p2_slot(i)] = make_float2(va, vb);
+    }
+    if (P.sq_partial) {
+        const double t = wg_sum(sq);
+        if (tid == 0) P.sq_partial[blk] = t;
+    }
+    __syncthreads();
+
+    if (P.weights) {
+        float2* row = sm + q * RS;
+        // forward transform with the weight multiply fused into its last pass,
+        // then the inverse transform of the conjugate with the 1/n scale fused
+        p2_fft<N, 0, kP2Weight>(row, j, twr, P.weights, 0.f);
+        p2_fft<N, 0, kP2Scale>(row, j, twr, nullptr, 1.0f / static_cast<float>(N));
+    }
+    // ---- cosine stretch (per row: its own view) and coalesced write
+    for (int idx = tid; idx < pairs * N; idx += kFilterThreads) {
+        const int qq = idx >> LOGN;
+        const int ox = idx & (N - 1);
+        const float2* row = sm + qq * RS;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int r = 2 * (pair0 + qq) + c;
+            if (r >= P.nrows) break;
+            const float s = P.stretch ? P.stretch[r / P.rows_per_view] : 0.f;
+            float v;
+            if (s != 0.f)
+                v = p2_stretch_sample(row, c, N, s, P.cubic, ox);
+            else
+                v = p2_at(row, ox, c);
+            P.dst[static_cast<long>(r) * N + ox] = v;
+        }
+    }
+}
+
+// Workgroup `blk` of one filter launch (mixed-radix passes).
+__device__ __forceinline__ void filter_body_mixed(const FilterParams& P, int blk) {
     extern __shared__ float2 sm[];
     const int n = P.n;
     const int ppw = P.pairs_per_wg;
@@ -236,14 +520,8 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
         b0[idx] = make_float2(va, vb);
     }
     if (P.sq_partial) {
-        __shared__ double red[256];
-        red[tid] = sq;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) P.sq_partial[blk] = red[0];
+        const double t = wg_sum(sq);
+        if (tid == 0) P.sq_partial[blk] = t;
     }
     __syncthreads();
 
@@ -293,6 +571,22 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
     }
 }
 
+// Workgroup `blk` of one filter launch: row pairs [blk*ppw, (blk+1)*ppw).
+__device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
+    if (P.tw8) {
+        switch (P.log2n) {
+            case 6: filter_body_p2<64>(P, blk); return;
+            case 7: filter_body_p2<128>(P, blk); return;
+            case 8: filter_body_p2<256>(P, blk); return;
+            case 9: filter_body_p2<512>(P, blk); return;
+            case 10: filter_body_p2<1024>(P, blk); return;
+            case 11: filter_body_p2<2048>(P, blk); return;
+            default: break;
+        }
+    }
+    filter_body_mixed(P, blk);
+}
+
 __global__ void __launch_bounds__(256) filter_kernel(FilterParams P) { filter_body(P, blockIdx.x); }
 
 // Several series in one launch: workgroup b belongs to the series r with
@@ -323,6 +617,9 @@ int filter_blocks(const FilterParams& p) {
 }
 
 size_t filter_lds_bytes(int n, int pairs_per_wg) {
+    // one padded in-place buffer on the power-of-two path, two buffers otherwise
+    if (pow2_twiddles(n, nullptr) > 0)
+        return static_cast<size_t>(pairs_per_wg) * p2_row_slots(n) * sizeof(float2);
     return static_cast<size_t>(2) * pairs_per_wg * n * sizeof(float2);
 }
 
@@ -338,6 +635,23 @@ int filter_plan(int n, int* radix, int* nradix) {
     if (m != 1) return -1;
     *nradix = k;
     return 0;
+}
+
+int pow2_twiddles(int n, float2* out) {
+    if (n < 64 || n > 2048 || (n & (n - 1)) != 0) return 0;
+    int off = 0;
+    for (int ns = 1; ns < n;) {
+        const int R = n / ns >= 8 ? 8 : n / ns;
+        for (int k = 0; k < ns; ++k)
+            for (int r = 0; r < R; ++r) {
+                const double a = -2.0 * M_PI * static_cast<double>(r) * k / (static_cast<double>(ns) * R);
+                if (out) out[off + k * R + r] = make_float2(static_cast<float>(std::cos(a)),
+                                                            static_cast<float>(std::sin(a)));
+            }
+        off += ns * R;
+        ns *= R;
+    }
+    return off;
 }
 
 hipError_t launch_filter(const FilterParams& p, hipStream_t st) {

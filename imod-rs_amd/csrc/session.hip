@@ -310,6 +310,14 @@ int prepare_filter(tiltgpu_session* s) {
     if ((rc = dev_alloc(s, &s->weights, n)) || (rc = dev_alloc(s, &s->tw, n))) return rc;
     HIP_TRY(hipMemcpy(s->weights, w.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    dev_free(s, s->tw8);
+    s->tw8 = nullptr;
+    if (const int m = pow2_twiddles(n, nullptr)) {
+        std::vector<float2> t8(m);
+        pow2_twiddles(n, t8.data());
+        if ((rc = dev_alloc(s, &s->tw8, m))) return rc;
+        HIP_TRY(hipMemcpy(s->tw8, t8.data(), m * sizeof(float2), hipMemcpyHostToDevice));
+    }
     return TILTGPU_OK;
 }
 
@@ -397,6 +405,9 @@ FilterParams make_filter(tiltgpu_session* s, const float* src, const float* sub,
     f.cubic = s->o.cosine_cubic;
     f.nradix = s->nradix;
     f.n_div = make_fastdiv(static_cast<uint32_t>(n));
+    f.tw8 = s->tw8;
+    f.log2n = 0;
+    while ((1 << f.log2n) < n) ++f.log2n;
     int ns = 1;
     for (int i = 0; i < s->nradix; ++i) {
         f.radix[i] = s->radix[i];

@@ -87,6 +87,7 @@ struct tiltgpu_session {
     float* stretch = nullptr;
     float* weights = nullptr;
     float2* tw = nullptr;
+    float2* tw8 = nullptr;     // per-pass twiddles of the power-of-two filter plan, or null
     int* err = nullptr;
     double* resid = nullptr;
     float* trace = nullptr;  // [sirt+1][trace_y1-trace_y0][out_nz][out_nx]

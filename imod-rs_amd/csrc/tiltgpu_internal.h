@@ -124,7 +124,17 @@ struct FilterParams {
     FastDiv m_div[kMaxRadix];       // by n / radix[s] (butterflies per row of pass s)
     FastDiv ns_div[kMaxRadix];      // by radix[0] * ... * radix[s-1] (pass s stride)
     FastDiv r_div[kMaxRadix];       // by radix[s] (outputs per butterfly, generic passes)
+    // Power-of-two plan (64 <= n <= 2048): log2 n and the per-pass twiddle table
+    // (pow2_twiddles); tw8 == null selects the mixed-radix passes above.
+    int log2n;
+    const float2* tw8;
 };
+
+// Per-pass twiddles of the power-of-two filter plan (radix-8 passes, then one
+// radix-4 or radix-2 pass): for pass s of radix R and span NS, entries
+// [off_s + k*R + r] = exp(-2*pi*i*r*k/(NS*R)), k < NS, r < R; returns the entry
+// count (0 when n is not a supported power of two).  out may be null.
+int pow2_twiddles(int n, float2* out);
 
 // Ordered-walk forward projection (no x-axis tilt, no local alignment).
 // Layout: vol [in_ny][out_nz][out_nx] f32, sino [nz][in_ny][in_nx] f32.

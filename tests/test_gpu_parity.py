@@ -143,6 +143,26 @@ def test_filter_stage_and_means(gpu, rel):
     assert rel(filt, st) <= 1e-5
 
 
+@pytest.mark.parametrize("nx", [64, 256, 512, 1024, 2048, 4096])
+def test_filter_power_of_two_widths(gpu, rel, nx):
+    """The power-of-two filter (register radix-8 passes, 64 <= n <= 2048; 4096
+    takes the mixed-radix passes) against the oracle's R-weighting + stretch,
+    with an odd row count so a row pair straddles two views and the last
+    workgroup holds fewer pairs than it can."""
+    ny, nv = 5, 7
+    ang = np.linspace(-60.0, 60.0, nv).astype(np.float32)
+    rng = np.random.default_rng(nx)
+    P = rng.normal(1.0, 0.5, size=(nv, ny, nx)).astype(np.float32)
+    print(f"case: filter width {nx}", flush=True)
+    vol, hs, s = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=8), return_session=True)
+    filt = s.read_filtered()
+    s.close()
+    rw = O.rweight(P.reshape(-1, nx), prec="f32").reshape(P.shape)
+    st = O.cosine_stretch(rw, ang, prec="f32")
+    print(f"width {nx}: vs FP32 oracle {rel(filt, st):.2e}")
+    assert rel(filt, st) <= 1e-5
+
+
 def test_sirt_matches_oracle(gpu, rel):
     P, ang = case(gpu, nx=96, ny=6, nviews=21, thick=32, seed=9)
     opts = gpu.TiltOptions(thickness=32, sirt=4, sirt_relax=0.8)
