@@ -174,6 +174,51 @@ def end_to_end_section(tg, P, ang, xt, local):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def section_transforms_section(tg, P, cpu=True, reps=3):
+    """newstack -x on the C1 stack (2048x2048x121): one seeded small rotation +
+    shift per section applied on the device (xf_apply_kernel: 4 B read + 4 B
+    written per pixel), timed with the session's HIP events; the oracle on two
+    sections, single-threaded as the reference's loop, is the CPU figure and the
+    bit-for-bit check."""
+    nz, ny, nx = P.shape
+    rng = np.random.default_rng(11)
+    a = np.deg2rad(rng.uniform(-3, 3, nz))
+    m = rng.uniform(0.99, 1.01, nz)
+    xf = np.stack([m * np.cos(a), -m * np.sin(a), m * np.sin(a), m * np.cos(a),
+                   rng.uniform(-8, 8, nz), rng.uniform(-8, 8, nz)], 1).astype(np.float32)
+    s = tg.Session(nx, ny, nz, tg.TiltOptions(thickness=512))
+    try:
+        s.set_angles(np.linspace(-60, 60, nz).astype(np.float32))
+        s.upload(P)
+        ms = []
+        got = None
+        for _ in range(reps + 1):
+            s.set_section_transforms(xf)
+            s.reset_timing()
+            got = s.read_projections()
+            ms.append(s.timing()["prep_ms"])
+        kms = float(np.median(ms[1:]))
+    finally:
+        s.close()
+    px = float(nx) * ny * nz
+    res = {"workload": f"newstack -x {nx}x{ny}x{nz}, per-section rotation +-3 deg, scale 0.99-1.01, "
+                       "shift +-8 px, bilinear, fill 0",
+           "kernel_ms": round(kms, 3), "gpixel_per_s": round(px / kms / 1e6, 2),
+           "hbm_gb_per_s": round(8.0 * px / kms / 1e6, 1),
+           "frac_of_hbm_peak": round(8.0 * px / kms / 1e6 / 8000.0, 3)}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O  # noqa: E402 -- checker and CPU figure only
+        t0 = time.perf_counter()
+        ref = O.apply_xf_stack(P[:2], xf[:2], nthreads=1)
+        dt = time.perf_counter() - t0
+        res["cpu_oracle"] = {"mpixel_per_s": round(2.0 * nx * ny / dt / 1e6, 1), "cores": 1,
+                             "sample": "2 sections, the reference's serial per-pixel loop restated"}
+        res["bitwise_equal_to_oracle_on_sample"] = bool(
+            np.array_equal(got[:2].view(np.uint32), ref.view(np.uint32)))
+    return res
+
+
 def cpu_baseline(P, ang, xt, local, rows=32):
     """The FP32 oracle (the reference's algorithm restated in C, its threading
     model) on a bounded slab: output rows [1008, 1040) of the same workload,
@@ -481,6 +526,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--stream-requests", type=int, default=48)
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
+    ap.add_argument("--no-xf", action="store_true",
+                    help="skip the newstack -x section-transform measurement")
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip the end-to-end run of the tilt command on files")
     ap.add_argument("--sirt-only", action="store_true", help="only the SIRT section")
@@ -595,6 +642,12 @@ def main():
         print(f"# error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
+    if not a.no_xf:
+        try:
+            line["section_transforms"] = section_transforms_section(tg, P, cpu=not a.no_cpu)
+        except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
+            line["section_transforms"] = {"error": str(e)[:300]}
+        print(f"# section transforms: {line['section_transforms']}", file=sys.stderr, flush=True)
     if not a.no_e2e:
         try:
             line["end_to_end"] = end_to_end_section(tg, P, ang, xt, local)

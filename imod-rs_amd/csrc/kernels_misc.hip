@@ -382,6 +382,51 @@ hipError_t launch_fma_rate(float* out, int blocks, int iters, hipStream_t st) {
     return hipGetLastError();
 }
 
+// newstack -x (apps/newstack/src/main.rs:1405-1462) over a stack: output pixel
+// (x, y) of section z samples section z at the inverse transform's position
+// about (nx/2, ny/2) (LinearTransform::apply, crates/imod-transforms/src/xf.rs:
+// 113-119) with interpolate_bilinear (crates/imod-slice/src/slice.rs:62-86);
+// taps outside the image take `fill`.  Compiled with -ffp-contract=off, so every
+// product and sum rounds as the reference's: the result is bit for bit the
+// reference's.  inv: [nz][6] inverse transforms (a11 a12 a21 a22 dx dy).
+__global__ void __launch_bounds__(256) xf_apply_kernel(const float* __restrict__ in,
+                                                       float* __restrict__ out, int nx, int ny,
+                                                       const float* __restrict__ inv, float fill) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const float* t = inv + 6 * z;
+    const float xcen = static_cast<float>(nx) / 2.0f, ycen = static_cast<float>(ny) / 2.0f;
+    const float xadj = static_cast<float>(x) - xcen, yadj = static_cast<float>(y) - ycen;
+    const float sx = ((t[0] * xadj + t[1] * yadj) + t[4]) + xcen;
+    const float sy = ((t[2] * xadj + t[3] * yadj) + t[5]) + ycen;
+    // x0 = floor(x) as isize; fx = x - x0 (exact for every x whose floor fits)
+    const float flx = floorf(sx), fly = floorf(sy);
+    const float fx = sx - flx, fy = sy - fly;
+    const float fnx = static_cast<float>(nx), fny = static_cast<float>(ny);
+    const bool x0in = flx >= 0.f && flx < fnx, x1in = flx + 1.f >= 0.f && flx + 1.f < fnx;
+    const bool y0in = fly >= 0.f && fly < fny, y1in = fly + 1.f >= 0.f && fly + 1.f < fny;
+    const int ix = x0in ? static_cast<int>(flx) : (x1in ? -1 : 0);
+    const int iy = y0in ? static_cast<int>(fly) : (y1in ? -1 : 0);
+    const float* sec = in + static_cast<int64_t>(z) * nx * ny;
+    const float v00 = (x0in && y0in) ? sec[static_cast<int64_t>(iy) * nx + ix] : fill;
+    const float v10 = (x1in && y0in) ? sec[static_cast<int64_t>(iy) * nx + ix + 1] : fill;
+    const float v01 = (x0in && y1in) ? sec[static_cast<int64_t>(iy + 1) * nx + ix] : fill;
+    const float v11 = (x1in && y1in) ? sec[static_cast<int64_t>(iy + 1) * nx + ix + 1] : fill;
+    const float ofx = 1.0f - fx, ofy = 1.0f - fy;
+    out[(static_cast<int64_t>(z) * ny + y) * nx + x] =
+        v00 * ofx * ofy + v10 * fx * ofy + v01 * ofx * fy + v11 * fx * fy;
+}
+
+hipError_t launch_apply_xf(const float* in, float* out, int nx, int ny, int nz, const float* inv,
+                           float fill, hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(xf_apply_kernel, dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st,
+                       in, out, nx, ny, inv, fill);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st) {
     const int64_t n4 = bytes / 16;
     if (n4 <= 0) return hipSuccess;

@@ -459,12 +459,12 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
         std::vector<FilterParams> f;
         std::vector<Req*> fr;
         for (Req* r : prep) {
-            const float* O = r->s->o.log ? r->s->logd : r->s->raw;
+            const float* O = r->s->o.log ? r->s->logd : stack_src(r->s);
             f.push_back(make_filter(r->s, O, nullptr, true));
             fr.push_back(r);
         }
         for (Req* r : iter) {
-            const float* O = r->s->o.log ? r->s->logd : r->s->raw;
+            const float* O = r->s->o.log ? r->s->logd : stack_src(r->s);
             FilterParams p = make_filter(r->s, O, r->s->sino, false);
             const int nblk = filter_blocks(p);
             if (r->s->partial_n < nblk) {

@@ -474,6 +474,21 @@ FpParams make_fp(tiltgpu_session* s) {
 }
 
 // --log and per-view means (main.rs:607-623)
+const float* stack_src(const tiltgpu_session* s) { return s->d_xf ? s->aligned : s->raw; }
+
+// The transformed stack from the uploaded one, once per upload or transform set.
+int ensure_aligned(tiltgpu_session* s) {
+    if (!s->d_xf || s->aligned_ok) return TILTGPU_OK;
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    int rc;
+    if (!s->aligned && (rc = dev_alloc(s, &s->aligned, nproj))) return rc;
+    Scope sc(s, kClsPrep);
+    HIP_TRY(launch_apply_xf(s->raw, s->aligned, s->d.in_nx, s->d.in_ny, s->d.nz, s->d_xf,
+                            s->xf_fill, s->stream));
+    s->aligned_ok = true;
+    return TILTGPU_OK;
+}
+
 int run_means(tiltgpu_session* s) {
     const int nz = s->d.nz;
     const int64_t nxy = static_cast<int64_t>(s->d.in_nx) * s->d.in_ny;
@@ -481,7 +496,7 @@ int run_means(tiltgpu_session* s) {
     int rc = ensure_partial(s, nblk * nz);
     if (rc) return rc;
     Scope sc(s, kClsPrep);
-    HIP_TRY(launch_log_means(s->raw, s->logd, nxy, nz, s->o.log, s->partial, nblk, s->means,
+    HIP_TRY(launch_log_means(stack_src(s), s->logd, nxy, nz, s->o.log, s->partial, nblk, s->means,
                              s->stream));
     return TILTGPU_OK;
 }
@@ -787,6 +802,7 @@ int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t co
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->uploaded = true;
+    s->aligned_ok = false;
     s->view_up.assign(s->d.nz, 1);
     s->views_up = s->d.nz;
     s->filtered = false;
@@ -812,7 +828,54 @@ int tiltgpu_upload_views(tiltgpu_session* s, int32_t v0, int32_t nv, const float
             ++s->views_up;
         }
     s->uploaded = s->views_up == s->d.nz;
+    s->aligned_ok = false;
     s->filtered = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_section_transforms(tiltgpu_session* s, const float* xf, int32_t n, float fill) {
+    if (!s || s->seam || n < 0 || (n > 0 && !xf)) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    s->aligned_ok = false;
+    s->filtered = false;
+    if (n == 0) {
+        dev_free(s, s->d_xf);
+        dev_free(s, s->aligned);
+        s->d_xf = s->aligned = nullptr;
+        return TILTGPU_OK;
+    }
+    // section z takes transform min(z, n-1) (apps/newstack/src/main.rs:1281-1283);
+    // LinearTransform::inverse in f32 (crates/imod-transforms/src/xf.rs:95-109)
+    std::vector<float> inv(6 * static_cast<size_t>(s->d.nz));
+    for (int z = 0; z < s->d.nz; ++z) {
+        const float* t = xf + 6 * std::min(z, n - 1);
+        const float a11 = t[0], a12 = t[1], a21 = t[2], a22 = t[3], dx = t[4], dy = t[5];
+        const float det = a11 * a22 - a12 * a21;
+        const float i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+        float* o = inv.data() + 6 * z;
+        o[0] = i11;
+        o[1] = i12;
+        o[2] = i21;
+        o[3] = i22;
+        o[4] = -(i11 * dx + i12 * dy);
+        o[5] = -(i21 * dx + i22 * dy);
+    }
+    int rc;
+    if (!s->d_xf && (rc = dev_alloc(s, &s->d_xf, inv.size()))) return rc;
+    HIP_TRY(hipMemcpy(s->d_xf, inv.data(), inv.size() * sizeof(float), hipMemcpyHostToDevice));
+    s->xf_fill = fill;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_read_projections(tiltgpu_session* s, float* out) {
+    if (!s || !out || s->seam) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
+    int rc;
+    if ((rc = ensure_aligned(s))) return rc;
+    const int64_t nproj = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
+    HIP_TRY(hipMemcpyAsync(out, stack_src(s), nproj * sizeof(float), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    collect_timing(s);
     return TILTGPU_OK;
 }
 
@@ -845,7 +908,8 @@ int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {
     int rc;
     if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
     if (!s->filtered) {
-        const float* O = s->o.log ? s->logd : s->raw;
+        if ((rc = ensure_aligned(s))) return rc;
+        const float* O = s->o.log ? s->logd : stack_src(s);
         if ((rc = run_means(s))) return rc;
         if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
         s->filtered = true;
@@ -865,7 +929,8 @@ int tiltgpu_run(tiltgpu_session* s) {
     if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
     int rc;
     if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
-    const float* O = s->o.log ? s->logd : s->raw;
+    if ((rc = ensure_aligned(s))) return rc;
+    const float* O = s->o.log ? s->logd : stack_src(s);
     // log + means
     if ((rc = run_means(s))) return rc;
     // R-weighting + cosine stretch (+ storage conversion)

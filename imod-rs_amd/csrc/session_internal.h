@@ -87,7 +87,13 @@ struct tiltgpu_session {
     float* stretch = nullptr;
     float* weights = nullptr;
     float2* tw = nullptr;
-    float2* tw8 = nullptr;     // per-pass twiddles of the power-of-two filter plan, or null
+    float2* tw8 = nullptr;
+    // newstack -x section transforms: inverses [nz][6] on the device; the
+    // transformed stack `aligned` is made from raw when first needed
+    float* d_xf = nullptr;
+    float xf_fill = 0.f;
+    float* aligned = nullptr;
+    bool aligned_ok = false;     // per-pass twiddles of the power-of-two filter plan, or null
     int* err = nullptr;
     double* resid = nullptr;
     float* trace = nullptr;  // [sirt+1][trace_y1-trace_y0][out_nz][out_nx]
@@ -228,6 +234,8 @@ int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stre
                double* sq_out);
 int run_forward(tiltgpu_session* s);
 int run_means(tiltgpu_session* s);
+const float* stack_src(const tiltgpu_session* s);  // the transformed stack, or raw
+int ensure_aligned(tiltgpu_session* s);
 int validate_dims(const tiltgpu_dims* d);
 // Allocates the session's buffers on `stream` (nullptr: create its own stream).
 // pool: optional buffer provider (a server's cache); its sessions also skip

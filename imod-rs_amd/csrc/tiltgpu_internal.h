@@ -224,6 +224,8 @@ hipError_t launch_geometry_f64(const float* angles, int n_angles, const float* x
                                int n_xtilt, const float* zfac, int n_zfac, int nviews, int nsec,
                                float max_tilt, int stretch, float* view_out, float* stretch_out,
                                hipStream_t st);
+hipError_t launch_apply_xf(const float* in, float* out, int nx, int ny, int nz, const float* inv,
+                           float fill, hipStream_t st);
 hipError_t launch_zc_f64(const float* angles, const float* zfac, int n_zfac, int nviews,
                          int nz_pad, int out_nz, float* zc, hipStream_t st);
 

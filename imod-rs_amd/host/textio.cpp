@@ -113,6 +113,38 @@ extern "C" int tiltgpu_read_per_view_file(const char* path, float* out, int32_t 
     return n;
 }
 
+// read_xf_file, crates/imod-transforms/src/xf.rs:147-174: blank lines skipped;
+// of each line the whitespace-separated tokens that parse as f32 are kept, and
+// a line with at least six of them is one transform (a11 a12 a21 a22 dx dy).
+extern "C" int tiltgpu_read_xf_file(const char* path, float* out, int32_t cap) {
+    std::ifstream f(path);
+    if (!f) return TILTGPU_ERR_IO;
+    std::string line;
+    int n = 0;
+    while (std::getline(f, line)) {
+        const std::string t = trim(line);
+        if (t.empty()) continue;
+        float vals[6];
+        int k = 0;
+        size_t i = 0;
+        while (i < t.size() && k < 6) {
+            while (i < t.size() && std::isspace(static_cast<unsigned char>(t[i]))) ++i;
+            size_t j = i;
+            while (j < t.size() && !std::isspace(static_cast<unsigned char>(t[j]))) ++j;
+            if (j > i) {
+                float v;
+                if (parse_f32(t.substr(i, j - i), &v)) vals[k++] = v;
+            }
+            i = j;
+        }
+        if (k < 6) continue;
+        if (n < cap && out)
+            for (int c = 0; c < 6; ++c) out[6 * n + c] = vals[c];
+        ++n;
+    }
+    return n;
+}
+
 // ------------------------------------------------------------ local alignment
 // load_local_alignment, apps/tilt/src/main.rs:232-307: lines "view x y dx dy"
 // ('#' and blank lines skipped, short lines skipped, each field unwrap_or(0)),

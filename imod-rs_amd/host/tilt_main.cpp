@@ -111,6 +111,8 @@ struct Args {
     int fake_sirt = 0;
     std::string storage = "f32";
     std::string geometry = "f32";
+    std::string xf_file;
+    float xf_fill = 0.0f;
     int slab_rows = 64;
     int upload_views = 0;
 };
@@ -144,6 +146,9 @@ struct Args {
                  "      --hamming              Hamming window instead of Gaussian roll-off\n"
                  "      --fake-sirt <n>        low-pass passes after reconstruction [0]\n"
                  "      --storage <s>          f32 | f16 | bf16 projection storage in HBM [f32]\n"
+                 "      --xf-file <f>          apply the .xf section transforms first (newstack -x,\n"
+                 "                             on the GPU)\n"
+                 "      --xf-fill <v>          fill value outside transformed sections [0]\n"
                  "      --geometry <g>         f32 (the reference's tables) | f64 (computed in\n"
                  "                             FP64 on the GPU, rounded once) [f32]\n"
                  "      --slab-rows <n>        output rows per device->file transfer [64]\n"
@@ -198,6 +203,8 @@ Args parse_args(int argc, char** argv) {
         {"fake-sirt", 0, 2, &a.fake_sirt},
         {"storage", 0, 1, &a.storage},
         {"geometry", 0, 1, &a.geometry},
+        {"xf-file", 0, 1, &a.xf_file},
+        {"xf-fill", 0, 3, &a.xf_fill},
         {"slab-rows", 0, 2, &a.slab_rows},
         {"upload-views", 0, 2, &a.upload_views},
     };
@@ -355,6 +362,17 @@ int main(int argc, char** argv) {
         const auto zf = read_values(a.zfactor_file, true, "zfactor file");
         check(tiltgpu_set_zfactor(S, zf.data(), static_cast<int32_t>(zf.size())), "zfactor");
     }
+    if (!a.xf_file.empty()) {
+        const int n = tiltgpu_read_xf_file(a.xf_file.c_str(), nullptr, 0);
+        if (n <= 0) {
+            std::fprintf(stderr, "Error reading transform file %s\n", a.xf_file.c_str());
+            return 1;
+        }
+        std::vector<float> xf(6 * static_cast<size_t>(n));
+        tiltgpu_read_xf_file(a.xf_file.c_str(), xf.data(), n);
+        check(tiltgpu_set_section_transforms(S, xf.data(), n, a.xf_fill), "section transforms");
+    }
+
     // projections (main.rs:601-605): chunks of sections read into page-locked
     // buffers on a worker thread while the previous chunk uploads
     const double t_start = now_s();

@@ -124,6 +124,9 @@ SIGNATURES = [
     ("tiltgpu_set_sirt_trace", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_read_sirt_trace", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_read_filtered", C.c_int, [_vp, _fp]),
+    ("tiltgpu_set_section_transforms", C.c_int, [_vp, _fp, C.c_int32, C.c_float]),
+    ("tiltgpu_read_projections", C.c_int, [_vp, _fp]),
+    ("tiltgpu_read_xf_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_means", C.c_int, [_vp, _fp]),
     ("tiltgpu_synchronize", C.c_int, [_vp]),
     ("tiltgpu_get_timing", C.c_int, [_vp, C.POINTER(Timing)]),
@@ -405,6 +408,17 @@ class Session:
         _check(lib().tiltgpu_reproject(self.h, _f(vol), _f(out)))
         return out
 
+    def set_section_transforms(self, xf, fill=0.0):
+        """newstack -x transforms [n][6] applied to the uploaded stack (n = 0 removes them)."""
+        x = np.ascontiguousarray(np.asarray(xf, np.float32).reshape(-1, 6))
+        _check(lib().tiltgpu_set_section_transforms(self.h, _f(x), x.shape[0], fill))
+
+    def read_projections(self):
+        """The stack the reconstruction reads (after the section transforms)."""
+        out = np.empty((self.dims.nz, self.dims.in_ny, self.dims.in_nx), np.float32)
+        _check(lib().tiltgpu_read_projections(self.h, _f(out)))
+        return out
+
     def read_filtered(self):
         out = np.empty((self.dims.nz, self.dims.in_ny, self.dims.in_nx), np.float32)
         _check(lib().tiltgpu_read_filtered(self.h, _f(out)))
@@ -568,6 +582,14 @@ def mrc_read(path):
     out = np.empty((info.nz, info.ny, info.nx), np.float32)
     _check(lib().tiltgpu_mrc_read_sections(path.encode(), 0, info.nz, _f(out), out.size))
     return out
+
+
+def read_xf_file(path):
+    """Transforms of an .xf file as [n][6] (crates/imod-transforms/src/xf.rs:147-174)."""
+    n = _check(lib().tiltgpu_read_xf_file(path.encode(), None, 0))
+    out = np.zeros((max(n, 1), 6), np.float32)
+    _check(lib().tiltgpu_read_xf_file(path.encode(), _f(out), n))
+    return out[:n]
 
 
 def mrc_write(path, vol, xlen=0.0, ylen=0.0, zlen=0.0, label=None):

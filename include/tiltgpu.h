@@ -217,6 +217,21 @@ int tiltgpu_read_sirt_trace(tiltgpu_session* s, int32_t k, float* out);
  * the session's volume; sections beyond the tilt angles come out zero. */
 int tiltgpu_reproject(tiltgpu_session* s, const float* vol, float* out);
 
+/* Section transforms applied to the uploaded stack before the reconstruction:
+ * `newstack -x` (apps/newstack/src/main.rs:1405-1462), the step that makes the
+ * aligned stack `tilt` reads.  xf is [n][6] (a11 a12 a21 a22 dx dy, the .xf
+ * order); section z takes transform min(z, n-1); output pixels sample the
+ * section bilinearly at the inverse transform's position about (nx/2, ny/2)
+ * (crates/imod-transforms/src/xf.rs:95-119, crates/imod-slice/src/slice.rs:
+ * 62-86), `fill` outside.  n = 0 removes them.  Bit-identical to the
+ * reference's FP32 arithmetic. */
+int tiltgpu_set_section_transforms(tiltgpu_session* s, const float* xf, int32_t n, float fill);
+
+/* Copy the stack the reconstruction reads back to the host, [nz][in_ny][in_nx]:
+ * the uploaded projections, after the section transforms when they are set
+ * (newstack's output).  No counterpart in the reference as an entry point. */
+int tiltgpu_read_projections(tiltgpu_session* s, float* out);
+
 /* Copy the filtered (and stretched) projections back, dequantised to FP32,
  * [nz][in_ny][in_nx]; for parity tests of the filter stage. */
 int tiltgpu_read_filtered(tiltgpu_session* s, float* out);
@@ -306,6 +321,10 @@ int tiltgpu_fma_calibration(int32_t iters, double* tflops);
 int tiltgpu_read_tilt_file(const char* path, float* out, int32_t cap);
 /* load_per_view_file (main.rs:310-326); a parse error is TILTGPU_ERR_IO. */
 int tiltgpu_read_per_view_file(const char* path, float* out, int32_t cap);
+/* read_xf_file (crates/imod-transforms/src/xf.rs:147-174): transforms of an .xf
+ * file as [n][6] (a11 a12 a21 a22 dx dy) into out (up to cap transforms);
+ * returns n or TILTGPU_ERR_IO. */
+int tiltgpu_read_xf_file(const char* path, float* out, int32_t cap);
 
 /* --local-file (load_local_alignment, main.rs:232-307) parsed into per-view
  * grids for tiltgpu_set_local_alignment.  sizes[3] receives the total lengths of

@@ -84,6 +84,10 @@ def _declare(L):
         getattr(L, f"or_dft_float_{suf}").argtypes = [_fp, _fp, C.c_int, C.c_int]
         getattr(L, f"or_local_interp_float_{suf}").argtypes = [
             _fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float, C.c_float, _fp]
+        getattr(L, f"or_apply_xf_stack_{suf}").argtypes = [
+            _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_float, _fp, C.c_int]
+    L.or_xf_inverse_f32.argtypes = [_fp, _fp]
+    L.or_xf_inverse_f64.argtypes = [_fp, _dp]
     L.or_radial_weights_f32.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _fp]
     L.or_radial_weights_f64.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _dp]
     L.or_proj_params_f32.argtypes = [_fp, C.c_int, _fp, C.c_int, _fp, C.c_int,
@@ -115,6 +119,43 @@ def read_tilt_file(path):
             if v is not None:
                 out.append(v)
     return np.array(out, dtype=np.float32)
+
+
+def read_xf_file(path):
+    """read_xf_file, crates/imod-transforms/src/xf.rs:147-174: blank lines
+    skipped; the whitespace-separated tokens that parse as f32 are kept
+    (filter_map), and a line with at least six of them gives
+    (a11, a12, a21, a22, dx, dy)."""
+    out = []
+    with open(path, "r") as fh:
+        for line in fh:
+            t = line.strip()
+            if not t:
+                continue
+            vals = [v for v in (_rust_parse_f32(tok) for tok in t.split()) if v is not None]
+            if len(vals) >= 6:
+                out.append(vals[:6])
+    return np.array(out, dtype=np.float32).reshape(-1, 6)
+
+
+def xf_inverse(xf6, prec="f32"):
+    """LinearTransform::inverse (xf.rs:95-109)."""
+    a = _f32c(np.asarray(xf6).reshape(6))
+    dt = np.float32 if prec == "f32" else np.float64
+    out = np.zeros(6, dt)
+    getattr(lib(), f"or_xf_inverse_{prec}")(_f(a), out.ctypes.data_as(_fp if prec == "f32" else _dp))
+    return out
+
+
+def apply_xf_stack(P, xf, fill=0.0, prec="f32", nthreads=8):
+    """newstack -x (apps/newstack/src/main.rs:1405-1462) over a stack [nz][ny][nx]."""
+    a = _f32c(P)
+    nz, ny, nx = a.shape
+    x = _f32c(np.asarray(xf).reshape(-1, 6))
+    out = np.empty_like(a)
+    getattr(lib(), f"or_apply_xf_stack_{prec}")(_f(a), nx, ny, nz, _f(x), x.shape[0], fill,
+                                                _f(out), nthreads)
+    return out
 
 
 def load_per_view_file(path):

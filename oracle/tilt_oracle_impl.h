@@ -884,6 +884,79 @@ void F(or_dft_float)(float* re, float* im, int n, int sign) {
     free(b);
 }
 
+/* ----------------------------------------------- newstack -x section transforms
+ * LinearTransform::inverse, crates/imod-transforms/src/xf.rs:95-109: det, the
+ * four divisions and the shifted offsets, each in REAL in the reference's order. */
+void F(or_xf_inverse)(const float* xf6, REAL* inv6) {
+    const REAL a11 = xf6[0], a12 = xf6[1], a21 = xf6[2], a22 = xf6[3], dx = xf6[4], dy = xf6[5];
+    const REAL det = a11 * a22 - a12 * a21;
+    const REAL i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+    inv6[0] = i11;
+    inv6[1] = i12;
+    inv6[2] = i21;
+    inv6[3] = i22;
+    inv6[4] = -(i11 * dx + i12 * dy);
+    inv6[5] = -(i21 * dx + i22 * dy);
+}
+
+/* Slice::get_clamped + interpolate_bilinear, crates/imod-slice/src/slice.rs:62-86.
+ * x0 = floor(x) as isize and fx = x - x0 as f32: for every x whose floor fits an
+ * isize, (x0 as f32) == floor(x), so fx = x - floor(x) and the tap bounds are
+ * compared on floor(x) itself (a NaN position gives NaN either way). */
+static REAL F(xf_bilinear)(const REAL* img, int nx, int ny, REAL x, REAL y, REAL fill) {
+    const REAL flx = floor(x), fly = floor(y);
+    const REAL fx = x - flx, fy = y - fly;
+#define XF_IN(f, n) ((f) >= 0 && (f) < (REAL)(n))
+#define XF_AT(fx0, fy0) ((XF_IN(fx0, nx) && XF_IN(fy0, ny)) ? img[(long)(fy0) * nx + (long)(fx0)] : fill)
+    const REAL v00 = XF_AT(flx, fly), v10 = XF_AT(flx + 1, fly);
+    const REAL v01 = XF_AT(flx, fly + 1), v11 = XF_AT(flx + 1, fly + 1);
+#undef XF_AT
+#undef XF_IN
+    const REAL ofx = (REAL)1.0 - fx, ofy = (REAL)1.0 - fy;
+    return v00 * ofx * ofy + v10 * fx * ofy + v01 * ofx * fy + v11 * fx * fy;
+}
+
+typedef struct {
+    const float* in;
+    float* out;
+    int nx, ny, nxf;
+    const float* xf;
+    REAL fill;
+} F(xf_ctx);
+
+static void F(xf_sections)(void* vctx, long z0, long z1) {
+    const F(xf_ctx)* c = (const F(xf_ctx)*)vctx;
+    const long plane = (long)c->nx * c->ny;
+    REAL* img = (REAL*)malloc(sizeof(REAL) * plane);
+    const REAL xcen = (REAL)((float)c->nx / 2.0f), ycen = (REAL)((float)c->ny / 2.0f);
+    for (long z = z0; z < z1; z++) {
+        const float* src = c->in + z * plane;
+        for (long i = 0; i < plane; i++) img[i] = (REAL)src[i];
+        REAL inv[6];
+        F(or_xf_inverse)(c->xf + 6 * (z < c->nxf - 1 ? z : c->nxf - 1), inv);
+        float* dst = c->out + z * plane;
+        for (int y = 0; y < c->ny; y++)
+            for (int x = 0; x < c->nx; x++) {
+                /* LinearTransform::apply, xf.rs:113-119 */
+                const REAL xadj = (REAL)x - xcen, yadj = (REAL)y - ycen;
+                const REAL sx = inv[0] * xadj + inv[1] * yadj + inv[4] + xcen;
+                const REAL sy = inv[2] * xadj + inv[3] * yadj + inv[5] + ycen;
+                dst[(long)y * c->nx + x] = (float)F(xf_bilinear)(img, c->nx, c->ny, sx, sy, c->fill);
+            }
+    }
+    free(img);
+}
+
+/* newstack -x, apps/newstack/src/main.rs:1405-1462: section z takes transform
+ * min(z, nxf-1) (:1281-1283 in the montage path, the same rule), every output
+ * pixel samples the input at the inverse-transformed position about the centre
+ * (nx/2, ny/2) (:1411-1412) with bilinear interpolation and fill outside. */
+void F(or_apply_xf_stack)(const float* in, int nx, int ny, int nz, const float* xf, int nxf,
+                          float fill, float* out, int nthreads) {
+    F(xf_ctx) c = {in, out, nx, ny, nxf, xf, (REAL)fill};
+    or_parallel_for(nz, 1, nthreads, F(xf_sections), &c);
+}
+
 void F(or_local_interp_float)(const float* xs, int nx, const float* ys, int ny,
                               const float* gdx, const float* gdy, float x, float y,
                               float* out2) {

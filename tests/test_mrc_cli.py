@@ -152,3 +152,30 @@ def test_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
     scale = float(np.sqrt(np.mean(ref.astype(np.float64) ** 2)))
     assert abs(info.amin - mn) <= 1e-4 * scale and abs(info.amax - mx) <= 1e-4 * scale
     assert abs(info.amean - mean) <= 1e-4 * scale
+
+
+@pytest.mark.gpu
+def test_cli_section_transforms(gpu, tmp_path, rel):
+    """`tilt --xf-file` applies the newstack -x transforms on the GPU before the
+    reconstruction: the result equals the oracle's reconstruction of the
+    oracle-transformed stack."""
+    ang = np.linspace(-50, 50, 11).astype(np.float32)
+    P = gpu.synth_series(80, 10, ang, vol_nz=30, n_ellipsoids=15, n_beads=3, seed=4,
+                         axis=(2.0, 8.0), bead_radius=2.0)
+    inp, out = str(tmp_path / "in.mrc"), str(tmp_path / "out.rec")
+    gpu.mrc_write(inp, P)
+    (tmp_path / "a.tlt").write_text("".join(f"{a:10.2f}\n" for a in ang))
+    xf = [[0.9961947, -0.0871557, 0.0871557, 0.9961947, 2.25, -1.5],
+          [1.0, 0.0, 0.0, 1.0, -3.0, 0.75]]
+    (tmp_path / "a.xf").write_text("".join("%12.7f%12.7f%12.7f%12.7f%12.3f%12.3f\n" % tuple(x)
+                                           for x in xf))
+    cmd = [TILT, "-i", inp, "-o", out, "-t", str(tmp_path / "a.tlt"), "-z", "30",
+           "--xf-file", str(tmp_path / "a.xf"), "--xf-fill", "0.5"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    vol = gpu.mrc_read(out)
+    aligned = O.apply_xf_stack(P, O.read_xf_file(str(tmp_path / "a.xf")), fill=0.5)
+    ref, _ = O.tilt_run(aligned, O.read_tilt_file(str(tmp_path / "a.tlt")),
+                        O.TiltOptions(thickness=30))
+    assert rel(vol, ref) <= 1e-4

@@ -393,3 +393,126 @@ def test_output_transform_and_header_stats():
     assert np.array_equal(v1, (v0 * np.float32(2.0)) * np.float32(0.5) + np.float32(3.0))
     assert mn == v1.min() and mx == v1.max()
     assert mean == pytest.approx(float(v1.astype(np.float64).mean()), rel=1e-6)
+
+
+# ------------------------------------------------- newstack -x transforms --
+
+def _then(a, b):
+    """LinearTransform::then, crates/imod-transforms/src/xf.rs:83-92 (a, then b)."""
+    a11, a12, a21, a22, adx, ady = [np.float32(v) for v in a]
+    b11, b12, b21, b22, bdx, bdy = [np.float32(v) for v in b]
+    return [b11 * a11 + b12 * a21, b11 * a12 + b12 * a22, b21 * a11 + b22 * a21,
+            b21 * a12 + b22 * a22, b11 * adx + b12 * ady + bdx, b21 * adx + b22 * ady + bdy]
+
+
+def test_xf_inverse_roundtrip_known_answer():
+    # crates/imod-transforms/src/tests.rs:40-57
+    xf = [0.95, -0.31, 0.31, 0.95, 5.0, -3.0]
+    r = _then(xf, O.xf_inverse(xf))
+    assert abs(r[0] - 1.0) < 1e-5 and abs(r[1]) < 1e-5 and abs(r[2]) < 1e-5
+    assert abs(r[3] - 1.0) < 1e-5 and abs(r[4]) < 1e-4 and abs(r[5]) < 1e-4
+    # the FP64 form inverts exactly to rounding
+    inv64 = O.xf_inverse(xf, prec="f64")
+    A = np.array([[0.95, -0.31], [0.31, 0.95]], np.float64)
+    assert np.allclose(np.linalg.inv(np.array(A, np.float32).astype(np.float64)),
+                       inv64[:4].reshape(2, 2), rtol=1e-12)
+
+
+def test_xf_file_roundtrip_known_answer(tmp_path):
+    # crates/imod-transforms/src/tests.rs:81-115, written in the 4f12.7 2f12.3 format
+    # of write_xf_file (xf.rs:176-191); both parsers (oracle and library host) read it
+    import tiltgpu
+    xforms = [[1.0, 0.0, 0.0, 1.0, 0.0, 0.0],
+              [0.9848077, -0.1736482, 0.1736482, 0.9848077, 5.123, -3.456]]
+    p = tmp_path / "test.xf"
+    p.write_text("".join("%12.7f%12.7f%12.7f%12.7f%12.3f%12.3f\n" % tuple(x) for x in xforms)
+                 + "\n  bad line 1 2\n")
+    for got in (O.read_xf_file(str(p)), tiltgpu.read_xf_file(str(p))):
+        assert got.shape == (2, 6)
+        for a, b in zip(got, xforms):
+            assert np.all(np.abs(a[:4] - np.float32(b[:4])) < 1e-5)
+            assert np.all(np.abs(a[4:] - np.float32(b[4:])) < 0.01)
+    assert np.array_equal(O.read_xf_file(str(p)), tiltgpu.read_xf_file(str(p)))
+
+
+def test_xf_apply_identity_translation_rotation():
+    rng = np.random.default_rng(5)
+    P = rng.normal(size=(3, 20, 30)).astype(np.float32)
+    ident = np.tile(np.array([1, 0, 0, 1, 0, 0], np.float32), (3, 1))
+    assert np.array_equal(O.apply_xf_stack(P, ident), P)
+    # an integer translation moves the image and fills the uncovered band
+    Q = O.apply_xf_stack(P, [1, 0, 0, 1, 3, -2], fill=7.0)
+    assert np.array_equal(Q[:, :18, 3:], P[:, 2:, :27])
+    assert np.all(Q[:, 18:, :] == 7.0) and np.all(Q[:, :, :3] == 7.0)
+    # +90 degrees about the centre of an even square image: a permutation,
+    # out[y][x] = in[n - x][y], with the x = 0 column outside
+    S = P[:, :, :20]
+    Q = O.apply_xf_stack(S, [0, -1, 1, 0, 0, 0], fill=-5.0)
+    assert np.all(Q[:, :, 0] == -5.0)
+    for x in range(1, 20):
+        assert np.array_equal(Q[:, :, x], S[:, 20 - x, :])
+
+
+def test_xf_apply_linear_ramp_is_reproduced():
+    # bilinear interpolation is exact for an affine image: inside the image the
+    # output is the ramp at the inverse-transformed position
+    ny, nx = 40, 50
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    img = (0.3 * xx - 0.7 * yy + 2.0).astype(np.float32)[None]
+    xf = np.array([0.98, -0.12, 0.11, 1.02, 1.7, -2.3], np.float32)
+    Q = O.apply_xf_stack(img, xf, fill=np.nan)
+    inv = O.xf_inverse(xf, prec="f64")
+    sx = inv[0] * (xx - nx / 2) + inv[1] * (yy - ny / 2) + inv[4] + nx / 2
+    sy = inv[2] * (xx - nx / 2) + inv[3] * (yy - ny / 2) + inv[5] + ny / 2
+    inside = (sx >= 0) & (sx <= nx - 1.001) & (sy >= 0) & (sy <= ny - 1.001)
+    ref = 0.3 * sx - 0.7 * sy + 2.0
+    assert inside.sum() > 0.8 * nx * ny
+    assert np.max(np.abs(Q[0][inside] - ref[inside])) < 1e-4
+
+
+def _xf_numpy_f32(P, xf, fill):
+    """An independent vectorised restatement in numpy float32, same operation order."""
+    nz, ny, nx = P.shape
+    out = np.empty_like(P)
+    f32 = np.float32
+    xcen, ycen = f32(nx) / f32(2), f32(ny) / f32(2)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    xadj = xx.astype(f32) - xcen
+    yadj = yy.astype(f32) - ycen
+    for z in range(nz):
+        t = np.asarray(xf, f32).reshape(-1, 6)[min(z, len(np.asarray(xf).reshape(-1, 6)) - 1)]
+        a11, a12, a21, a22, dx, dy = [f32(v) for v in t]
+        det = a11 * a22 - a12 * a21
+        i11, i12, i21, i22 = a22 / det, -a12 / det, -a21 / det, a11 / det
+        idx, idy = -(i11 * dx + i12 * dy), -(i21 * dx + i22 * dy)
+        sx = i11 * xadj + i12 * yadj + idx + xcen
+        sy = i21 * xadj + i22 * yadj + idy + ycen
+        fx0, fy0 = np.floor(sx), np.floor(sy)
+        fx, fy = sx - fx0, sy - fy0
+
+        def tap(fxx, fyy):
+            ok = (fxx >= 0) & (fxx < nx) & (fyy >= 0) & (fyy < ny)
+            v = np.full(sx.shape, f32(fill), f32)
+            v[ok] = P[z][fyy[ok].astype(np.int64), fxx[ok].astype(np.int64)]
+            return v
+        v00, v10 = tap(fx0, fy0), tap(fx0 + 1, fy0)
+        v01, v11 = tap(fx0, fy0 + 1), tap(fx0 + 1, fy0 + 1)
+        ofx, ofy = f32(1) - fx, f32(1) - fy
+        out[z] = v00 * ofx * ofy + v10 * fx * ofy + v01 * ofx * fy + v11 * fx * fy
+    return out
+
+
+def test_xf_apply_equals_independent_numpy_restatement():
+    rng = np.random.default_rng(9)
+    P = rng.normal(size=(5, 33, 47)).astype(np.float32)
+    xf = []
+    for _ in range(3):  # fewer transforms than sections: the last one repeats
+        a = np.deg2rad(rng.uniform(-20, 20))
+        m = rng.uniform(0.9, 1.1)
+        xf.append([m * np.cos(a), -m * np.sin(a), m * np.sin(a), m * np.cos(a),
+                   rng.uniform(-6, 6), rng.uniform(-6, 6)])
+    xf = np.array(xf, np.float32)
+    Q = O.apply_xf_stack(P, xf, fill=0.25)
+    assert np.array_equal(Q.view(np.uint32), _xf_numpy_f32(P, xf, 0.25).view(np.uint32))
+    Q64 = O.apply_xf_stack(P, xf, fill=0.25, prec="f64")
+    assert np.max(np.abs(Q - Q64)) < 1e-4 * np.sqrt(np.mean(Q64.astype(np.float64) ** 2)) * 50
