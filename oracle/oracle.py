@@ -87,6 +87,7 @@ def _declare(L):
         getattr(L, f"or_apply_xf_stack_{suf}").argtypes = [
             _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_float, _fp, C.c_int]
     L.or_xf_inverse_f32.argtypes = [_fp, _fp]
+    L.or_studio_reconstruct.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp]
     L.or_xf_inverse_f64.argtypes = [_fp, _dp]
     L.or_radial_weights_f32.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _fp]
     L.or_radial_weights_f64.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, _dp]
@@ -155,6 +156,17 @@ def apply_xf_stack(P, xf, fill=0.0, prec="f32", nthreads=8):
     out = np.empty_like(a)
     getattr(lib(), f"or_apply_xf_stack_{prec}")(_f(a), nx, ny, nz, _f(x), x.shape[0], fill,
                                                 _f(out), nthreads)
+    return out
+
+
+def studio_reconstruct(P, angles):
+    """imod-studio step_reconstruction (apps/imod-studio/src/main.rs:725-770):
+    [in_ny][in_nx (z)][in_nx]."""
+    a = _f32c(P)
+    nz, ny, nx = a.shape
+    ang = _f32c(angles)
+    out = np.empty((ny, nx, nx), np.float32)
+    lib().or_studio_reconstruct(_f(a), nx, ny, nz, _f(ang), ang.size, _f(out))
     return out
 
 

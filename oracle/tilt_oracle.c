@@ -109,3 +109,39 @@ typedef struct {
 #undef REAL
 #undef SUF
 #undef REAL_IS_DOUBLE
+
+/* imod-studio step_reconstruction, apps/imod-studio/src/main.rs:725-770: the
+ * unweighted, unstretched backprojection of the studio's pipeline, in f32 in
+ * its operation order: out_nz = in_nx; for each Y row, views pi < min(n, nz)
+ * ascending, px = (ox - cx)*cos + (oz - cz)*sin + cx, a lerp of the row where
+ * 0 <= p0 && p0 + 1 < in_nx (nothing otherwise), then * (1/nz).
+ * out: [in_ny][in_nx][in_nx]. */
+void or_studio_reconstruct(const float* P, int in_nx, int in_ny, int nz, const float* angles,
+                           int n_angles, float* out) {
+    const int out_nz = in_nx;
+    const float cx = (float)in_nx / 2.0f, cz = (float)out_nz / 2.0f;
+    const float pi_f = 3.14159265358979323846f;
+    for (int iy = 0; iy < in_ny; iy++) {
+        float* sl = out + (long)iy * out_nz * in_nx;
+        memset(sl, 0, sizeof(float) * (size_t)out_nz * in_nx);
+        for (int pi = 0; pi < n_angles && pi < nz; pi++) {
+            const float r = angles[pi] * pi_f / 180.0f;
+            const float cos_t = cosf(r), sin_t = sinf(r);
+            const float* row = P + ((long)pi * in_ny + iy) * in_nx;
+            for (int oz = 0; oz < out_nz; oz++) {
+                const float bo = ((float)oz - cz) * sin_t;
+                for (int ox = 0; ox < in_nx; ox++) {
+                    const float px = ((float)ox - cx) * cos_t + bo + cx;
+                    const float fl = floorf(px);
+                    if (fl >= 0.0f && fl + 1.0f < (float)in_nx) {
+                        const long p0 = (long)fl;
+                        const float f = px - (float)p0;
+                        sl[(long)oz * in_nx + ox] += row[p0] * (1.0f - f) + row[p0 + 1] * f;
+                    }
+                }
+            }
+        }
+        const float inv = 1.0f / (float)nz;
+        for (long i = 0; i < (long)out_nz * in_nx; i++) sl[i] *= inv;
+    }
+}

@@ -512,6 +512,20 @@ def test_section_transforms_bitwise(gpu, rel):
     assert rel(vol, ref) <= TOL
 
 
+def test_studio_reconstruction_route(gpu, rel):
+    """imod-studio's step_reconstruction (apps/imod-studio/src/main.rs:725-770)
+    routed through the library: no weighting, no stretch, zero fill, thickness
+    = width, against the restatement of the studio's own loop."""
+    P, ang = case(gpu, nx=96, ny=6, nviews=21, thick=32, seed=26)
+    print("case: studio route 96x6x21", flush=True)
+    g, _ = gpu.reconstruct(P, ang[:19], gpu.TiltOptions(thickness=96, no_weight=True,
+                                                         no_cosine_stretch=True,
+                                                         fill_mode="zero"))
+    ref = O.studio_reconstruct(P, ang[:19])
+    assert g.shape == ref.shape == (6, 96, 96)
+    assert rel(g, ref) <= TOL
+
+
 def test_c0_config(gpu, rel):
     """Config 0: 512x512 projections, 61 views -60..60 step 2, thickness 128,
     radial 0.35,0.05, phantom of 200 ellipsoids + 30 beads, noise 0.1."""

@@ -516,3 +516,18 @@ def test_xf_apply_equals_independent_numpy_restatement():
     assert np.array_equal(Q.view(np.uint32), _xf_numpy_f32(P, xf, 0.25).view(np.uint32))
     Q64 = O.apply_xf_stack(P, xf, fill=0.25, prec="f64")
     assert np.max(np.abs(Q - Q64)) < 1e-4 * np.sqrt(np.mean(Q64.astype(np.float64) ** 2)) * 50
+
+
+def test_studio_reconstruction_equals_tilt_without_weighting():
+    """imod-studio's reconstruction step (apps/imod-studio/src/main.rs:725-770)
+    restated on its own is bit for bit the tilt oracle with --no-weight
+    --no-cosine-stretch --fill-mode zero and thickness = width: two independent
+    restatements of the same loop (cos_xt = zfactor = 1 multiply exactly)."""
+    rng = np.random.default_rng(13)
+    nz, ny, nx = 9, 4, 40
+    P = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    ang = np.linspace(-55, 55, 8).astype(np.float32)  # fewer angles than sections
+    studio = O.studio_reconstruct(P, ang)
+    vol, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=nx, no_weight=True,
+                                              no_cosine_stretch=True, fill_mode="zero"))
+    assert np.array_equal(studio.view(np.uint32), vol.view(np.uint32))
