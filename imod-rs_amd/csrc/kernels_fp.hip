@@ -122,7 +122,10 @@ __device__ __forceinline__ void walk_window(const WalkLane& L, float zcv, int& e
 template <int B>
 __device__ __forceinline__ void walk_voxel(const WalkLane& L, char* lds, float dx, float zcv,
                                            float v) {
-    const float px = ((dx * L.ct) * L.cxt + zcv) + L.cx;
+    // The walk serves the single-row geometry only: every view has sin_xt == 0,
+    // hence xtilt == 0 and cos_xt == 1.0f exactly, and (dx*cos_t)*cos_xt is
+    // dx*cos_t bit for bit.
+    const float px = (dx * L.ct + zcv) + L.cx;
     const int fi = cvt_floor(px);
     const float frac = __builtin_amdgcn_fractf(px);
     const float w = static_cast<unsigned>(fi) < L.nvalid ? v : 0.0f;
@@ -145,13 +148,18 @@ __device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
     const float dx_end = static_cast<float>(e1) - L.ocx;
     const float* vp = vz + e0;
     if constexpr (STAGED) {
-        // the staged row is padded, so the read one past e1 <= out_nx stays in it
-        float vn = *vp;
-        for (; dx < dx_end; dx += 1.0f) {
-            const float v = vn;
-            vn = *++vp;
-            walk_voxel<B>(L, lds, dx, zcv, v);
+        // two voxels per step (one paired LDS read), the odd one after; dx + 1
+        // is exact, so every voxel sees the same dx as in a one-by-one walk
+        (void)dx_end;
+        const int n = e1 - e0;
+        int i = 0;
+        for (; i + 1 < n; i += 2) {
+            const float v0 = vp[i], v1 = vp[i + 1];
+            walk_voxel<B>(L, lds, dx, zcv, v0);
+            walk_voxel<B>(L, lds, dx + 1.0f, zcv, v1);
+            dx += 2.0f;
         }
+        if (i < n) walk_voxel<B>(L, lds, dx, zcv, vp[i]);
     } else {
         const float* vlast = vz + L.last;
         float vn = *(vp < vlast ? vp : vlast);

@@ -85,8 +85,10 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
     const int nv = static_cast<int>(vp.size());
     s->nviews = nv;
     s->two_row = s->has_local;
+    // (sin_xt == 0 implies cos_xt == 1 exactly; the second test keeps the
+    // single-row kernels, which rely on it, safe for any table)
     for (const auto& g : vp)
-        if (g.sin_xt != 0.0f) s->two_row = true;
+        if (g.sin_xt != 0.0f || g.cos_xt != 1.0f) s->two_row = true;
     // tile depth: minimise padded depth x (1 + per-view overhead / voxels per lane);
     // LDS capacity from the per-view bounds
     const int max_lds_bytes = 64 * 1024;
