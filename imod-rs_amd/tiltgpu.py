@@ -239,7 +239,7 @@ class TiltOptions:
         o.densweight, o.outscale, o.outadd = self.densweight, self.outscale, self.outadd
         o.sirt, o.sirt_relax, o.fake_sirt = int(self.sirt), float(self.sirt_relax), int(self.fake_sirt)
         o.storage = STORAGE[self.storage]
-        o.geometry = GEOMETRY[self.geometry]
+        o.geometry = GEOMETRY.get(self.geometry.lower(), -1)  # unknown: refused by the library
         return o
 
 

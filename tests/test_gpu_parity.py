@@ -526,6 +526,34 @@ def test_studio_reconstruction_route(gpu, rel):
     assert rel(g, ref) <= TOL
 
 
+def test_new_entry_points_reject_bad_arguments(gpu):
+    """Chunked uploads outside the sections, a run before every section has
+    arrived, negative transform counts and bad geometry modes are refused with
+    an error code and leave the session usable."""
+    P, ang = case(gpu, seed=27, ny=4, nviews=5)
+    s = gpu.Session(P.shape[2], P.shape[1], P.shape[0], gpu.TiltOptions(thickness=16))
+    try:
+        s.set_angles(ang)
+        with pytest.raises(gpu.TiltGpuError):
+            s.upload_views(4, P[:2])        # sections 4..5 of 5
+        with pytest.raises(gpu.TiltGpuError):
+            s.upload_views(-1, P[:1])
+        s.upload_views(0, P[:3])
+        with pytest.raises(gpu.TiltGpuError):
+            s.run()                         # sections 3, 4 missing
+        with pytest.raises(gpu.TiltGpuError):
+            s.read_projections()
+        assert gpu.lib().tiltgpu_set_section_transforms(s.h, None, -1, 0.0) < 0
+        s.upload_views(3, P[3:])
+        s.run()
+        vol, _ = s.read_rows()
+        assert np.isfinite(vol).all()
+    finally:
+        s.close()
+    with pytest.raises(gpu.TiltGpuError):
+        gpu.Session(32, 4, 5, gpu.TiltOptions(thickness=16, geometry="f128"))
+
+
 def test_c0_config(gpu, rel):
     """Config 0: 512x512 projections, 61 views -60..60 step 2, thickness 128,
     radial 0.35,0.05, phantom of 200 ellipsoids + 30 beads, noise 0.1."""
