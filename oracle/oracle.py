@@ -86,6 +86,9 @@ def _declare(L):
             _fp, C.c_int, _fp, C.c_int, _fp, _fp, C.c_float, C.c_float, _fp]
         getattr(L, f"or_apply_xf_stack_{suf}").argtypes = [
             _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_float, _fp, C.c_int]
+        getattr(L, f"or_ctf_phaseflip_{suf}").argtypes = [
+            _fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp, C.c_float, _dp, C.c_int, _fp,
+            C.c_int]
     L.or_xf_inverse_f32.argtypes = [_fp, _fp]
     L.or_studio_reconstruct.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp]
     L.or_xf_inverse_f64.argtypes = [_fp, _dp]
@@ -167,6 +170,115 @@ def studio_reconstruct(P, angles):
     ang = _f32c(angles)
     out = np.empty((ny, nx, nx), np.float32)
     lib().or_studio_reconstruct(_f(a), nx, ny, nz, _f(ang), ang.size, _f(out))
+    return out
+
+
+def electron_wavelength(voltage_kv):
+    """Relativistic electron wavelength in Angstroms, apps/ctfphaseflip/src/main.rs:707-717."""
+    v = float(np.float32(voltage_kv)) * 1000.0
+    m0, e, c, h = 9.10938e-31, 1.60218e-19, 2.99792e8, 6.62607e-34
+    lam = h / np.sqrt(2.0 * m0 * e * v * (1.0 + e * v / (2.0 * m0 * c * c)))
+    return lam * 1e10
+
+
+def ctf_strip_width(strip_width, nx, pixel_a, wavelength, defocus_nm, vdef=None,
+                    max_strip_width=None):
+    """The strip width of apps/ctfphaseflip/src/main.rs:272-298: --max-strip-width,
+    else pixel / (wavelength * max|defocus| / nx) (at least 16) when the defocus is
+    meaningful; the requested width clamped to it."""
+    if max_strip_width is not None:
+        max_w = int(max_strip_width)
+    else:
+        if vdef is not None and len(vdef):
+            mx = max(max(abs(float(a)), abs(float(b))) * 10.0 for a, b, _ in vdef)
+        else:
+            mx = abs(float(np.float32(defocus_nm))) * 10.0
+        if mx > 1.0 and wavelength > 0.0:
+            per_px = mx / float(nx)
+            max_w = int(max(float(np.float32(pixel_a)) / (wavelength * per_px), 16.0))
+        else:
+            max_w = int(strip_width)
+    return min(int(strip_width), max_w)
+
+
+def read_defocus_file(path):
+    """read_defocus_file, apps/ctfphaseflip/src/main.rs:103-157: '#' and blank lines
+    skipped; 'view d1 [d2 [astig]]' (nm, nm, degrees); a malformed line raises."""
+    entries, views = [], []
+    with open(path) as fh:
+        for ln, line in enumerate(fh, 1):
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            f = t.split()
+            if len(f) < 2:
+                raise ValueError(f"line {ln}: expected at least 'view defocus1'")
+            if not f[0].isdigit():
+                raise ValueError(f"line {ln}: invalid view number {f[0]!r}")
+            d1 = _rust_parse_f32(f[1])
+            if d1 is None:
+                raise ValueError(f"line {ln}: invalid defocus1 {f[1]!r}")
+            d2, aa = d1, 0.0
+            if len(f) >= 3:
+                d2 = _rust_parse_f32(f[2])
+                if d2 is None:
+                    raise ValueError(f"line {ln}: invalid defocus2 {f[2]!r}")
+            if len(f) >= 4:
+                aa = _rust_parse_f32(f[3])
+                if aa is None:
+                    raise ValueError(f"line {ln}: invalid astig_angle {f[3]!r}")
+            views.append(int(f[0]))
+            entries.append((np.float32(d1), np.float32(d2), np.float32(aa)))
+    return entries, views
+
+
+def interpolate_defocus(entries, views, n_views):
+    """interpolate_defocus, apps/ctfphaseflip/src/main.rs:164-198 (f32 lerp
+    between the nearest listed views, the nearest one beyond the ends)."""
+    if not entries or n_views == 0:
+        return np.zeros((0, 3), np.float32)
+    res = np.zeros((n_views, 3), np.float32)
+    have = np.zeros(n_views, bool)
+    for v, e in zip(views, entries):
+        if v < n_views:
+            res[v] = e
+            have[v] = True
+    for v in range(n_views):
+        if have[v]:
+            continue
+        b = next((i for i in range(v - 1, -1, -1) if have[i]), None)
+        a = next((i for i in range(v + 1, n_views) if have[i]), None)
+        if b is not None and a is not None:
+            t = np.float32(v - b) / np.float32(a - b)
+            res[v] = res[b] + t * (res[a] - res[b])
+        elif b is not None:
+            res[v] = res[b]
+        elif a is not None:
+            res[v] = res[a]
+    return res
+
+
+def ctf_phaseflip(P, tilt_deg, pixel_a, voltage_kv=300.0, cs_mm=2.7, defocus_nm=-3000.0,
+                  vdef=None, strip_width=256, amp=0.07, cuton=0.0, plate_deg=0.0,
+                  tilt_axis_deg=0.0, prec="f32", nthreads=8):
+    """ctfphaseflip's CPU path (apps/ctfphaseflip/src/main.rs:369-712) over a stack;
+    strip_width is the width actually used (ctf_strip_width)."""
+    a = _f32c(P)
+    nz, ny, nx = a.shape
+    t = _f32c(tilt_deg)
+    f32pi = float(np.float32(np.pi))
+    phys = np.array([electron_wavelength(voltage_kv), float(np.float32(cs_mm)) * 1e7,
+                     float(np.float32(amp)), float(np.float32(cuton)),
+                     float(np.float32(plate_deg)) * f32pi / 180.0,
+                     float(np.float32(tilt_axis_deg)) * f32pi / 180.0,
+                     float(np.float32(pixel_a))], np.float64)
+    vd = None if vdef is None or len(vdef) == 0 else _f32c(np.asarray(vdef).reshape(-1, 3))
+    if vd is not None:
+        assert vd.shape[0] >= nz
+    out = np.empty_like(a)
+    getattr(lib(), f"or_ctf_phaseflip_{prec}")(_f(a), nx, ny, nz, _f(t), t.size, _f(vd),
+                                               float(defocus_nm), phys.ctypes.data_as(_dp),
+                                               int(strip_width), _f(out), nthreads)
     return out
 
 

@@ -957,6 +957,167 @@ void F(or_apply_xf_stack)(const float* in, int nx, int ny, int nz, const float* 
     or_parallel_for(nz, 1, nthreads, F(xf_sections), &c);
 }
 
+/* --------------------------------------------------------- ctfphaseflip --
+ * The CPU path of apps/ctfphaseflip/src/main.rs:369-712 for one section: strips
+ * across the tilt axis, each strip's bounding box 2D-transformed (rows, then
+ * columns), the Fourier coefficients negated where the CTF (evaluated in
+ * double, :563-630) is negative, transformed back (columns, then rows) and
+ * blended with the triangular strip weight; pixels with no weight keep their
+ * value.  Transforms and pixel arithmetic in REAL. */
+typedef struct {
+    const float* in;
+    float* out;
+    int nx, ny, nz, strip_width;
+    const float* tilt_deg;
+    int n_tilt;
+    const float* vdef;  /* [nz][3] defocus1 nm, defocus2 nm, astig deg, or NULL */
+    float default_defocus_nm;
+    double wavelength, cs_a, amp, cuton, plate_rad, tilt_axis_rad, pixel_a;
+} F(ctf_ctx);
+
+static void F(ctf_sections)(void* vctx, long z0, long z1) {
+    const F(ctf_ctx)* c = (const F(ctf_ctx)*)vctx;
+    const int nx = c->nx, ny = c->ny;
+    const double pi = 3.14159265358979323846;
+    const double perp_x = cos(c->tilt_axis_rad), perp_y = sin(c->tilt_axis_rad);
+    const double center_x = nx / 2.0, center_y = ny / 2.0;
+    REAL* wsum = (REAL*)malloc(sizeof(REAL) * (size_t)nx * ny);
+    REAL* wtot = (REAL*)malloc(sizeof(REAL) * (size_t)nx * ny);
+    F(cpx)* strip = (F(cpx)*)malloc(sizeof(F(cpx)) * (size_t)nx * ny);
+    const int nmax = nx > ny ? nx : ny;
+    F(cpx)* line = (F(cpx)*)malloc(sizeof(F(cpx)) * (size_t)nmax);
+    F(cpx)* work = (F(cpx)*)malloc(sizeof(F(cpx)) * 3 * (size_t)nmax);
+    const double w = c->amp, w_phase = sqrt(1.0 - w * w);
+    for (long z = z0; z < z1; z++) {
+        const float* data = c->in + z * (long)nx * ny;
+        float* output = c->out + z * (long)nx * ny;
+        const float tdeg = z < c->n_tilt ? c->tilt_deg[z] : 0.0f;
+        const double tilt_rad = (double)tdeg * (double)(float)pi / 180.0;
+        float d1 = c->default_defocus_nm, d2 = c->default_defocus_nm, aa = 0.0f;
+        if (c->vdef) {
+            d1 = c->vdef[3 * z];
+            d2 = c->vdef[3 * z + 1];
+            aa = c->vdef[3 * z + 2];
+        }
+        const double def1_a = (double)d1 * 10.0, def2_a = (double)d2 * 10.0;
+        const double astig_rad = (double)aa * (double)(float)pi / 180.0;
+        const int astig = fabs(def1_a - def2_a) > 0.01;
+        for (long i = 0; i < (long)nx * ny; i++) {
+            output[i] = data[i];
+            wsum[i] = 0;
+            wtot[i] = 0;
+        }
+        double min_perp = DBL_MAX, max_perp = -DBL_MAX;
+        const double cxs[4] = {0.0, (double)nx, 0.0, (double)nx}, cys[4] = {0.0, 0.0, (double)ny, (double)ny};
+        for (int k = 0; k < 4; k++) {
+            const double d = (cxs[k] - center_x) * perp_x + (cys[k] - center_y) * perp_y;
+            if (d < min_perp) min_perp = d;
+            if (d > max_perp) max_perp = d;
+        }
+        const double sw_d = (double)c->strip_width;
+        const int n_strips = (int)ceil((max_perp - min_perp) / sw_d);
+        for (int si = 0; si < n_strips; si++) {
+            const double scp = min_perp + ((double)si + 0.5) * sw_d;
+            const double dx_a = scp * (double)(float)c->pixel_a;
+            const double sbd = def1_a + dx_a * sin(tilt_rad);
+            const double sbd2 = def2_a + dx_a * sin(tilt_rad);
+            const double lo = scp - sw_d / 2.0, hi = scp + sw_d / 2.0;
+            int bx0 = nx, bx1 = 0, by0 = ny, by1 = 0;
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    const double d = ((double)x - center_x) * perp_x + ((double)y - center_y) * perp_y;
+                    if (d >= lo && d < hi) {
+                        if (x < bx0) bx0 = x;
+                        if (x >= bx1) bx1 = x + 1;
+                        if (y < by0) by0 = y;
+                        if (y >= by1) by1 = y + 1;
+                    }
+                }
+            if (bx1 <= bx0 || by1 <= by0) continue;
+            const int sw = bx1 - bx0, sh = by1 - by0;
+            for (int y = 0; y < sh; y++)
+                for (int x = 0; x < sw; x++) {
+                    strip[y * sw + x].re = (REAL)data[(long)(by0 + y) * nx + bx0 + x];
+                    strip[y * sw + x].im = 0;
+                }
+            for (int y = 0; y < sh; y++) F(dft)(strip + (long)y * sw, sw, -1, work);
+            for (int x = 0; x < sw; x++) {
+                for (int y = 0; y < sh; y++) line[y] = strip[(long)y * sw + x];
+                F(dft)(line, sh, -1, work);
+                for (int y = 0; y < sh; y++) strip[(long)y * sw + x] = line[y];
+            }
+            for (int fy = 0; fy < sh; fy++) {
+                const double freq_y = fy <= sh / 2 ? (double)fy : (double)fy - (double)sh;
+                for (int fx = 0; fx < sw; fx++) {
+                    const double freq_x = fx <= sw / 2 ? (double)fx : (double)fx - (double)sw;
+                    double rx = freq_x, ry = freq_y;
+                    if (fabs(c->tilt_axis_rad) > 1e-6) {
+                        const double ca = cos(c->tilt_axis_rad), sa = sin(c->tilt_axis_rad);
+                        rx = freq_x * ca + freq_y * sa;
+                        ry = -freq_x * sa + freq_y * ca;
+                    }
+                    const double sx = rx / ((double)sw * (double)(float)c->pixel_a);
+                    const double sy = ry / ((double)sh * (double)(float)c->pixel_a);
+                    const double s2 = sx * sx + sy * sy;
+                    const double sm = sqrt(s2);
+                    if (sm < c->cuton) continue;
+                    double def = sbd;
+                    if (astig) {
+                        const double da = atan2(sy, sx) - astig_rad;
+                        const double cos2 = cos(da) * cos(da), sin2 = sin(da) * sin(da);
+                        def = sbd * cos2 + sbd2 * sin2;
+                    }
+                    const double chi = pi * c->wavelength * s2 *
+                                       (def - 0.5 * c->cs_a * c->wavelength * c->wavelength * s2);
+                    const double chi_t = chi + c->plate_rad;
+                    const double ctf = -sin(chi_t) * w_phase + cos(chi_t) * w;
+                    if (ctf < 0.0) {
+                        strip[(long)fy * sw + fx].re = -strip[(long)fy * sw + fx].re;
+                        strip[(long)fy * sw + fx].im = -strip[(long)fy * sw + fx].im;
+                    }
+                }
+            }
+            for (int x = 0; x < sw; x++) {
+                for (int y = 0; y < sh; y++) line[y] = strip[(long)y * sw + x];
+                F(dft)(line, sh, +1, work);
+                for (int y = 0; y < sh; y++) strip[(long)y * sw + x] = line[y];
+            }
+            for (int y = 0; y < sh; y++) F(dft)(strip + (long)y * sw, sw, +1, work);
+            const REAL scale = (REAL)1 / (REAL)(sw * sh);
+            for (int y = by0; y < by1; y++)
+                for (int x = bx0; x < bx1; x++) {
+                    const double d = ((double)x - center_x) * perp_x + ((double)y - center_y) * perp_y;
+                    if (d >= lo && d < hi) {
+                        const float dist = (float)fabs((d - scp) / (sw_d / 2.0));
+                        const float wt0 = 1.0f - dist;
+                        const REAL wt = (REAL)(wt0 > 0.0f ? wt0 : 0.0f);
+                        const REAL val = strip[(long)(y - by0) * sw + (x - bx0)].re * scale;
+                        wsum[(long)y * nx + x] += val * wt;
+                        wtot[(long)y * nx + x] += wt;
+                    }
+                }
+        }
+        for (long i = 0; i < (long)nx * ny; i++)
+            if (wtot[i] > (REAL)1e-10) output[i] = (float)(wsum[i] / wtot[i]);
+    }
+    free(wsum);
+    free(wtot);
+    free(strip);
+    free(line);
+    free(work);
+}
+
+void F(or_ctf_phaseflip)(const float* in, int nx, int ny, int nz, const float* tilt_deg,
+                         int n_tilt, const float* vdef, float default_defocus_nm,
+                         const double* phys /* wavelength, cs_a, amp, cuton, plate_rad,
+                                               tilt_axis_rad, pixel_a */,
+                         int strip_width, float* out, int nthreads) {
+    F(ctf_ctx) c = {in, out, nx, ny, nz, strip_width, tilt_deg, n_tilt, vdef,
+                    default_defocus_nm, phys[0], phys[1], phys[2], phys[3], phys[4], phys[5],
+                    phys[6]};
+    or_parallel_for(nz, 1, nthreads, F(ctf_sections), &c);
+}
+
 void F(or_local_interp_float)(const float* xs, int nx, const float* ys, int ny,
                               const float* gdx, const float* gdy, float x, float y,
                               float* out2) {

@@ -531,3 +531,75 @@ def test_studio_reconstruction_equals_tilt_without_weighting():
     vol, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=nx, no_weight=True,
                                               no_cosine_stretch=True, fill_mode="zero"))
     assert np.array_equal(studio.view(np.uint32), vol.view(np.uint32))
+
+
+# ------------------------------------------------------------ ctfphaseflip --
+
+def test_electron_wavelength_known_values():
+    # relativistic wavelength (textbook values: 0.01969 A at 300 kV, 0.02508 A at 200 kV)
+    assert abs(O.electron_wavelength(300.0) - 0.019687) < 2e-6
+    assert abs(O.electron_wavelength(200.0) - 0.025079) < 2e-6
+
+
+@pytest.mark.parametrize("defocus_nm", [-3000.0, 3000.0, 12000.0])
+def test_ctf_flip_of_a_single_frequency(defocus_nm):
+    """One strip covering a cosine of 5 cycles: its two Fourier coefficients are
+    negated exactly when CTF(s) = -sin(pi*lambda*s^2*def) (Cs = 0, w = 0) is
+    negative, so the output is +-input; column 0 sits on the strip edge where the
+    triangular weight is 0 and keeps the input."""
+    nx, ny, pix = 64, 48, 2.0
+    x = np.arange(nx)
+    img = np.repeat(np.cos(2 * np.pi * 5 * x / nx)[None, :], ny, 0).astype(np.float32)
+    P = np.stack([img, 2 * img])
+    s = 5.0 / (nx * pix)
+    ctf = -np.sin(np.pi * O.electron_wavelength(300.0) * s * s * defocus_nm * 10.0)
+    Q = O.ctf_phaseflip(P, [0.0, 0.0], pix, cs_mm=0.0, amp=0.0, defocus_nm=defocus_nm,
+                        strip_width=nx)
+    sign = np.sign(ctf)
+    assert np.max(np.abs(Q[:, :, 1:] - sign * P[:, :, 1:])) < 1e-5
+    assert np.array_equal(Q[:, :, 0], P[:, :, 0])
+
+
+def test_ctf_no_flip_is_identity_and_precisions_agree():
+    """Zero defocus, Cs 0, full amplitude contrast: CTF = cos(0) = 1 everywhere,
+    nothing is flipped, and the strips give back the image to transform
+    rounding; on a real case the f32 and f64 paths agree."""
+    rng = np.random.default_rng(17)
+    P = rng.normal(size=(3, 40, 56)).astype(np.float32)
+    Q = O.ctf_phaseflip(P, [0.0, 20.0, -40.0], 2.5, cs_mm=0.0, amp=1.0, defocus_nm=0.0,
+                        strip_width=16)
+    assert np.max(np.abs(Q - P)) < 1e-5
+    kw = dict(defocus_nm=-4000.0, strip_width=16, tilt_axis_deg=7.0, plate_deg=30.0,
+              vdef=[(-4000, -4300, 20), (-4100, -4100, 0), (-3900, -4200, 35)])
+    a = O.ctf_phaseflip(P, [0.0, 20.0, -40.0], 2.5, **kw)
+    b = O.ctf_phaseflip(P, [0.0, 20.0, -40.0], 2.5, prec="f64", **kw)
+    assert np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)) < 1e-5 * np.sqrt(np.mean(b ** 2))
+
+
+def test_ctf_defocus_file_and_interpolation(tmp_path):
+    # read_defocus_file / interpolate_defocus, apps/ctfphaseflip/src/main.rs:103-198
+    p = tmp_path / "d.txt"
+    p.write_text("# view d1 d2 astig\n1 -3000\n\n4 -3600 -3900 30\n6 -4000 -4100\n")
+    e, v = O.read_defocus_file(str(p))
+    assert v == [1, 4, 6]
+    assert e[0] == (np.float32(-3000), np.float32(-3000), np.float32(0))
+    assert e[2] == (np.float32(-4000), np.float32(-4100), np.float32(0))
+    d = O.interpolate_defocus(e, v, 8)
+    assert np.array_equal(d[0], d[1]) and np.array_equal(d[7], d[6])
+    t = np.float32(1) / np.float32(3)
+    assert d[2][0] == np.float32(-3000) + t * (np.float32(-3600) - np.float32(-3000))
+    assert d[5][1] == np.float32(-3900) + np.float32(0.5) * (np.float32(-4100) - np.float32(-3900))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("2\n")
+    with pytest.raises(ValueError):
+        O.read_defocus_file(str(bad))
+
+
+def test_ctf_strip_width_rule():
+    # apps/ctfphaseflip/src/main.rs:272-298
+    lam = O.electron_wavelength(300.0)
+    # pixel / (lambda * max|defocus| / nx), at least 16: 6.9 -> 16 here
+    assert O.ctf_strip_width(256, 2048, 2.0, lam, -3000.0) == 16
+    assert O.ctf_strip_width(256, 2048, 20.0, lam, -3000.0) == int(20.0 / (lam * 30000 / 2048))
+    assert O.ctf_strip_width(256, 2048, 2.0, lam, -3000.0, max_strip_width=100) == 100
+    assert O.ctf_strip_width(256, 2048, 2.0, lam, 0.0) == 256
