@@ -219,6 +219,53 @@ def section_transforms_section(tg, P, cpu=True, reps=3):
     return res
 
 
+def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0):
+    """ctfphaseflip on the C1 stack (2048x2048x121): 300 kV, Cs 2.7 mm, defocus
+    -3000 nm, 4 A pixels (automatic strip width: 16 px), tilt axis at 0 deg (the
+    reference's default: strips are column bands) and at 85 deg (bounding boxes
+    the full width and ~195 rows).  Device time of tiltgpu_ctf_run (HIP events,
+    the whole stack resident); the oracle (FP32, the reference's CPU path
+    restated) on a few sections is the CPU figure and the parity check."""
+    nz, ny, nx = P.shape
+    tilt = np.linspace(-60.0, 60.0, nz).astype(np.float32)
+    out = {}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O  # noqa: E402 -- checker and CPU figure only
+    for axis, ncpu in ((0.0, 8), (85.0, 1)):
+        prm = tg.ctf_params(pixel_a=pixel, tilt_axis_deg=axis)
+        c = tg.CtfCorrector(nx, ny, nz, prm, tilt)
+        try:
+            c.upload(P)
+            ms = []
+            for _ in range(reps + 1):
+                c.run()
+                ms.append(c.last_run())
+            g = c.download(0, ncpu) if cpu else None
+            width = c.strip_width
+        finally:
+            c.close()
+        kms = float(np.median([m for m, _ in ms[1:]]))
+        boxes, _ = tg.ctf_strip_layout(nx, ny, axis, width)
+        area = sum(int(b[1] - b[0]) * int(b[3] - b[2]) for b in boxes if b[1] > b[0] and b[3] > b[2])
+        r = {"workload": f"ctfphaseflip {nx}x{ny}x{nz}, pixel {pixel} A, defocus -3000 nm, "
+                         f"tilt axis {axis:g} deg, strip width {width}",
+             "strips": int(len(boxes)), "transformed_pixels_per_section": area,
+             "device_ms": round(kms, 2), "launches": int(ms[-1][1]),
+             "mpixel_per_s": round(float(nx) * ny * nz / kms / 1e3, 1)}
+        if cpu:
+            t0 = time.perf_counter()
+            ref = O.ctf_phaseflip(P[:ncpu], tilt[:ncpu], pixel, strip_width=width,
+                                  tilt_axis_deg=axis, nthreads=ncpu)
+            dt = time.perf_counter() - t0
+            r["cpu_oracle"] = {"mpixel_per_s": round(ncpu * float(nx) * ny / dt / 1e6, 2),
+                               "cores": ncpu, "sample": f"{ncpu} section(s), FP32 restatement"}
+            r["max_abs_err_over_rms_vs_oracle"] = float(f"{err_vs(g, ref)['max_abs_over_rms']:.3e}")
+        out[f"axis_{axis:g}"] = r
+        print(f"# ctfphaseflip axis {axis:g}: {r}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(P, ang, xt, local, rows=32):
     """The FP32 oracle (the reference's algorithm restated in C, its threading
     model) on a bounded slab: output rows [1008, 1040) of the same workload,
@@ -530,6 +577,8 @@ def main():
                     help="skip the newstack -x section-transform measurement")
     ap.add_argument("--no-e2e", action="store_true",
                     help="skip the end-to-end run of the tilt command on files")
+    ap.add_argument("--no-ctf", action="store_true", help="skip the ctfphaseflip section")
+    ap.add_argument("--ctf-only", action="store_true", help="only the ctfphaseflip section")
     ap.add_argument("--sirt-only", action="store_true", help="only the SIRT section")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one plain 2 GiB device copy, then one step; for rocprofv3 --pmc runs")
@@ -567,6 +616,9 @@ def main():
     t0 = time.perf_counter()
     P, ang, xt, local = c1_inputs(tg)
     print(f"# inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if a.ctf_only:
+        print(json.dumps({"ctfphaseflip": ctf_section(tg, P, cpu=not a.no_cpu)}), flush=True)
+        return
     nx, ny, nv = P.shape[2], P.shape[1], P.shape[0]
     vv = float(nx) * ny * 512 * nv
 
@@ -654,6 +706,11 @@ def main():
         except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
             line["end_to_end"] = {"error": str(e)[:300]}
         print(f"# end to end: {line['end_to_end']}", file=sys.stderr, flush=True)
+    if not a.no_ctf:
+        try:
+            line["ctfphaseflip"] = ctf_section(tg, P, cpu=not a.no_cpu)
+        except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
+            line["ctfphaseflip"] = {"error": str(e)[:300]}
     if not a.no_sirt:
         line["sirt"] = sirt_section(tg, oracle=not a.no_cpu)
     if not a.no_serve:

@@ -20,9 +20,10 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["csrc/kernels_bp.hip", "csrc/kernels_filter.hip", "csrc/kernels_fp.hip",
-               "csrc/kernels_misc.hip", "csrc/session.hip", "csrc/server.hip"]
+               "csrc/kernels_misc.hip", "csrc/session.hip", "csrc/server.hip", "csrc/ctf.hip"]
 HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp", "csrc/tuning.cpp"]
-CLI_SOURCES = ["host/tilt_main.cpp"]
+# drop-in executables: name -> sources (linked against libtiltgpu)
+CLIS = {"tilt": ["host/tilt_main.cpp"], "ctfphaseflip": ["host/ctfphaseflip_main.cpp"]}
 
 # -ffp-contract=off: coordinate arithmetic must round like the reference (which
 # never fuses); fused multiply-adds are written explicitly where wanted.
@@ -47,8 +48,8 @@ def lib_path() -> str:
     return os.path.join(OUT, "libtiltgpu.so")
 
 
-def cli_path() -> str:
-    return os.path.join(OUT, "tilt")
+def cli_path(name: str = "tilt") -> str:
+    return os.path.join(OUT, name)
 
 
 def build(jobs: int = 8, verbose: bool = False) -> str:
@@ -77,12 +78,16 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
           "-lpthread"])
     os.replace(tmp, lib_path())
-    cli = [os.path.join(HERE, s) for s in CLI_SOURCES if os.path.exists(os.path.join(HERE, s))]
-    if cli:
-        tmpc = cli_path() + ".tmp"
-        _run([CXX, *HOST_FLAGS, *cli, "-o", tmpc, "-L" + OUT, "-ltiltgpu",
-              "-Wl,-rpath,$ORIGIN", "-lpthread"])
-        os.replace(tmpc, cli_path())
+    cli_steps = []
+    for name, srcs in CLIS.items():
+        paths = [os.path.join(HERE, s) for s in srcs if os.path.exists(os.path.join(HERE, s))]
+        if paths:
+            cli_steps.append((name, [CXX, *HOST_FLAGS, *paths, "-o", cli_path(name) + ".tmp",
+                                     "-L" + OUT, "-ltiltgpu", "-Wl,-rpath,$ORIGIN", "-lpthread"]))
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, [cmd for _, cmd in cli_steps]))
+    for name, _ in cli_steps:
+        os.replace(cli_path(name) + ".tmp", cli_path(name))
     return lib_path()
 
 

@@ -145,6 +145,95 @@ extern "C" int tiltgpu_read_xf_file(const char* path, float* out, int32_t cap) {
     return n;
 }
 
+// ------------------------------------------------------------ defocus files
+// read_defocus_file, apps/ctfphaseflip/src/main.rs:103-157: '#' and blank lines
+// skipped; "view defocus1 [defocus2 [astig]]" (nm, nm, degrees; one value means
+// defocus2 = defocus1, three mean astig = 0).  A malformed line makes the whole
+// file an error (the reference exits), reported as -(line number) - 100.
+extern "C" int tiltgpu_read_defocus_file(const char* path, int32_t* views, float* entries,
+                                         int32_t cap) {
+    std::ifstream f(path);
+    if (!f) return TILTGPU_ERR_IO;
+    std::string line;
+    int n = 0, ln = 0;
+    while (std::getline(f, line)) {
+        ++ln;
+        const std::string t = trim(line);
+        if (t.empty() || t[0] == '#') continue;
+        std::vector<std::string> fl;
+        size_t i = 0;
+        while (i < t.size()) {
+            while (i < t.size() && std::isspace(static_cast<unsigned char>(t[i]))) ++i;
+            size_t j = i;
+            while (j < t.size() && !std::isspace(static_cast<unsigned char>(t[j]))) ++j;
+            if (j > i) fl.push_back(t.substr(i, j - i));
+            i = j;
+        }
+        long long view;
+        float d1, d2, aa = 0.0f;
+        if (fl.size() < 2 || !parse_usize(fl[0], &view) || !parse_f32(fl[1], &d1))
+            return -100 - ln;
+        d2 = d1;
+        if (fl.size() >= 3 && !parse_f32(fl[2], &d2)) return -100 - ln;
+        if (fl.size() >= 4 && !parse_f32(fl[3], &aa)) return -100 - ln;
+        if (n < cap && views && entries) {
+            views[n] = static_cast<int32_t>(std::min<long long>(view, INT32_MAX));
+            entries[3 * n] = d1;
+            entries[3 * n + 1] = d2;
+            entries[3 * n + 2] = aa;
+        }
+        ++n;
+    }
+    return n;
+}
+
+// interpolate_defocus, apps/ctfphaseflip/src/main.rs:164-198: listed views
+// (< n_views) take their entry (a later line for the same view wins); a gap
+// takes the f32 linear interpolation between the nearest listed views around
+// it, or the nearest one where it lies beyond the ends.  Returns the row count
+// written to out [n_views][3] (0 when nothing is listed).
+extern "C" int tiltgpu_interpolate_defocus(const int32_t* views, const float* entries, int32_t n,
+                                           int32_t n_views, float* out) {
+    if (n < 0 || n_views < 0 || (n > 0 && (!views || !entries)) || (n_views > 0 && !out))
+        return TILTGPU_ERR_ARG;
+    if (n == 0 || n_views == 0) return 0;
+    std::vector<char> have(n_views, 0);
+    for (int k = 0; k < 3 * n_views; ++k) out[k] = 0.0f;
+    for (int e = 0; e < n; ++e) {
+        const int v = views[e];
+        if (v < 0 || v >= n_views) continue;
+        for (int c = 0; c < 3; ++c) out[3 * v + c] = entries[3 * e + c];
+        have[v] = 1;
+    }
+    for (int v = 0; v < n_views; ++v) {
+        if (have[v]) continue;
+        int b = -1, a = -1;
+        for (int i = v - 1; i >= 0; --i)
+            if (have[i]) {
+                b = i;
+                break;
+            }
+        for (int i = v + 1; i < n_views; ++i)
+            if (have[i]) {
+                a = i;
+                break;
+            }
+        if (b >= 0 && a >= 0) {
+            const float t = static_cast<float>(v - b) / static_cast<float>(a - b);
+            for (int c = 0; c < 3; ++c) {
+                const float lo = out[3 * b + c], hi = out[3 * a + c];
+                const float d = hi - lo;
+                const float td = t * d;
+                out[3 * v + c] = lo + td;
+            }
+        } else if (b >= 0 || a >= 0) {
+            const int src = b >= 0 ? b : a;
+            for (int c = 0; c < 3; ++c) out[3 * v + c] = out[3 * src + c];
+        }
+    }
+    return n_views;
+}
+
 // ------------------------------------------------------------ local alignment
 // load_local_alignment, apps/tilt/src/main.rs:232-307: lines "view x y dx dy"
 // ('#' and blank lines skipped, short lines skipped, each field unwrap_or(0)),

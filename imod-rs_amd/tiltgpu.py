@@ -89,6 +89,14 @@ class ServerStats(C.Structure):
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double)]
 
 
+class CtfParams(C.Structure):
+    _fields_ = [("voltage_kv", C.c_float), ("cs_mm", C.c_float), ("pixel_a", C.c_float),
+                ("amplitude_contrast", C.c_float), ("cuton_freq", C.c_float),
+                ("plate_phase_deg", C.c_float), ("tilt_axis_deg", C.c_float),
+                ("defocus_nm", C.c_float), ("strip_width", C.c_int32),
+                ("max_strip_width", C.c_int32)]
+
+
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -153,6 +161,23 @@ SIGNATURES = [
     ("tiltgpu_server_destroy", None, [_vp]),
     ("tiltgpu_copy_calibration", C.c_int, [C.c_int64, C.c_int32, _dp]),
     ("tiltgpu_fma_calibration", C.c_int, [C.c_int32, _dp]),
+    ("tiltgpu_ctf_default_params", None, [C.POINTER(CtfParams)]),
+    ("tiltgpu_electron_wavelength", C.c_double, [C.c_float]),
+    ("tiltgpu_ctf_strip_width", C.c_int, [C.POINTER(CtfParams), C.c_int32, _fp, C.c_int32]),
+    ("tiltgpu_ctf_strip_layout", C.c_int, [C.c_int32, C.c_int32, C.c_float, C.c_int32, _ip, _dp,
+                                           C.c_int32]),
+    ("tiltgpu_ctf_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(CtfParams), _fp,
+                                     C.c_int32, _fp, C.c_int32, C.POINTER(_vp)]),
+    ("tiltgpu_ctf_strip_width_used", C.c_int, [_vp]),
+    ("tiltgpu_ctf_upload", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_ctf_download", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_ctf_run", C.c_int, [_vp]),
+    ("tiltgpu_ctf_last_run", C.c_int, [_vp, _dp, C.POINTER(C.c_int64)]),
+    ("tiltgpu_ctf_destroy", None, [_vp]),
+    ("tiltgpu_ctf_phaseflip", C.c_int, [_fp, _fp, C.c_int32, C.c_int32, C.c_int32, _fp,
+                                        C.c_int32, _fp, C.c_int32, C.POINTER(CtfParams)]),
+    ("tiltgpu_read_defocus_file", C.c_int, [C.c_char_p, _ip, _fp, C.c_int32]),
+    ("tiltgpu_interpolate_defocus", C.c_int, [_ip, _fp, C.c_int32, C.c_int32, _fp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_parse_local_file", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _fp, _fp,
@@ -611,6 +636,130 @@ def fma_calibration(iters=1 << 14):
     t = C.c_double()
     _check(lib().tiltgpu_fma_calibration(iters, C.byref(t)))
     return t.value
+
+
+def read_defocus_file(path):
+    """(entries [n][3] f32, views [n] int) of a ctfphaseflip defocus file."""
+    n = lib().tiltgpu_read_defocus_file(str(path).encode(), None, None, 0)
+    if n <= -100:
+        raise TiltGpuError(ERR_IO, f"{path}: malformed line {-100 - n}")
+    _check(n)
+    views = np.zeros(max(n, 1), np.int32)
+    ent = np.zeros((max(n, 1), 3), np.float32)
+    lib().tiltgpu_read_defocus_file(str(path).encode(), views.ctypes.data_as(_ip), _f(ent), n)
+    return ent[:n], views[:n]
+
+
+def interpolate_defocus(entries, views, n_views):
+    ent = np.ascontiguousarray(np.asarray(entries, np.float32).reshape(-1, 3))
+    vw = np.ascontiguousarray(np.asarray(views, np.int32))
+    out = np.zeros((max(n_views, 1), 3), np.float32)
+    n = _check(lib().tiltgpu_interpolate_defocus(vw.ctypes.data_as(_ip), _f(ent), vw.size,
+                                                  int(n_views), _f(out)))
+    return out[:n]
+
+
+def ctf_params(pixel_a=1.0, voltage_kv=300.0, cs_mm=2.7, defocus_nm=-3000.0, strip_width=256,
+               amp=0.07, cuton=0.0, plate_deg=0.0, tilt_axis_deg=0.0, max_strip_width=0):
+    """tiltgpu_ctf_params with ctfphaseflip's option names (main.rs:20-89);
+    max_strip_width 0 selects the automatic limit."""
+    p = CtfParams()
+    lib().tiltgpu_ctf_default_params(C.byref(p))
+    p.pixel_a, p.voltage_kv, p.cs_mm, p.defocus_nm = pixel_a, voltage_kv, cs_mm, defocus_nm
+    p.strip_width, p.amplitude_contrast, p.cuton_freq = int(strip_width), amp, cuton
+    p.plate_phase_deg, p.tilt_axis_deg, p.max_strip_width = plate_deg, tilt_axis_deg, int(max_strip_width)
+    return p
+
+
+def electron_wavelength(voltage_kv):
+    return float(lib().tiltgpu_electron_wavelength(float(voltage_kv)))
+
+
+def _vdef(vdef):
+    if vdef is None:
+        return None, 0
+    v = np.ascontiguousarray(np.asarray(vdef, np.float32).reshape(-1, 3))
+    return v, v.shape[0]
+
+
+def ctf_strip_width(params: CtfParams, nx, vdef=None):
+    v, n = _vdef(vdef)
+    return _check(lib().tiltgpu_ctf_strip_width(C.byref(params), int(nx), _f(v), n))
+
+
+def ctf_strip_layout(nx, ny, tilt_axis_deg, strip_width):
+    """(boxes [n][4] = bx0, bx1, by0, by1; centres [n]) of the strips."""
+    n = _check(lib().tiltgpu_ctf_strip_layout(int(nx), int(ny), float(tilt_axis_deg),
+                                              int(strip_width), None, None, 0))
+    boxes = np.zeros((max(n, 1), 4), np.int32)
+    cen = np.zeros(max(n, 1), np.float64)
+    _check(lib().tiltgpu_ctf_strip_layout(int(nx), int(ny), float(tilt_axis_deg), int(strip_width),
+                                          boxes.ctypes.data_as(_ip), cen.ctypes.data_as(_dp), n))
+    return boxes[:n], cen[:n]
+
+
+class CtfCorrector:
+    """A device-resident stack for the CTF phase flip (tiltgpu_ctf_*)."""
+
+    def __init__(self, nx, ny, nz, params: CtfParams, tilt_deg=None, vdef=None):
+        self._h = _vp()
+        t = _f32(tilt_deg)
+        v, n = _vdef(vdef)
+        self.shape = (int(nz), int(ny), int(nx))
+        _check(lib().tiltgpu_ctf_create(int(nx), int(ny), int(nz), C.byref(params), _f(t),
+                                        0 if t is None else t.size, _f(v), n, C.byref(self._h)))
+
+    @property
+    def strip_width(self):
+        return _check(lib().tiltgpu_ctf_strip_width_used(self._h))
+
+    def upload(self, P, z0=0):
+        a = _f32(P)
+        nsec = a.shape[0] if a.ndim == 3 else 1
+        _check(lib().tiltgpu_ctf_upload(self._h, int(z0), nsec, _f(a)))
+
+    def run(self):
+        _check(lib().tiltgpu_ctf_run(self._h))
+
+    def download(self, z0=0, nsec=None):
+        nz, ny, nx = self.shape
+        nsec = nz - z0 if nsec is None else nsec
+        out = np.empty((nsec, ny, nx), np.float32)
+        _check(lib().tiltgpu_ctf_download(self._h, int(z0), int(nsec), _f(out)))
+        return out
+
+    def last_run(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(lib().tiltgpu_ctf_last_run(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self._h:
+            lib().tiltgpu_ctf_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ctf_phaseflip(P, tilt_deg, pixel_a, voltage_kv=300.0, cs_mm=2.7, defocus_nm=-3000.0,
+                  vdef=None, strip_width=256, amp=0.07, cuton=0.0, plate_deg=0.0,
+                  tilt_axis_deg=0.0, max_strip_width=0):
+    """CTF phase flip of a host stack [nz][ny][nx] on the GPU (one call)."""
+    a = _f32(P)
+    nz, ny, nx = a.shape
+    p = ctf_params(pixel_a, voltage_kv, cs_mm, defocus_nm, strip_width, amp, cuton, plate_deg,
+                   tilt_axis_deg, max_strip_width)
+    t = _f32(tilt_deg)
+    v, n = _vdef(vdef)
+    out = np.empty_like(a)
+    _check(lib().tiltgpu_ctf_phaseflip(_f(a), _f(out), nx, ny, nz, _f(t),
+                                       0 if t is None else t.size, _f(v), n, C.byref(p)))
+    return out
 
 
 class Server:

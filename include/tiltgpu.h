@@ -314,6 +314,60 @@ int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy);
  * from `iters` iterations of eight independent FMA chains per lane; TFLOP/s. */
 int tiltgpu_fma_calibration(int32_t iters, double* tflops);
 
+/* ------------------------------------------------------------ ctfphaseflip
+ * CTF phase flip of a tilt series by 2D strips: the CPU path of
+ * apps/ctfphaseflip/src/main.rs:353-705 (and, for the flip itself, the work of
+ * GpuCtfSession::correct_strip, apps/ctfphaseflip/src/gpu.rs:107), the whole
+ * stack on the GPU: per strip the bounding box's 2D transform, the sign flip
+ * where the CTF is negative (decided in double precision like the CPU path),
+ * the inverse transform and the triangular-weight blend. */
+typedef struct {
+    float voltage_kv;          /* -v (default 300)                                   */
+    float cs_mm;               /* -c (default 2.7)                                   */
+    float pixel_a;             /* -p; the CLI defaults it to the header's xlen / mx  */
+    float amplitude_contrast;  /* --amplitude-contrast (default 0.07)                */
+    float cuton_freq;          /* --cuton-freq, 1/Angstrom (default 0)               */
+    float plate_phase_deg;     /* --plate-phase (default 0)                          */
+    float tilt_axis_deg;       /* --tilt-axis-angle (default 0)                      */
+    float defocus_nm;          /* -d, used where no per-view defocus (default -3000) */
+    int32_t strip_width;       /* -w, requested width (default 256)                  */
+    int32_t max_strip_width;   /* --max-strip-width; 0 = automatic (main.rs:272-298)  */
+} tiltgpu_ctf_params;
+/* Args defaults, apps/ctfphaseflip/src/main.rs:20-89 (pixel_a = 1). */
+void tiltgpu_ctf_default_params(tiltgpu_ctf_params* p);
+/* electron_wavelength (main.rs:709-719), Angstrom. */
+double tiltgpu_electron_wavelength(float voltage_kv);
+/* The strip width actually used (main.rs:272-298): vdef = the interpolated
+ * per-view defocus table [n_vdef][3] (defocus1 nm, defocus2 nm, astig deg) or
+ * NULL when no defocus file is given.  Returns the width or an error. */
+int tiltgpu_ctf_strip_width(const tiltgpu_ctf_params* p, int32_t nx, const float* vdef,
+                            int32_t n_vdef);
+/* The strips of main.rs:389-487 for an nx x ny image: returns the strip count n;
+ * writes min(n, cap) bounding boxes [bx0, bx1, by0, by1] (empty when bx1 <= bx0)
+ * and strip centres (perpendicular distance from the image centre). */
+int tiltgpu_ctf_strip_layout(int32_t nx, int32_t ny, float tilt_axis_deg, int32_t strip_width,
+                             int32_t* boxes, double* centres, int32_t cap);
+/* A stack of nz sections of ny x nx resident on the device, with tilt angles
+ * (read_tilt_file; missing angles are 0) and the per-view defocus table as for
+ * tiltgpu_ctf_strip_width (sections beyond n_vdef use params->defocus_nm). */
+typedef struct tiltgpu_ctf tiltgpu_ctf;
+int tiltgpu_ctf_create(int32_t nx, int32_t ny, int32_t nz, const tiltgpu_ctf_params* p,
+                       const float* tilt_deg, int32_t n_tilt, const float* vdef, int32_t n_vdef,
+                       tiltgpu_ctf** out);
+int tiltgpu_ctf_strip_width_used(const tiltgpu_ctf* c);
+/* Sections [z0, z0 + nsec) from / to host memory ([nsec][ny][nx] floats). */
+int tiltgpu_ctf_upload(tiltgpu_ctf* c, int32_t z0, int32_t nsec, const float* host);
+int tiltgpu_ctf_download(tiltgpu_ctf* c, int32_t z0, int32_t nsec, float* host);
+/* Corrects the uploaded stack (main.rs:353-691 for every section); blocks. */
+int tiltgpu_ctf_run(tiltgpu_ctf* c);
+/* Device time (ms) and kernel launches of the last tiltgpu_ctf_run. */
+int tiltgpu_ctf_last_run(const tiltgpu_ctf* c, double* ms, int64_t* launches);
+void tiltgpu_ctf_destroy(tiltgpu_ctf* c);
+/* create + upload + run + download + destroy: host stack in, host stack out. */
+int tiltgpu_ctf_phaseflip(const float* in, float* out, int32_t nx, int32_t ny, int32_t nz,
+                          const float* tilt_deg, int32_t n_tilt, const float* vdef,
+                          int32_t n_vdef, const tiltgpu_ctf_params* p);
+
 /* -------------------------------------------------------- host utilities */
 
 /* read_tilt_file (crates/imod-transforms/src/tilt.rs:6-23).  Writes at most cap
@@ -325,6 +379,16 @@ int tiltgpu_read_per_view_file(const char* path, float* out, int32_t cap);
  * file as [n][6] (a11 a12 a21 a22 dx dy) into out (up to cap transforms);
  * returns n or TILTGPU_ERR_IO. */
 int tiltgpu_read_xf_file(const char* path, float* out, int32_t cap);
+
+/* read_defocus_file (apps/ctfphaseflip/src/main.rs:103-157): entries of a
+ * defocus file, views[k] and entries[k][3] = defocus1 nm, defocus2 nm, astig deg
+ * (up to cap); returns the count, TILTGPU_ERR_IO when unreadable, or
+ * -100 - line for the first malformed line. */
+int tiltgpu_read_defocus_file(const char* path, int32_t* views, float* entries, int32_t cap);
+/* interpolate_defocus (main.rs:164-198) into out [n_views][3]; returns n_views,
+ * or 0 when n == 0 (no per-view table). */
+int tiltgpu_interpolate_defocus(const int32_t* views, const float* entries, int32_t n,
+                                int32_t n_views, float* out);
 
 /* --local-file (load_local_alignment, main.rs:232-307) parsed into per-view
  * grids for tiltgpu_set_local_alignment.  sizes[3] receives the total lengths of

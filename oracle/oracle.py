@@ -189,8 +189,9 @@ def ctf_strip_width(strip_width, nx, pixel_a, wavelength, defocus_nm, vdef=None,
     if max_strip_width is not None:
         max_w = int(max_strip_width)
     else:
-        if vdef is not None and len(vdef):
-            mx = max(max(abs(float(a)), abs(float(b))) * 10.0 for a, b, _ in vdef)
+        if vdef is not None:  # a defocus file, possibly without entries (fold from 0)
+            mx = max([max(abs(float(np.float32(a))), abs(float(np.float32(b)))) * 10.0
+                      for a, b, _ in vdef], default=0.0)
         else:
             mx = abs(float(np.float32(defocus_nm))) * 10.0
         if mx > 1.0 and wavelength > 0.0:
@@ -213,7 +214,7 @@ def read_defocus_file(path):
             f = t.split()
             if len(f) < 2:
                 raise ValueError(f"line {ln}: expected at least 'view defocus1'")
-            if not f[0].isdigit():
+            if _rust_parse_usize(f[0]) is None:
                 raise ValueError(f"line {ln}: invalid view number {f[0]!r}")
             d1 = _rust_parse_f32(f[1])
             if d1 is None:
@@ -227,7 +228,7 @@ def read_defocus_file(path):
                 aa = _rust_parse_f32(f[3])
                 if aa is None:
                     raise ValueError(f"line {ln}: invalid astig_angle {f[3]!r}")
-            views.append(int(f[0]))
+            views.append(_rust_parse_usize(f[0]))
             entries.append((np.float32(d1), np.float32(d2), np.float32(aa)))
     return entries, views
 
@@ -315,9 +316,11 @@ def _rust_parse_f32(t):
 
 
 def _rust_parse_usize(t):
-    if not t or not t.lstrip("+").isdigit():
+    """str::parse::<usize>: ASCII digits after at most one leading '+'."""
+    d = t[1:] if t.startswith("+") else t
+    if not d or not all("0" <= ch <= "9" for ch in d):
         return None
-    return int(t)
+    return int(d)
 
 
 @dataclass
