@@ -187,6 +187,153 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_rows_inv_kernel(CtfStrip s, F
     }
 }
 
+// ---- power-of-two lengths 64 .. 2048: the register radix-8 passes, 2048 / N
+// rows per workgroup (N / 8 threads per row).
+
+// Stage 1 for rows of length N: two real rows packed as one complex row
+// z = a + i b, one transform, the spectra separated as
+// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
+template <int N>
+__global__ void __launch_bounds__(kCtfThreads) ctf_rows_fwd_p2(CtfStrip s, const float2* tw8) {
+    extern __shared__ float2 sm[];
+    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
+    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int rows = s.zc * s.sh;
+    const int pair0 = blockIdx.x * PPW;
+    const int pairs = min(PPW, (rows + 1) / 2 - pair0);
+    float2 twr[P2<N>::tw_total()];
+    p2_load_tw<N, 0>(twr, j, tw8);
+    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
+        const int qq = idx / N, x = idx & (N - 1);
+        float v[2] = {0.f, 0.f};
+        if (qq < pairs) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int r = 2 * (pair0 + qq) + c;
+                if (r < rows) {
+                    const int z = static_cast<int>(s.sh_div.div(static_cast<uint32_t>(r)));
+                    const int y = r - z * s.sh;
+                    v[c] = s.in[z * s.plane + static_cast<long>(s.by0 + y) * s.nx + s.bx0 + x];
+                }
+            }
+        }
+        sm[qq * RS + p2_slot(x)] = make_float2(v[0], v[1]);
+    }
+    __syncthreads();
+    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < pairs * N; idx += kCtfThreads) {
+        const int qq = idx / N, k = idx & (N - 1);
+        const float2* row = sm + qq * RS;
+        const float2 X = row[p2_slot(k)], Y = row[p2_slot((N - k) & (N - 1))];
+        const int ra = 2 * (pair0 + qq);
+        s.a[static_cast<long>(ra) * N + k] = make_float2(0.5f * (X.x + Y.x), 0.5f * (X.y - Y.y));
+        if (ra + 1 < rows)
+            s.a[static_cast<long>(ra + 1) * N + k] =
+                make_float2(0.5f * (X.y + Y.y), 0.5f * (Y.x - X.x));
+    }
+}
+
+// Stage 3 for columns of length N.
+template <int N>
+__global__ void __launch_bounds__(kCtfThreads) ctf_cols_p2(CtfStrip s, const float2* tw8) {
+    extern __shared__ float2 sm[];
+    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
+    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int rows = s.zc * s.sw;
+    const int r0 = blockIdx.x * PPW;
+    const int cnt = min(PPW, rows - r0);
+    float2 twr[P2<N>::tw_total()];
+    p2_load_tw<N, 0>(twr, j, tw8);
+    float2* base = s.b + static_cast<long>(r0) * N;
+    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
+        const int qq = idx / N, x = idx & (N - 1);
+        sm[qq * RS + p2_slot(x)] = qq < cnt ? base[idx] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < cnt * N; idx += kCtfThreads) {
+        const int qq = idx / N, fy = idx & (N - 1);
+        const int r = r0 + qq;
+        const int z = static_cast<int>(s.sw_div.div(static_cast<uint32_t>(r)));
+        const int fx = r - z * s.sw;
+        float2& e = sm[qq * RS + p2_slot(fy)];
+        float2 v = e;
+        if (ctf_negative(s, fx, fy, z)) {
+            v.x = -v.x;
+            v.y = -v.y;
+        }
+        e = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < cnt * N; idx += kCtfThreads) {
+        const int qq = idx / N, x = idx & (N - 1);
+        const float2 v = sm[qq * RS + p2_slot(x)];
+        base[idx] = make_float2(v.x, -v.y);
+    }
+}
+
+// Stage 5 for rows of length N, two rows per transform.  Only the real part of
+// each row's inverse is kept, and Re(IDFT(A)) = IDFT(H) with H the Hermitian
+// part H[k] = (A[k] + conj A[N-k]) / 2 (the flipped spectrum is not Hermitian
+// where the CTF differs between +-f on the Nyquist lines, e.g. with a rotated
+// tilt axis or astigmatism).  Z = H1 + i H2 then gives row 1 in the real part
+// and row 2 in the imaginary part of conj(DFT(conj Z)).
+template <int N>
+__global__ void __launch_bounds__(kCtfThreads) ctf_rows_inv_p2(CtfStrip s, const float2* tw8) {
+    extern __shared__ float2 sm[];
+    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
+    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int rows = s.zc * s.sh;
+    const int pair0 = blockIdx.x * PPW;
+    const int pairs = min(PPW, (rows + 1) / 2 - pair0);
+    float2 twr[P2<N>::tw_total()];
+    p2_load_tw<N, 0>(twr, j, tw8);
+    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
+        const int qq = idx / N, x = idx & (N - 1);
+        float2 z = make_float2(0.f, 0.f);
+        if (qq < pairs) {
+            const int ra = 2 * (pair0 + qq);
+            const int xm = (N - x) & (N - 1);
+            const float2* r1 = s.a + static_cast<long>(ra) * N;
+            const float2 a1 = r1[x], b1 = r1[xm];
+            float2 a2 = make_float2(0.f, 0.f), b2 = a2;
+            if (ra + 1 < rows) {
+                a2 = r1[N + x];
+                b2 = r1[N + xm];
+            }
+            const float2 h1 = make_float2(0.5f * (a1.x + b1.x), 0.5f * (a1.y - b1.y));
+            const float2 h2 = make_float2(0.5f * (a2.x + b2.x), 0.5f * (a2.y - b2.y));
+            z = make_float2(h1.x - h2.y, -(h1.y + h2.x));
+        }
+        sm[qq * RS + p2_slot(x)] = z;
+    }
+    __syncthreads();
+    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < pairs * N; idx += kCtfThreads) {
+        const int qq = idx / N, x = idx & (N - 1);
+        const float2 w = sm[qq * RS + p2_slot(x)];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int r = 2 * (pair0 + qq) + c;
+            if (r >= rows) break;
+            const int z = static_cast<int>(s.sh_div.div(static_cast<uint32_t>(r)));
+            const int gx = s.bx0 + x, gy = s.by0 + r - z * s.sh;
+            const double d = (static_cast<double>(gx) - s.cx) * s.px +
+                             (static_cast<double>(gy) - s.cy) * s.py;
+            if (d >= s.lo && d < s.hi) {
+                const float dist = static_cast<float>(fabs((d - s.scp) / s.half));
+                const float wt0 = 1.0f - dist;
+                const float wt = wt0 > 0.0f ? wt0 : 0.0f;
+                const float val = (c ? -w.y : w.x) * s.scale;
+                const long o = z * s.plane + static_cast<long>(gy) * s.nx + gx;
+                s.wsum[o] += val * wt;
+                s.wtot[o] += wt;
+            }
+        }
+    }
+}
+
 // src [batch][rows][cols] -> dst [batch][cols][rows] through a 32 x 32 LDS tile.
 __global__ void __launch_bounds__(256) ctf_transpose_kernel(const float2* __restrict__ src,
                                                             float2* __restrict__ dst, int rows,
@@ -317,6 +464,7 @@ using tiltgpu_sess::fail;
 struct CtfPlan {
     FilterParams p{};
     float2* tw = nullptr;
+    float2* tw8 = nullptr;  // power-of-two plan table (64 <= n <= 2048), else null
 };
 
 struct tiltgpu_ctf {
@@ -343,7 +491,10 @@ struct tiltgpu_ctf {
 namespace {
 
 void ctf_release(tiltgpu_ctf* c) {
-    for (auto& kv : c->plans) (void)hipFree(kv.second.tw);
+    for (auto& kv : c->plans) {
+        (void)hipFree(kv.second.tw);
+        if (kv.second.tw8) (void)hipFree(kv.second.tw8);
+    }
     void* bufs[] = {c->in, c->out, c->wtot, c->a, c->b, c->def, c->astig};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -395,8 +546,51 @@ int ctf_plan(tiltgpu_ctf* c, int n) {
     if (rc) return rc;
     HIP_TRY(hipMemcpy(pl.tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
     f.tw = pl.tw;
+    if (const int m = pow2_twiddles(n, nullptr)) {
+        std::vector<float2> t8(m);
+        pow2_twiddles(n, t8.data());
+        if ((rc = ctf_alloc(&pl.tw8, m))) {
+            (void)hipFree(pl.tw);
+            return rc;
+        }
+        HIP_TRY(hipMemcpy(pl.tw8, t8.data(), m * sizeof(float2), hipMemcpyHostToDevice));
+    }
     c->plans[n] = pl;
     return TILTGPU_OK;
+}
+
+using P2Kernel = void (*)(CtfStrip, const float2*);
+
+// The power-of-two kernel of stage 1 (0), 3 (1) or 5 (2) for length n.
+P2Kernel p2_kernel(int stage, int n) {
+#define TILTGPU_CTF_P2(N)                                                            \
+    case N:                                                                          \
+        return stage == 0 ? ctf_rows_fwd_p2<N> : stage == 1 ? ctf_cols_p2<N> : ctf_rows_inv_p2<N>;
+    switch (n) {
+        TILTGPU_CTF_P2(64)
+        TILTGPU_CTF_P2(128)
+        TILTGPU_CTF_P2(256)
+        TILTGPU_CTF_P2(512)
+        TILTGPU_CTF_P2(1024)
+        TILTGPU_CTF_P2(2048)
+        default:
+            return nullptr;
+    }
+#undef TILTGPU_CTF_P2
+}
+
+// Stage `stage` over `items` rows (stage 0 and 2: real rows, two per
+// transform) with the power-of-two kernels.
+hipError_t launch_p2(int stage, const CtfStrip& s, const CtfPlan& pl, int items, hipStream_t st) {
+    const int n = pl.p.n, ppw = 2048 / n;
+    const int units = stage == 1 ? items : (items + 1) / 2;
+    const int nblk = (units + ppw - 1) / ppw;
+    CtfStrip a = s;
+    const float2* t8 = pl.tw8;
+    void* args[] = {&a, &t8};
+    const size_t lds = static_cast<size_t>(ppw) * p2_row_slots(n) * sizeof(float2);
+    return hipLaunchKernel(reinterpret_cast<const void*>(p2_kernel(stage, n)), dim3(nblk),
+                           dim3(kCtfThreads), args, lds, st);
 }
 
 size_t plan_lds(const FilterParams& f) {
@@ -678,13 +872,22 @@ int tiltgpu_ctf_run(tiltgpu_ctf* c) {
             s.den_y = static_cast<double>(s.sh) * pixel;
             s.scale = 1.0f / static_cast<float>(s.sw * s.sh);
             s.def = c->def + si * c->nz + z0;
-            const FilterParams& fx = c->plans.at(s.sw).p;
-            const FilterParams& fy = c->plans.at(s.sh).p;
-            HIP_TRY(launch_rows(ctf_rows_fwd_kernel, s, fx, zc * s.sh, st));
+            const CtfPlan& px = c->plans.at(s.sw);
+            const CtfPlan& py = c->plans.at(s.sh);
+            if (px.tw8)
+                HIP_TRY(launch_p2(0, s, px, zc * s.sh, st));
+            else
+                HIP_TRY(launch_rows(ctf_rows_fwd_kernel, s, px.p, zc * s.sh, st));
             HIP_TRY(launch_transpose(s.a, s.b, s.sh, s.sw, zc, st));
-            HIP_TRY(launch_rows(ctf_cols_kernel, s, fy, zc * s.sw, st));
+            if (py.tw8)
+                HIP_TRY(launch_p2(1, s, py, zc * s.sw, st));
+            else
+                HIP_TRY(launch_rows(ctf_cols_kernel, s, py.p, zc * s.sw, st));
             HIP_TRY(launch_transpose(s.b, s.a, s.sw, s.sh, zc, st));
-            HIP_TRY(launch_rows(ctf_rows_inv_kernel, s, fx, zc * s.sh, st));
+            if (px.tw8)
+                HIP_TRY(launch_p2(2, s, px, zc * s.sh, st));
+            else
+                HIP_TRY(launch_rows(ctf_rows_inv_kernel, s, px.p, zc * s.sh, st));
             launches += 5;
         }
         const long n = plane * zc;

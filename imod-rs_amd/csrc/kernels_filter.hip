@@ -82,160 +82,10 @@ __device__ float stretch_sample(const float2* row, int c, int n, float s, int cu
 
 // ---------------------------------------------------------------- power of two
 //
-// n = 2^L, 64 <= n <= 2048: every thread holds 8 complex values of one row pair
-// (n/8 threads per pair, 2048/n pairs per workgroup).  Radix-8 Stockham passes
-// with the butterfly in registers, then one radix-4 or radix-2 pass; sizes are
-// compile-time, so indices are shifts and masks.  The weight multiply is fused
-// into the last forward pass and the 1/n scale into the last inverse pass.
-
-// DFT8 by decimation in frequency: b_m = x_m + x_{m+4}, c_m = (x_m - x_{m+4}) W8^m,
-// X[2m] = DFT4(b)[m], X[2m+1] = DFT4(c)[m].
-__device__ __forceinline__ void dft8(float2 (&x)[8]) {
-    const float h = 0.70710678118654752440f;  // 1/sqrt(2)
-    float2 b[4], c[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        b[m] = cadd(x[m], x[m + 4]);
-        c[m] = csub(x[m], x[m + 4]);
-    }
-    // W8 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = -(1 + i)/sqrt2
-    c[1] = make_float2((c[1].x + c[1].y) * h, (c[1].y - c[1].x) * h);
-    c[2] = cmul_mi(c[2]);
-    c[3] = make_float2((c[3].y - c[3].x) * h, -(c[3].x + c[3].y) * h);
-    dft_small<4>(b);
-    dft_small<4>(c);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        x[2 * m] = b[m];
-        x[2 * m + 1] = c[m];
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void dft_r(float2* x) {
-    if constexpr (R == 8)
-        dft8(*reinterpret_cast<float2(*)[8]>(x));
-    else
-        dft_small<R>(x);
-}
-
-// Fused operation on a pass's outputs before they are stored.
-enum { kP2None = 0, kP2Weight = 1, kP2Scale = 2 };
-
-// LDS slot of element i: one pad slot every 8 elements, which spreads the
-// strided stores of the first passes over the banks.
-__device__ __forceinline__ int p2_slot(int i) { return i + (i >> 3); }
-__host__ __device__ constexpr int p2_row_slots(int n) { return n + n / 8; }
-
-// Radix, span and per-thread butterflies of pass S, and the thread's twiddle
-// count before pass S (compile time).
-template <int N>
-struct P2 {
-    static constexpr int T = N / 8;
-    static constexpr int radix(int s) {
-        int ns = 1;
-        for (int i = 0; i < s; ++i) ns *= (N / ns >= 8 ? 8 : N / ns);
-        return N / ns >= 8 ? 8 : N / ns;
-    }
-    static constexpr int span(int s) {
-        int ns = 1;
-        for (int i = 0; i < s; ++i) ns *= radix(i);
-        return ns;
-    }
-    static constexpr int passes() {
-        int s = 0;
-        while (span(s) < N) ++s;
-        return s;
-    }
-    static constexpr int per(int s) { return (N / radix(s)) / T; }
-    static constexpr int tw_regs(int s) { return span(s) > 1 ? per(s) * (radix(s) - 1) : 0; }
-    static constexpr int tw_before(int s) {
-        int c = 0;
-        for (int i = 0; i < s; ++i) c += tw_regs(i);
-        return c;
-    }
-    static constexpr int tw_total() { return tw_before(passes()); }
-    static constexpr int tab_off(int s) {
-        int c = 0;
-        for (int i = 0; i < s; ++i) c += span(i) * radix(i);
-        return c;
-    }
-};
-
-// The thread's twiddles of every pass, loaded once (the same for the forward
-// and the inverse transform).
-template <int N, int S>
-__device__ __forceinline__ void p2_load_tw(float2* twr, int j, const float2* __restrict__ tw8) {
-    if constexpr (S < P2<N>::passes()) {
-        constexpr int R = P2<N>::radix(S), NS = P2<N>::span(S), PER = P2<N>::per(S);
-        if constexpr (NS > 1) {
-            float2* w = twr + P2<N>::tw_before(S);
-#pragma unroll
-            for (int u = 0; u < PER; ++u) {
-                const int k = (j + u * P2<N>::T) & (NS - 1);
-#pragma unroll
-                for (int r = 1; r < R; ++r)
-                    w[u * (R - 1) + r - 1] = tw8[P2<N>::tab_off(S) + k * R + r];
-            }
-        }
-        p2_load_tw<N, S + 1>(twr, j, tw8);
-    }
-}
-
-// One in-place Stockham pass of radix R and span NS over a pair's padded row:
-// thread j does the butterflies b = j + u*T (T = N/8, u < N/(R*T)); inputs
-// b + r*N/R, outputs (b - k)*R + k + r*NS with k = b mod NS.  All reads finish
-// (barrier) before any write.
-template <int N, int S, int FUSE>
-__device__ __forceinline__ void p2_pass(float2* row, int j, const float2* twr,
-                                        const float* __restrict__ weights, float inv_n) {
-    constexpr int R = P2<N>::radix(S), NS = P2<N>::span(S), PER = P2<N>::per(S);
-    constexpr int T = P2<N>::T, BF = N / R;
-    float2 x[PER][R];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int b = j + u * T;
-#pragma unroll
-        for (int r = 0; r < R; ++r) x[u][r] = row[p2_slot(b + r * BF)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int b = j + u * T;
-        const int k = b & (NS - 1);
-        if constexpr (NS > 1) {
-            const float2* w = twr + P2<N>::tw_before(S) + u * (R - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) x[u][r] = cmul(x[u][r], w[r - 1]);
-        }
-        dft_r<R>(x[u]);
-        const int base = (b - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float2 v = x[u][r];
-            if constexpr (FUSE == kP2Weight) {
-                // Z' = conj(W * X), the input of the inverse transform
-                const float wt = weights[base + r * NS];
-                v = make_float2(v.x * wt, -(v.y * wt));
-            } else if constexpr (FUSE == kP2Scale) {
-                // a = Re(Y) / n, b = -Im(Y) / n  (main.rs:376-379: re * (1/n))
-                v = make_float2(v.x * inv_n, -v.y * inv_n);
-            }
-            row[p2_slot(base + r * NS)] = v;
-        }
-    }
-    __syncthreads();
-}
-
-template <int N, int S, int FUSE>
-__device__ __forceinline__ void p2_fft(float2* row, int j, const float2* twr,
-                                       const float* __restrict__ weights, float inv_n) {
-    if constexpr (S < P2<N>::passes()) {
-        constexpr bool LAST = S + 1 == P2<N>::passes();
-        p2_pass<N, S, LAST ? FUSE : kP2None>(row, j, twr, weights, inv_n);
-        p2_fft<N, S + 1, FUSE>(row, j, twr, weights, inv_n);
-    }
-}
+// n = 2^L, 64 <= n <= 2048: the register radix-8 passes of fft_lds.h over row
+// pairs (n/8 threads per pair, 2048/n pairs per workgroup).  The weight
+// multiply is fused into the last forward pass and the 1/n scale into the last
+// inverse pass.
 
 __device__ __forceinline__ float p2_at(const float2* row, int i, int c) {
     const float2 v = row[p2_slot(i)];

@@ -109,6 +109,19 @@ CASES = [
     (2, 96, 128, dict(pixel_a=6.0, voltage_kv=200.0, cs_mm=2.0, tilt_axis_deg=5.5)),
     (2, 256, 256, dict(pixel_a=2.0, max_strip_width=64, strip_width=64, tilt_axis_deg=-12.0)),
     (2, 5000, 20, dict(pixel_a=3.0)),  # columns of 5000: 80 KB of LDS per workgroup
+    # power-of-two rows / columns (register radix-8 kernels), odd row counts
+    (2, 256, 192, dict(pixel_a=3.0, max_strip_width=64, strip_width=64)),
+    (3, 45, 128, dict(pixel_a=3.0, max_strip_width=64, strip_width=64)),
+    (3, 128, 128, dict(pixel_a=2.0, tilt_axis_deg=90.0, max_strip_width=64, strip_width=64,
+                       vdef=[(-2000.0, -2500.0, 30.0)] * 3)),
+    (1, 2048, 32, dict(pixel_a=4.0)),
+    (1, 40, 2048, dict(pixel_a=4.0, tilt_axis_deg=90.0)),
+    (2, 96, 1024, dict(pixel_a=4.0, tilt_axis_deg=85.0)),
+    # rotated axis / astigmatism with power-of-two rows: the Nyquist column of
+    # the flipped spectrum is not Hermitian, the inverse keeps the real part
+    (3, 128, 128, dict(pixel_a=2.0, tilt_axis_deg=89.0, max_strip_width=64, strip_width=64)),
+    (3, 128, 128, dict(pixel_a=2.0, max_strip_width=64, strip_width=64,
+                       vdef=[(-2000.0, -2500.0, 30.0)] * 3)),
 ]
 
 

@@ -73,7 +73,7 @@ def test_wbp_options(gpu, rel, opts):
     assert rel(g, o) <= TOL, opts
 
 
-@pytest.mark.parametrize("nx", [96, 120, 160, 250, 768, 131, 194, 259, 1958])
+@pytest.mark.parametrize("nx", [96, 120, 160, 250, 768, 131, 194, 259, 1958, 182, 195, 1001, 154])
 def test_wbp_mixed_radix_widths(gpu, rel, nx):
     P, ang = case(gpu, nx=nx, ny=6, nviews=21, thick=32, seed=nx)
     g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=32))
