@@ -25,6 +25,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -37,7 +38,7 @@ namespace tiltgpu {
 namespace {
 
 constexpr int kCtfThreads = 256;
-constexpr int kCtfMaxSide = 8192;            // 2 * 8192 * 8 B = 128 KiB of LDS per row pair
+constexpr size_t kCtfMaxLds = 160 * 1024;    // LDS per workgroup on gfx950
 constexpr long kCtfChunkElems = 64L << 20;   // complex elements per work buffer
 constexpr double kPi = 3.14159265358979323846;
 
@@ -187,26 +188,85 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_rows_inv_kernel(CtfStrip s, F
     }
 }
 
-// ---- power-of-two lengths 64 .. 2048: the register radix-8 passes, 2048 / N
-// rows per workgroup (N / 8 threads per row).
+// ---- register transforms: rows of length n held in LDS rows of M = 2^L
+// (64 <= M <= 4096) slots, max(1, 2048 / M) rows per workgroup, M / 8 threads
+// per row.  n == M: the radix-8 passes directly.  n < M (Bluestein): the
+// length-n DFT as the circular convolution of length M >= 2n - 1
+//   X[k] = conj(w_k) * sum_j (x_j conj(w_j)) w_{k-j},  w_m = exp(i pi m^2 / n),
+// i.e. two length-M transforms around the product with B = DFT_M(w) / M.
+struct RegPlan {
+    const float2* tw8;    // per-pass twiddles of the length-M transform
+    const float2* chirp;  // [n] w_m, or null when n == M
+    const float2* bspec;  // [M] DFT_M of the zero-padded symmetric chirp, / M
+    int n;
+};
 
-// Stage 1 for rows of length N: two real rows packed as one complex row
-// z = a + i b, one transform, the spectra separated as
-// A[k] = (Z[k] + conj Z[N-k]) / 2, B[k] = (Z[k] - conj Z[N-k]) / (2i).
-template <int N>
-__global__ void __launch_bounds__(kCtfThreads) ctf_rows_fwd_p2(CtfStrip s, const float2* tw8) {
+template <int M>
+struct RegShape {
+    static constexpr int T = M / 8;                      // threads per row
+    static constexpr int PPW = M >= 2048 ? 1 : 2048 / M;  // rows per workgroup
+    static constexpr int TH = T * PPW;                    // threads per workgroup
+    static constexpr int RS = p2_row_slots(M);            // LDS slots per row
+};
+
+__device__ __forceinline__ float2 cmul_conj(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
+}
+
+// Forward DFT of length R.n of every row of the workgroup (element i of row q
+// at sm[q * RS + p2_slot(i)], i < n); all threads call it.
+template <int M>
+__device__ __forceinline__ void reg_dft(float2* sm, const RegPlan& R, const float2* twr) {
+    using S = RegShape<M>;
+    const int tid = threadIdx.x, q = tid / S::T, j = tid & (S::T - 1);
+    if (!R.chirp) {
+        p2_fft<M, 0, kP2None>(sm + q * S::RS, j, twr, nullptr, 0.f);
+        return;
+    }
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, i = idx & (M - 1);
+        float2& e = sm[qq * S::RS + p2_slot(i)];
+        e = i < R.n ? cmul_conj(e, R.chirp[i]) : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    p2_fft<M, 0, kP2None>(sm + q * S::RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, i = idx & (M - 1);
+        float2& e = sm[qq * S::RS + p2_slot(i)];
+        const float2 v = cmul(e, R.bspec[i]);
+        e = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    p2_fft<M, 0, kP2None>(sm + q * S::RS, j, twr, nullptr, 0.f);
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, i = idx & (M - 1);
+        if (i < R.n) {
+            float2& e = sm[qq * S::RS + p2_slot(i)];
+            const float2 v = cmul(e, R.chirp[i]);
+            e = make_float2(v.x, -v.y);
+        }
+    }
+    __syncthreads();
+}
+
+// Stage 1: two real rows packed as one complex row z = a + i b, one
+// transform, the spectra separated as A[k] = (Z[k] + conj Z[n-k]) / 2,
+// B[k] = (Z[k] - conj Z[n-k]) / (2i).
+template <int M>
+__global__ void __launch_bounds__(RegShape<M>::TH) ctf_rows_fwd_reg(CtfStrip s, RegPlan R) {
+    using S = RegShape<M>;
     extern __shared__ float2 sm[];
-    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
-    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int tid = threadIdx.x, j = tid & (S::T - 1);
+    const int n = R.n;
     const int rows = s.zc * s.sh;
-    const int pair0 = blockIdx.x * PPW;
-    const int pairs = min(PPW, (rows + 1) / 2 - pair0);
-    float2 twr[P2<N>::tw_total()];
-    p2_load_tw<N, 0>(twr, j, tw8);
-    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
-        const int qq = idx / N, x = idx & (N - 1);
+    const int pair0 = blockIdx.x * S::PPW;
+    const int pairs = min(S::PPW, (rows + 1) / 2 - pair0);
+    float2 twr[P2<M>::tw_total()];
+    p2_load_tw<M, 0>(twr, j, R.tw8);
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, x = idx & (M - 1);
         float v[2] = {0.f, 0.f};
-        if (qq < pairs) {
+        if (qq < pairs && x < n) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const int r = 2 * (pair0 + qq) + c;
@@ -217,46 +277,51 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_rows_fwd_p2(CtfStrip s, const
                 }
             }
         }
-        sm[qq * RS + p2_slot(x)] = make_float2(v[0], v[1]);
+        sm[qq * S::RS + p2_slot(x)] = make_float2(v[0], v[1]);
     }
     __syncthreads();
-    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
-    for (int idx = tid; idx < pairs * N; idx += kCtfThreads) {
-        const int qq = idx / N, k = idx & (N - 1);
-        const float2* row = sm + qq * RS;
-        const float2 X = row[p2_slot(k)], Y = row[p2_slot((N - k) & (N - 1))];
+    reg_dft<M>(sm, R, twr);
+    for (int idx = tid; idx < pairs * M; idx += S::TH) {
+        const int qq = idx / M, k = idx & (M - 1);
+        if (k >= n) continue;
+        const float2* row = sm + qq * S::RS;
+        const float2 X = row[p2_slot(k)], Y = row[p2_slot(k == 0 ? 0 : n - k)];
         const int ra = 2 * (pair0 + qq);
-        s.a[static_cast<long>(ra) * N + k] = make_float2(0.5f * (X.x + Y.x), 0.5f * (X.y - Y.y));
+        s.a[static_cast<long>(ra) * n + k] = make_float2(0.5f * (X.x + Y.x), 0.5f * (X.y - Y.y));
         if (ra + 1 < rows)
-            s.a[static_cast<long>(ra + 1) * N + k] =
+            s.a[static_cast<long>(ra + 1) * n + k] =
                 make_float2(0.5f * (X.y + Y.y), 0.5f * (Y.x - X.x));
     }
 }
 
-// Stage 3 for columns of length N.
-template <int N>
-__global__ void __launch_bounds__(kCtfThreads) ctf_cols_p2(CtfStrip s, const float2* tw8) {
+// Stage 3: columns of length n, forward DFT, the flip, inverse as
+// conj(DFT(conj(.))).
+template <int M>
+__global__ void __launch_bounds__(RegShape<M>::TH) ctf_cols_reg(CtfStrip s, RegPlan R) {
+    using S = RegShape<M>;
     extern __shared__ float2 sm[];
-    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
-    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int tid = threadIdx.x, j = tid & (S::T - 1);
+    const int n = R.n;
     const int rows = s.zc * s.sw;
-    const int r0 = blockIdx.x * PPW;
-    const int cnt = min(PPW, rows - r0);
-    float2 twr[P2<N>::tw_total()];
-    p2_load_tw<N, 0>(twr, j, tw8);
-    float2* base = s.b + static_cast<long>(r0) * N;
-    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
-        const int qq = idx / N, x = idx & (N - 1);
-        sm[qq * RS + p2_slot(x)] = qq < cnt ? base[idx] : make_float2(0.f, 0.f);
+    const int r0 = blockIdx.x * S::PPW;
+    const int cnt = min(S::PPW, rows - r0);
+    float2 twr[P2<M>::tw_total()];
+    p2_load_tw<M, 0>(twr, j, R.tw8);
+    float2* base = s.b + static_cast<long>(r0) * n;
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, x = idx & (M - 1);
+        sm[qq * S::RS + p2_slot(x)] =
+            qq < cnt && x < n ? base[qq * n + x] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
-    for (int idx = tid; idx < cnt * N; idx += kCtfThreads) {
-        const int qq = idx / N, fy = idx & (N - 1);
+    reg_dft<M>(sm, R, twr);
+    for (int idx = tid; idx < cnt * M; idx += S::TH) {
+        const int qq = idx / M, fy = idx & (M - 1);
+        if (fy >= n) continue;
         const int r = r0 + qq;
         const int z = static_cast<int>(s.sw_div.div(static_cast<uint32_t>(r)));
         const int fx = r - z * s.sw;
-        float2& e = sm[qq * RS + p2_slot(fy)];
+        float2& e = sm[qq * S::RS + p2_slot(fy)];
         float2 v = e;
         if (ctf_negative(s, fx, fy, z)) {
             v.x = -v.x;
@@ -265,54 +330,57 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_p2(CtfStrip s, const flo
         e = make_float2(v.x, -v.y);
     }
     __syncthreads();
-    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
-    for (int idx = tid; idx < cnt * N; idx += kCtfThreads) {
-        const int qq = idx / N, x = idx & (N - 1);
-        const float2 v = sm[qq * RS + p2_slot(x)];
-        base[idx] = make_float2(v.x, -v.y);
+    reg_dft<M>(sm, R, twr);
+    for (int idx = tid; idx < cnt * M; idx += S::TH) {
+        const int qq = idx / M, x = idx & (M - 1);
+        if (x >= n) continue;
+        const float2 v = sm[qq * S::RS + p2_slot(x)];
+        base[qq * n + x] = make_float2(v.x, -v.y);
     }
 }
 
-// Stage 5 for rows of length N, two rows per transform.  Only the real part of
-// each row's inverse is kept, and Re(IDFT(A)) = IDFT(H) with H the Hermitian
-// part H[k] = (A[k] + conj A[N-k]) / 2 (the flipped spectrum is not Hermitian
-// where the CTF differs between +-f on the Nyquist lines, e.g. with a rotated
-// tilt axis or astigmatism).  Z = H1 + i H2 then gives row 1 in the real part
-// and row 2 in the imaginary part of conj(DFT(conj Z)).
-template <int N>
-__global__ void __launch_bounds__(kCtfThreads) ctf_rows_inv_p2(CtfStrip s, const float2* tw8) {
+// Stage 5, two rows per transform.  Only the real part of each row's inverse
+// is kept, and Re(IDFT(A)) = IDFT(H) with H the Hermitian part
+// H[k] = (A[k] + conj A[n-k]) / 2 (the flipped spectrum is not Hermitian where
+// the CTF differs between +-f on the Nyquist lines, e.g. with a rotated tilt
+// axis or astigmatism).  Z = H1 + i H2 then gives row 1 in the real part and
+// row 2 in the imaginary part of conj(DFT(conj Z)).
+template <int M>
+__global__ void __launch_bounds__(RegShape<M>::TH) ctf_rows_inv_reg(CtfStrip s, RegPlan R) {
+    using S = RegShape<M>;
     extern __shared__ float2 sm[];
-    constexpr int T = N / 8, PPW = 2048 / N, RS = p2_row_slots(N);
-    const int tid = threadIdx.x, q = tid / T, j = tid & (T - 1);
+    const int tid = threadIdx.x, j = tid & (S::T - 1);
+    const int n = R.n;
     const int rows = s.zc * s.sh;
-    const int pair0 = blockIdx.x * PPW;
-    const int pairs = min(PPW, (rows + 1) / 2 - pair0);
-    float2 twr[P2<N>::tw_total()];
-    p2_load_tw<N, 0>(twr, j, tw8);
-    for (int idx = tid; idx < PPW * N; idx += kCtfThreads) {
-        const int qq = idx / N, x = idx & (N - 1);
+    const int pair0 = blockIdx.x * S::PPW;
+    const int pairs = min(S::PPW, (rows + 1) / 2 - pair0);
+    float2 twr[P2<M>::tw_total()];
+    p2_load_tw<M, 0>(twr, j, R.tw8);
+    for (int idx = tid; idx < S::PPW * M; idx += S::TH) {
+        const int qq = idx / M, x = idx & (M - 1);
         float2 z = make_float2(0.f, 0.f);
-        if (qq < pairs) {
+        if (qq < pairs && x < n) {
             const int ra = 2 * (pair0 + qq);
-            const int xm = (N - x) & (N - 1);
-            const float2* r1 = s.a + static_cast<long>(ra) * N;
+            const int xm = x == 0 ? 0 : n - x;
+            const float2* r1 = s.a + static_cast<long>(ra) * n;
             const float2 a1 = r1[x], b1 = r1[xm];
             float2 a2 = make_float2(0.f, 0.f), b2 = a2;
             if (ra + 1 < rows) {
-                a2 = r1[N + x];
-                b2 = r1[N + xm];
+                a2 = r1[n + x];
+                b2 = r1[n + xm];
             }
             const float2 h1 = make_float2(0.5f * (a1.x + b1.x), 0.5f * (a1.y - b1.y));
             const float2 h2 = make_float2(0.5f * (a2.x + b2.x), 0.5f * (a2.y - b2.y));
             z = make_float2(h1.x - h2.y, -(h1.y + h2.x));
         }
-        sm[qq * RS + p2_slot(x)] = z;
+        sm[qq * S::RS + p2_slot(x)] = z;
     }
     __syncthreads();
-    p2_fft<N, 0, kP2None>(sm + q * RS, j, twr, nullptr, 0.f);
-    for (int idx = tid; idx < pairs * N; idx += kCtfThreads) {
-        const int qq = idx / N, x = idx & (N - 1);
-        const float2 w = sm[qq * RS + p2_slot(x)];
+    reg_dft<M>(sm, R, twr);
+    for (int idx = tid; idx < pairs * M; idx += S::TH) {
+        const int qq = idx / M, x = idx & (M - 1);
+        if (x >= n) continue;
+        const float2 w = sm[qq * S::RS + p2_slot(x)];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             const int r = 2 * (pair0 + qq) + c;
@@ -461,10 +529,16 @@ const double kF32Pi = static_cast<double>(static_cast<float>(kPi));  // std::f32
 using namespace tiltgpu;
 using tiltgpu_sess::fail;
 
+// Transform plan of one length n: the register kernels (M > 0: length-M
+// radix-8 passes, directly when n == M, by Bluestein otherwise) or the
+// mixed-radix LDS passes (M == 0).
 struct CtfPlan {
     FilterParams p{};
-    float2* tw = nullptr;
-    float2* tw8 = nullptr;  // power-of-two plan table (64 <= n <= 2048), else null
+    int M = 0;
+    float2* tw = nullptr;     // mixed-radix twiddles [n]
+    float2* tw8 = nullptr;    // per-pass twiddles of the length-M transform
+    float2* chirp = nullptr;  // Bluestein chirp [n]
+    float2* bspec = nullptr;  // Bluestein kernel spectrum [M]
 };
 
 struct tiltgpu_ctf {
@@ -490,11 +564,14 @@ struct tiltgpu_ctf {
 
 namespace {
 
+void ctf_free_plan(CtfPlan& pl) {
+    float2* ps[] = {pl.tw, pl.tw8, pl.chirp, pl.bspec};
+    for (float2* q : ps)
+        if (q) (void)hipFree(q);
+}
+
 void ctf_release(tiltgpu_ctf* c) {
-    for (auto& kv : c->plans) {
-        (void)hipFree(kv.second.tw);
-        if (kv.second.tw8) (void)hipFree(kv.second.tw8);
-    }
+    for (auto& kv : c->plans) ctf_free_plan(kv.second);
     void* bufs[] = {c->in, c->out, c->wtot, c->a, c->b, c->def, c->astig};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -516,13 +593,106 @@ int ctf_alloc(T** p, size_t count) {
     return TILTGPU_OK;
 }
 
-// Transform plan of length n: the mixed-radix passes of the LDS FFT with a
-// double-precision twiddle table; max(1, 2048 / n) rows per workgroup.
+int largest_prime_factor(int n) {
+    int best = 1;
+    for (int p = 2; p * p <= n; ++p)
+        while (n % p == 0) {
+            best = p;
+            n /= p;
+        }
+    return std::max(best, n);
+}
+
+// In-place radix-2 DFT of length m (a power of two) in double precision.
+void host_fft(std::vector<double>& re, std::vector<double>& im) {
+    const int m = static_cast<int>(re.size());
+    for (int i = 1, j = 0; i < m; ++i) {
+        int bit = m >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (int len = 2; len <= m; len <<= 1) {
+        const int h = len / 2;
+        for (int k = 0; k < h; ++k) {
+            const double a = -2.0 * kPi * k / len, wr = std::cos(a), wi = std::sin(a);
+            for (int i = 0; i < m; i += len) {
+                const double xr = re[i + k + h] * wr - im[i + k + h] * wi;
+                const double xi = re[i + k + h] * wi + im[i + k + h] * wr;
+                re[i + k + h] = re[i + k] - xr;
+                im[i + k + h] = im[i + k] - xi;
+                re[i + k] += xr;
+                im[i + k] += xi;
+            }
+        }
+    }
+}
+
+template <class T>
+int upload(T** dst, const std::vector<T>& v) {
+    int rc = ctf_alloc(dst, v.size());
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return TILTGPU_OK;
+}
+
+// Transform plan of length n.  Powers of two from 64 to 4096 use the register
+// radix-8 passes; other lengths up to 2048 with a prime factor above 13 (the
+// mixed-radix passes cost O(n p) for a prime factor p) use Bluestein over the
+// smallest power of two M >= max(64, 2n - 1); the rest use the mixed-radix LDS
+// passes, max(1, 2048 / n) rows per workgroup.  TILTGPU_CTF_BLUESTEIN=all
+// selects Bluestein for every other length up to 2048 as well.
 int ctf_plan(tiltgpu_ctf* c, int n) {
     if (c->plans.count(n)) return TILTGPU_OK;
     CtfPlan pl;
+    const char* env = std::getenv("TILTGPU_CTF_BLUESTEIN");
+    const bool all = env && std::strcmp(env, "all") == 0;
+    const bool pow2 = (n & (n - 1)) == 0;
+    if (pow2 && n >= 64 && n <= 4096) {
+        pl.M = n;
+    } else if (n >= 2 && n <= 2048 && !pow2 && (all || largest_prime_factor(n) > 13)) {
+        int m = 64;
+        while (m < 2 * n - 1) m <<= 1;
+        pl.M = m;
+    }
+    int rc;
+    pl.p.n = n;
+    if (pl.M > 0) {
+        std::vector<float2> t8(pow2_twiddles(pl.M, nullptr, 4096));
+        pow2_twiddles(pl.M, t8.data(), 4096);
+        if ((rc = upload(&pl.tw8, t8))) return rc;
+        if (pl.M != n) {
+            // chirp w_m = exp(i pi m^2 / n), the phase reduced exactly (m^2 mod 2n)
+            std::vector<float2> w(n);
+            std::vector<double> br(pl.M, 0.0), bi(pl.M, 0.0);
+            for (int k = 0; k < n; ++k) {
+                const long long e = (static_cast<long long>(k) * k) % (2LL * n);
+                const double a = kPi * static_cast<double>(e) / static_cast<double>(n);
+                w[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+                br[k] = std::cos(a);
+                bi[k] = std::sin(a);
+                if (k > 0) {
+                    br[pl.M - k] = br[k];
+                    bi[pl.M - k] = bi[k];
+                }
+            }
+            host_fft(br, bi);
+            std::vector<float2> bs(pl.M);
+            for (int k = 0; k < pl.M; ++k)
+                bs[k] = make_float2(static_cast<float>(br[k] / pl.M),
+                                    static_cast<float>(bi[k] / pl.M));
+            if ((rc = upload(&pl.chirp, w)) || (rc = upload(&pl.bspec, bs))) {
+                ctf_free_plan(pl);
+                return rc;
+            }
+        }
+        c->plans[n] = pl;
+        return TILTGPU_OK;
+    }
     FilterParams& f = pl.p;
-    f.n = n;
     f.pairs_per_wg = std::max(1, 2048 / n);
     if (n == 1) {
         f.nradix = 0;
@@ -542,58 +712,48 @@ int ctf_plan(tiltgpu_ctf* c, int n) {
         const double ang = -2.0 * kPi * static_cast<double>(t) / static_cast<double>(n);
         tw[t] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
     }
-    int rc = ctf_alloc(&pl.tw, n);
-    if (rc) return rc;
-    HIP_TRY(hipMemcpy(pl.tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    if ((rc = upload(&pl.tw, tw))) return rc;
     f.tw = pl.tw;
-    if (const int m = pow2_twiddles(n, nullptr)) {
-        std::vector<float2> t8(m);
-        pow2_twiddles(n, t8.data());
-        if ((rc = ctf_alloc(&pl.tw8, m))) {
-            (void)hipFree(pl.tw);
-            return rc;
-        }
-        HIP_TRY(hipMemcpy(pl.tw8, t8.data(), m * sizeof(float2), hipMemcpyHostToDevice));
-    }
     c->plans[n] = pl;
     return TILTGPU_OK;
 }
 
-using P2Kernel = void (*)(CtfStrip, const float2*);
+using RegKernel = void (*)(CtfStrip, RegPlan);
 
-// The power-of-two kernel of stage 1 (0), 3 (1) or 5 (2) for length n.
-P2Kernel p2_kernel(int stage, int n) {
-#define TILTGPU_CTF_P2(N)                                                            \
-    case N:                                                                          \
-        return stage == 0 ? ctf_rows_fwd_p2<N> : stage == 1 ? ctf_cols_p2<N> : ctf_rows_inv_p2<N>;
-    switch (n) {
-        TILTGPU_CTF_P2(64)
-        TILTGPU_CTF_P2(128)
-        TILTGPU_CTF_P2(256)
-        TILTGPU_CTF_P2(512)
-        TILTGPU_CTF_P2(1024)
-        TILTGPU_CTF_P2(2048)
+// The register kernel of stage 1 (0), 3 (1) or 5 (2) for transform length M.
+RegKernel reg_kernel(int stage, int M) {
+#define TILTGPU_CTF_REG(N) \
+    case N:                \
+        return stage == 0 ? ctf_rows_fwd_reg<N> : stage == 1 ? ctf_cols_reg<N> : ctf_rows_inv_reg<N>;
+    switch (M) {
+        TILTGPU_CTF_REG(64)
+        TILTGPU_CTF_REG(128)
+        TILTGPU_CTF_REG(256)
+        TILTGPU_CTF_REG(512)
+        TILTGPU_CTF_REG(1024)
+        TILTGPU_CTF_REG(2048)
+        TILTGPU_CTF_REG(4096)
         default:
             return nullptr;
     }
-#undef TILTGPU_CTF_P2
+#undef TILTGPU_CTF_REG
 }
 
-// Stage `stage` over `items` rows (stage 0 and 2: real rows, two per
-// transform) with the power-of-two kernels.
-hipError_t launch_p2(int stage, const CtfStrip& s, const CtfPlan& pl, int items, hipStream_t st) {
-    const int n = pl.p.n, ppw = 2048 / n;
+// Stage `stage` over `items` rows (stages 0 and 2: real rows, two per
+// transform) with the register kernels.
+hipError_t launch_reg(int stage, const CtfStrip& s, const CtfPlan& pl, int items, hipStream_t st) {
+    const int M = pl.M, ppw = M >= 2048 ? 1 : 2048 / M, threads = (M / 8) * ppw;
     const int units = stage == 1 ? items : (items + 1) / 2;
     const int nblk = (units + ppw - 1) / ppw;
     CtfStrip a = s;
-    const float2* t8 = pl.tw8;
-    void* args[] = {&a, &t8};
-    const size_t lds = static_cast<size_t>(ppw) * p2_row_slots(n) * sizeof(float2);
-    return hipLaunchKernel(reinterpret_cast<const void*>(p2_kernel(stage, n)), dim3(nblk),
-                           dim3(kCtfThreads), args, lds, st);
+    RegPlan r{pl.tw8, pl.chirp, pl.bspec, pl.p.n};
+    void* args[] = {&a, &r};
+    const size_t lds = static_cast<size_t>(ppw) * p2_row_slots(M) * sizeof(float2);
+    return hipLaunchKernel(reinterpret_cast<const void*>(reg_kernel(stage, M)), dim3(nblk),
+                           dim3(threads), args, lds, st);
 }
 
-size_t plan_lds(const FilterParams& f) {
+size_t plan_lds(const FilterParams& f) {  // two row buffers
     return static_cast<size_t>(2) * f.pairs_per_wg * f.n * sizeof(float2);
 }
 
@@ -751,11 +911,6 @@ int tiltgpu_ctf_create(int32_t nx, int32_t ny, int32_t nz, const tiltgpu_ctf_par
     for (const StripBox& b : c->strips) {
         if (b.bx1 <= b.bx0 || b.by1 <= b.by0) continue;
         const int sw = b.bx1 - b.bx0, sh = b.by1 - b.by0;
-        if (sw > kCtfMaxSide || sh > kCtfMaxSide) {
-            ctf_release(c);
-            return fail(TILTGPU_ERR_ARG, "strip bounding box %d x %d exceeds %d", sw, sh,
-                        kCtfMaxSide);
-        }
         max_area = std::max(max_area, static_cast<long>(sw) * sh);
     }
     c->zc = static_cast<int>(std::max(1L, std::min<long>(nz, kCtfChunkElems / max_area)));
@@ -788,7 +943,14 @@ int tiltgpu_ctf_create(int32_t nx, int32_t ny, int32_t nz, const tiltgpu_ctf_par
             return rc;
         }
     }
-    for (const auto& kv : c->plans) lds_max = std::max(lds_max, plan_lds(kv.second.p));
+    for (const auto& kv : c->plans)
+        if (kv.second.M == 0) lds_max = std::max(lds_max, plan_lds(kv.second.p));
+    if (lds_max > kCtfMaxLds) {  // mixed-radix rows beyond 10240 points
+        ctf_release(c);
+        return fail(TILTGPU_ERR_ARG,
+                    "a strip bounding box side needs %zu bytes of LDS (more than %zu)", lds_max,
+                    kCtfMaxLds);
+    }
     if (lds_max > 65536) {  // rows longer than 4096: opt in to the larger LDS allocation
         const void* ks[] = {reinterpret_cast<const void*>(ctf_rows_fwd_kernel),
                             reinterpret_cast<const void*>(ctf_cols_kernel),
@@ -874,18 +1036,18 @@ int tiltgpu_ctf_run(tiltgpu_ctf* c) {
             s.def = c->def + si * c->nz + z0;
             const CtfPlan& px = c->plans.at(s.sw);
             const CtfPlan& py = c->plans.at(s.sh);
-            if (px.tw8)
-                HIP_TRY(launch_p2(0, s, px, zc * s.sh, st));
+            if (px.M)
+                HIP_TRY(launch_reg(0, s, px, zc * s.sh, st));
             else
                 HIP_TRY(launch_rows(ctf_rows_fwd_kernel, s, px.p, zc * s.sh, st));
             HIP_TRY(launch_transpose(s.a, s.b, s.sh, s.sw, zc, st));
-            if (py.tw8)
-                HIP_TRY(launch_p2(1, s, py, zc * s.sw, st));
+            if (py.M)
+                HIP_TRY(launch_reg(1, s, py, zc * s.sw, st));
             else
                 HIP_TRY(launch_rows(ctf_cols_kernel, s, py.p, zc * s.sw, st));
             HIP_TRY(launch_transpose(s.b, s.a, s.sw, s.sh, zc, st));
-            if (px.tw8)
-                HIP_TRY(launch_p2(2, s, px, zc * s.sh, st));
+            if (px.M)
+                HIP_TRY(launch_reg(2, s, px, zc * s.sh, st));
             else
                 HIP_TRY(launch_rows(ctf_rows_inv_kernel, s, px.p, zc * s.sh, st));
             launches += 5;

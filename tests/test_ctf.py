@@ -122,6 +122,11 @@ CASES = [
     (3, 128, 128, dict(pixel_a=2.0, tilt_axis_deg=89.0, max_strip_width=64, strip_width=64)),
     (3, 128, 128, dict(pixel_a=2.0, max_strip_width=64, strip_width=64,
                        vdef=[(-2000.0, -2500.0, 30.0)] * 3)),
+    # Bluestein (a prime factor above 13): 68 = 4*17 and 97 over 256 points,
+    # 1843 = 19*97 over 4096 points (512 threads); 4096 directly
+    (2, 97, 68, dict(pixel_a=3.0, max_strip_width=68, strip_width=68, tilt_axis_deg=0.0)),
+    (1, 34, 1843, dict(pixel_a=4.0, tilt_axis_deg=90.0, plate_deg=30.0)),
+    (1, 4096, 24, dict(pixel_a=4.0, max_strip_width=24, strip_width=24)),
 ]
 
 
@@ -148,6 +153,20 @@ def test_ctf_phaseflip_matches_oracle(gpu, rel, ci):
     same = o32 == P
     assert np.array_equal(g[same], P[same])
     assert not np.allclose(g, P)
+
+
+@pytest.mark.gpu
+def test_ctf_bluestein_for_every_length(gpu, rel, monkeypatch):
+    # the transform choice changes rounding only
+    monkeypatch.setenv("TILTGPU_CTF_BLUESTEIN", "all")
+    nz, ny, nx = 2, 45, 195
+    P = _stack(nz, ny, nx, 5)
+    tilt = np.array([-30.0, 40.0], np.float32)
+    kw = dict(pixel_a=3.0, tilt_axis_deg=-20.0, amp=0.1)
+    g = gpu.ctf_phaseflip(P, tilt, **kw)
+    width = gpu.ctf_strip_width(gpu.ctf_params(**kw), nx)
+    o = O.ctf_phaseflip(P, tilt, strip_width=width, prec="f64", **kw)
+    assert rel(g, o) <= TOL
 
 
 @pytest.mark.gpu
