@@ -42,6 +42,12 @@ constexpr size_t kCtfMaxLds = 160 * 1024;    // LDS per workgroup on gfx950
 constexpr long kCtfChunkElems = 64L << 20;   // complex elements per work buffer
 constexpr double kPi = 3.14159265358979323846;
 
+struct CtfAstig {
+    double c2, s2;  // cos 2a, sin 2a of the astigmatism angle a
+    double on;      // 1 where |defocus1 - defocus2| > 0.01 A (main.rs:385)
+    double pad;
+};
+
 struct CtfStrip {
     const float* in;       // [zc][ny][nx] sections of the chunk
     float* wsum;           // [zc][ny][nx] weighted sums (the output stack)
@@ -49,21 +55,28 @@ struct CtfStrip {
     float2* a;             // [zc][sh][sw]
     float2* b;             // [zc][sw][sh]
     const double2* def;    // [zc] strip defocus 1 and 2 (Angstrom)
-    const double2* astig;  // [zc] astigmatism angle (rad), 1.0 where astigmatic
+    const CtfAstig* astig; // [zc] cos 2a, sin 2a of the astigmatism angle a; on
     long plane;            // nx * ny
     int nx, zc, bx0, by0, sw, sh;
     FastDiv sw_div, sh_div;
     double cx, cy, px, py;     // image centre, direction across the tilt axis
     double scp, lo, hi, half;  // strip centre, membership [lo, hi), width / 2
     double den_x, den_y;       // sw * pixel, sh * pixel (Angstrom)
-    double lambda, cs_a, amp, amp_phase, cuton, plate, ca, sa;
+    double lambda, cs_a, alpha, cuton, plate, ca, sa;  // alpha = asin(amplitude contrast)
     int rotate;                // |tilt axis| > 1e-6 rad
     float scale;               // 1 / (sw * sh)
 };
 
 // main.rs:564-630 for the coefficient (fx, fy) of section z: true where the
 // CTF (with amplitude contrast and phase plate) is negative and the spatial
-// frequency is at or above the cut-on.
+// frequency is at or above the cut-on.  Evaluated in double precision with two
+// identities instead of trigonometric calls:
+//   -sin(c) sqrt(1 - w^2) + cos(c) w = -sin(c - asin w), negative exactly when
+//     (c - asin w) / pi has an even floor and is not an integer;
+//   def1 cos^2(t - a) + def2 sin^2(t - a) = (def1 + def2)/2 + (def1 - def2)/2 cos(2t - 2a),
+//     with cos 2t = (sx^2 - sy^2)/s^2 and sin 2t = 2 sx sy / s^2 for t = atan2(sy, sx).
+// The decision can differ from the reference's only within ~1e-14 (relative)
+// of a CTF zero, the same as between two correctly rounded sin implementations.
 __device__ bool ctf_negative(const CtfStrip& s, int fx, int fy, int z) {
     const double freq_y = fy <= s.sh / 2 ? static_cast<double>(fy)
                                          : static_cast<double>(fy) - static_cast<double>(s.sh);
@@ -77,19 +90,18 @@ __device__ bool ctf_negative(const CtfStrip& s, int fx, int fy, int z) {
     const double sx = rx / s.den_x;
     const double sy = ry / s.den_y;
     const double s2 = sx * sx + sy * sy;
-    if (sqrt(s2) < s.cuton) return false;
+    if (s.cuton > 0.0 && sqrt(s2) < s.cuton) return false;
     const double2 d = s.def[z];
     double def = d.x;
-    const double2 ag = s.astig[z];
-    if (ag.y != 0.0) {
-        const double da = atan2(sy, sx) - ag.x;
-        const double c = cos(da), sn = sin(da);
-        def = d.x * (c * c) + d.y * (sn * sn);
+    const CtfAstig ag = s.astig[z];
+    if (ag.on != 0.0 && s2 > 0.0) {
+        const double c2t = (sx * sx - sy * sy) / s2, s2t = 2.0 * sx * sy / s2;
+        def = 0.5 * (d.x + d.y) + 0.5 * (d.x - d.y) * (c2t * ag.c2 + s2t * ag.s2);
     }
     const double chi = kPi * s.lambda * s2 * (def - 0.5 * s.cs_a * s.lambda * s.lambda * s2);
-    const double chi_t = chi + s.plate;
-    const double ctf = -sin(chi_t) * s.amp_phase + cos(chi_t) * s.amp;
-    return ctf < 0.0;
+    const double t = (chi + s.plate - s.alpha) * (1.0 / kPi);
+    const double fl = floor(t);
+    return t > fl && (static_cast<long long>(fl) & 1) == 0;
 }
 
 // Stage 1: rows [z][y] of the bounding box, forward DFT along x.
@@ -148,6 +160,47 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_kernel(CtfStrip s, Filte
     for (int idx = threadIdx.x; idx < cnt * n; idx += kCtfThreads) {
         const float2 v = Y[idx];
         row[idx] = make_float2(v.x, -v.y);
+    }
+}
+
+// Stages 2-4 in one, for mixed-radix columns: C adjacent columns of one
+// section read straight from A [z][sh][sw] (C * 8-byte row segments), the
+// forward DFT along y, the flip, the inverse, and the columns written back in
+// place, so A needs no transposes.
+template <int C>
+__global__ void __launch_bounds__(kCtfThreads) ctf_cols_tile_kernel(CtfStrip s, FilterParams P) {
+    extern __shared__ float2 sm[];
+    const int n = P.n;  // column length sh
+    const int ntile = (s.sw + C - 1) / C;
+    const int z = blockIdx.x / ntile;
+    const int x0 = (blockIdx.x - z * ntile) * C;
+    float2* b0 = sm;
+    float2* b1 = sm + C * n;
+    float2* A = s.a + static_cast<long>(z) * s.sh * s.sw + x0;
+    for (int idx = threadIdx.x; idx < C * n; idx += kCtfThreads) {
+        const int y = idx / C, c = idx - y * C;
+        b0[c * n + y] = x0 + c < s.sw ? A[static_cast<long>(y) * s.sw + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2* X = fft_forward(b0, b1, P, C);
+    for (int idx = threadIdx.x; idx < C * n; idx += kCtfThreads) {
+        const int c = static_cast<int>(P.n_div.div(static_cast<uint32_t>(idx)));
+        const int fy = idx - c * n;
+        float2 v = X[idx];
+        if (x0 + c < s.sw && ctf_negative(s, x0 + c, fy, z)) {
+            v.x = -v.x;
+            v.y = -v.y;
+        }
+        X[idx] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    const float2* Y = fft_forward(X, X == b0 ? b1 : b0, P, C);
+    for (int idx = threadIdx.x; idx < C * n; idx += kCtfThreads) {
+        const int y = idx / C, c = idx - y * C;
+        if (x0 + c < s.sw) {
+            const float2 v = Y[c * n + y];
+            A[static_cast<long>(y) * s.sw + c] = make_float2(v.x, -v.y);
+        }
     }
 }
 
@@ -546,7 +599,7 @@ struct tiltgpu_ctf {
     tiltgpu_ctf_params prm{};
     std::vector<StripBox> strips;
     std::vector<double2> def_host;    // [n_strips][nz]
-    std::vector<double2> astig_host;  // [nz]
+    std::vector<CtfAstig> astig_host;  // [nz]
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float* in = nullptr;
@@ -555,7 +608,7 @@ struct tiltgpu_ctf {
     float2* a = nullptr;
     float2* b = nullptr;
     double2* def = nullptr;
-    double2* astig = nullptr;
+    CtfAstig* astig = nullptr;
     std::map<int, CtfPlan> plans;
     double lambda = 0, cs_a = 0, amp = 0, amp_phase = 0, cuton = 0, plate = 0, axis = 0;
     double last_ms = 0;
@@ -765,6 +818,28 @@ hipError_t launch_rows(void (*k)(CtfStrip, FilterParams), CtfStrip s, FilterPara
                            plan_lds(f), st);
 }
 
+// Columns per workgroup of the in-place column kernel for column length n (0:
+// transpose instead): 16 columns (128-byte row segments) while the two LDS
+// buffers stay within 64 KiB, else 8.
+int cols_tile_width(int n) {
+    if (2 * 16 * n * static_cast<int>(sizeof(float2)) <= 65536) return 16;
+    if (2 * 8 * n * static_cast<int>(sizeof(float2)) <= 65536) return 8;
+    return 0;
+}
+
+hipError_t launch_cols_tile(int C, const CtfStrip& s, const FilterParams& f, int zc,
+                            hipStream_t st) {
+    const int ntile = (s.sw + C - 1) / C;
+    CtfStrip a = s;
+    FilterParams q = f;
+    void* args[] = {&a, &q};
+    const void* k = C == 16 ? reinterpret_cast<const void*>(ctf_cols_tile_kernel<16>)
+                            : reinterpret_cast<const void*>(ctf_cols_tile_kernel<8>);
+    const size_t lds = static_cast<size_t>(2) * C * f.n * sizeof(float2);
+    return hipLaunchKernel(k, dim3(static_cast<unsigned>(zc) * ntile), dim3(kCtfThreads), args,
+                           lds, st);
+}
+
 hipError_t launch_transpose(const float2* src, float2* dst, int rows, int cols, int batch,
                             hipStream_t st) {
     dim3 grid((cols + 31) / 32, (rows + 31) / 32, batch);
@@ -897,8 +972,9 @@ int tiltgpu_ctf_create(int32_t nx, int32_t ny, int32_t nz, const tiltgpu_ctf_par
             aa = vdef[3 * z + 2];
         }
         const double def1 = static_cast<double>(d1) * 10.0, def2 = static_cast<double>(d2) * 10.0;
-        c->astig_host[z] = make_double2(static_cast<double>(aa) * kF32Pi / 180.0,
-                                        std::fabs(def1 - def2) > 0.01 ? 1.0 : 0.0);
+        const double arad = static_cast<double>(aa) * kF32Pi / 180.0;
+        c->astig_host[z] = CtfAstig{std::cos(2.0 * arad), std::sin(2.0 * arad),
+                                    std::fabs(def1 - def2) > 0.01 ? 1.0 : 0.0, 0.0};
         const double st = std::sin(tilt_rad);
         for (int si = 0; si < ns; ++si) {
             const double dx_a = c->strips[si].scp * pixel;
@@ -930,7 +1006,7 @@ int tiltgpu_ctf_create(int32_t nx, int32_t ny, int32_t nz, const tiltgpu_ctf_par
     }
     if (hipMemcpy(c->def, c->def_host.data(), c->def_host.size() * sizeof(double2),
                   hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->astig, c->astig_host.data(), c->astig_host.size() * sizeof(double2),
+        hipMemcpy(c->astig, c->astig_host.data(), c->astig_host.size() * sizeof(CtfAstig),
                   hipMemcpyHostToDevice) != hipSuccess) {
         ctf_release(c);
         return fail(TILTGPU_ERR_HIP, "defocus table upload failed");
@@ -999,8 +1075,7 @@ int tiltgpu_ctf_run(tiltgpu_ctf* c) {
     s.half = static_cast<double>(c->width) / 2.0;
     s.lambda = c->lambda;
     s.cs_a = c->cs_a;
-    s.amp = c->amp;
-    s.amp_phase = c->amp_phase;
+    s.alpha = std::atan2(c->amp, c->amp_phase);
     s.cuton = c->cuton;
     s.plate = c->plate;
     s.rotate = std::fabs(c->axis) > 1e-6 ? 1 : 0;
@@ -1040,12 +1115,18 @@ int tiltgpu_ctf_run(tiltgpu_ctf* c) {
                 HIP_TRY(launch_reg(0, s, px, zc * s.sh, st));
             else
                 HIP_TRY(launch_rows(ctf_rows_fwd_kernel, s, px.p, zc * s.sh, st));
-            HIP_TRY(launch_transpose(s.a, s.b, s.sh, s.sw, zc, st));
-            if (py.M)
-                HIP_TRY(launch_reg(1, s, py, zc * s.sw, st));
-            else
-                HIP_TRY(launch_rows(ctf_cols_kernel, s, py.p, zc * s.sw, st));
-            HIP_TRY(launch_transpose(s.b, s.a, s.sw, s.sh, zc, st));
+            const int tile = py.M ? 0 : cols_tile_width(s.sh);
+            if (tile) {
+                HIP_TRY(launch_cols_tile(tile, s, py.p, zc, st));
+                launches -= 2;
+            } else {
+                HIP_TRY(launch_transpose(s.a, s.b, s.sh, s.sw, zc, st));
+                if (py.M)
+                    HIP_TRY(launch_reg(1, s, py, zc * s.sw, st));
+                else
+                    HIP_TRY(launch_rows(ctf_cols_kernel, s, py.p, zc * s.sw, st));
+                HIP_TRY(launch_transpose(s.b, s.a, s.sw, s.sh, zc, st));
+            }
             if (px.M)
                 HIP_TRY(launch_reg(2, s, px, zc * s.sh, st));
             else
