@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_rows_inv_kernel(CtfStrip s, F
 }
 
 // ---- register transforms: rows of length n held in LDS rows of M = 2^L
-// (64 <= M <= 4096) slots, max(1, 2048 / M) rows per workgroup, M / 8 threads
+// (16 <= M <= 4096) slots, max(1, 2048 / M) rows per workgroup, M / 8 threads
 // per row.  n == M: the radix-8 passes directly.  n < M (Bluestein): the
 // length-n DFT as the circular convolution of length M >= 2n - 1
 //   X[k] = conj(w_k) * sum_j (x_j conj(w_j)) w_{k-j},  w_m = exp(i pi m^2 / n),
@@ -692,7 +692,7 @@ int upload(T** dst, const std::vector<T>& v) {
     return TILTGPU_OK;
 }
 
-// Transform plan of length n.  Powers of two from 64 to 4096 use the register
+// Transform plan of length n.  Powers of two from 16 to 4096 use the register
 // radix-8 passes; other lengths up to 2048 with a prime factor above 13 (the
 // mixed-radix passes cost O(n p) for a prime factor p) use Bluestein over the
 // smallest power of two M >= max(64, 2n - 1); the rest use the mixed-radix LDS
@@ -704,7 +704,7 @@ int ctf_plan(tiltgpu_ctf* c, int n) {
     const char* env = std::getenv("TILTGPU_CTF_BLUESTEIN");
     const bool all = env && std::strcmp(env, "all") == 0;
     const bool pow2 = (n & (n - 1)) == 0;
-    if (pow2 && n >= 64 && n <= 4096) {
+    if (pow2 && n >= 16 && n <= 4096) {
         pl.M = n;
     } else if (n >= 2 && n <= 2048 && !pow2 && (all || largest_prime_factor(n) > 13)) {
         int m = 64;
@@ -714,8 +714,8 @@ int ctf_plan(tiltgpu_ctf* c, int n) {
     int rc;
     pl.p.n = n;
     if (pl.M > 0) {
-        std::vector<float2> t8(pow2_twiddles(pl.M, nullptr, 4096));
-        pow2_twiddles(pl.M, t8.data(), 4096);
+        std::vector<float2> t8(pow2_twiddles(pl.M, nullptr, 4096, 16));
+        pow2_twiddles(pl.M, t8.data(), 4096, 16);
         if ((rc = upload(&pl.tw8, t8))) return rc;
         if (pl.M != n) {
             // chirp w_m = exp(i pi m^2 / n), the phase reduced exactly (m^2 mod 2n)
@@ -779,6 +779,8 @@ RegKernel reg_kernel(int stage, int M) {
     case N:                \
         return stage == 0 ? ctf_rows_fwd_reg<N> : stage == 1 ? ctf_cols_reg<N> : ctf_rows_inv_reg<N>;
     switch (M) {
+        TILTGPU_CTF_REG(16)
+        TILTGPU_CTF_REG(32)
         TILTGPU_CTF_REG(64)
         TILTGPU_CTF_REG(128)
         TILTGPU_CTF_REG(256)

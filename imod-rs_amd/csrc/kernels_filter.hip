@@ -342,8 +342,8 @@ int filter_plan(int n, int* radix, int* nradix) {
     return 0;
 }
 
-int pow2_twiddles(int n, float2* out, int max_n) {
-    if (n < 64 || n > max_n || (n & (n - 1)) != 0) return 0;
+int pow2_twiddles(int n, float2* out, int max_n, int min_n) {
+    if (n < min_n || n > max_n || (n & (n - 1)) != 0) return 0;
     int off = 0;
     for (int ns = 1; ns < n;) {
         const int R = n / ns >= 8 ? 8 : n / ns;

@@ -133,8 +133,8 @@ struct FilterParams {
 // Per-pass twiddles of the power-of-two plan (radix-8 passes, then one
 // radix-4 or radix-2 pass): for pass s of radix R and span NS, entries
 // [off_s + k*R + r] = exp(-2*pi*i*r*k/(NS*R)), k < NS, r < R; returns the entry
-// count (0 when n is not a power of two in [64, max_n]).  out may be null.
-int pow2_twiddles(int n, float2* out, int max_n = 2048);
+// count (0 when n is not a power of two in [min_n, max_n]).  out may be null.
+int pow2_twiddles(int n, float2* out, int max_n = 2048, int min_n = 64);
 
 // Ordered-walk forward projection (no x-axis tilt, no local alignment).
 // Layout: vol [in_ny][out_nz][out_nx] f32, sino [nz][in_ny][in_nx] f32.
