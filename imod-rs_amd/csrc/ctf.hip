@@ -636,6 +636,11 @@ void ctf_release(tiltgpu_ctf* c) {
 
 template <class T>
 int ctf_alloc(T** p, size_t count) {
+    if (tiltgpu_sess::forced_alloc_failure()) {
+        *p = nullptr;
+        return fail(TILTGPU_ERR_BUDGET, "device buffer of %zu bytes refused (TILTGPU_FAIL_ALLOC_AT)",
+                    count * sizeof(T));
+    }
     hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(1, count) * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;

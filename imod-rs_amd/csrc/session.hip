@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,6 +32,23 @@ int fail(int code, const char* fmt, ...) {
     va_end(ap);
     g_last_error = buf;
     return code;
+}
+
+bool forced_alloc_failure() {
+    static std::mutex m;
+    static std::string seen;
+    static long count = 0;
+    const char* e = std::getenv("TILTGPU_FAIL_ALLOC_AT");
+    std::lock_guard<std::mutex> lk(m);
+    if (!e || !*e) {
+        seen.clear();
+        return false;
+    }
+    if (seen != e) {  // a new setting restarts the count
+        seen = e;
+        count = 0;
+    }
+    return ++count == std::atol(e);
 }
 
 }  // namespace tiltgpu_sess

@@ -16,6 +16,11 @@ namespace tiltgpu_sess {
 // Records the calling thread's last error message; returns code.
 int fail(int code, const char* fmt, ...);
 
+// Failure injection for tests of the error paths: with TILTGPU_FAIL_ALLOC_AT=n
+// in the environment, the n-th device allocation counted since the variable
+// last changed is refused (TILTGPU_ERR_BUDGET), as an exhausted device would.
+bool forced_alloc_failure();
+
 #define HIP_TRY(expr)                                                                 \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \
@@ -145,6 +150,9 @@ inline int dev_alloc(tiltgpu_session* s, T** p, size_t count) {
     if (count == 0) count = 1;
     void* q = nullptr;
     const size_t bytes = count * sizeof(T);
+    if (forced_alloc_failure())
+        return fail(TILTGPU_ERR_BUDGET, "device buffer of %zu bytes refused (TILTGPU_FAIL_ALLOC_AT)",
+                    bytes);
     if (s->pool) {
         q = s->pool->get(bytes);
         if (!q) return fail(TILTGPU_ERR_BUDGET, "device buffer of %zu bytes unavailable", bytes);

@@ -490,18 +490,22 @@ def reconstruct(P, angles, opts: TiltOptions | None = None, xtilt=None, zfactor=
     P = _f32(P)
     nz, ny, nx = P.shape
     s = Session(nx, ny, nz, opts)
-    s.set_angles(angles)
-    if xtilt is not None:
-        s.set_xtilt(xtilt)
-    if zfactor is not None:
-        s.set_zfactor(zfactor)
-    if local is not None:
-        s.set_local(*(local.flat() if hasattr(local, "flat") else local))
-    if trace_rows is not None:
-        s.set_sirt_trace(*trace_rows)
-    s.upload(P)
-    s.run()
-    vol, st = s.read_rows()
+    try:
+        s.set_angles(angles)
+        if xtilt is not None:
+            s.set_xtilt(xtilt)
+        if zfactor is not None:
+            s.set_zfactor(zfactor)
+        if local is not None:
+            s.set_local(*(local.flat() if hasattr(local, "flat") else local))
+        if trace_rows is not None:
+            s.set_sirt_trace(*trace_rows)
+        s.upload(P)
+        s.run()
+        vol, st = s.read_rows()
+    except BaseException:
+        s.close()  # release the device buffers now, not when the traceback goes
+        raise
     hs = header_stats(st, s.out_nx * s.out_nz)
     if return_session:
         return vol, hs, s

@@ -569,6 +569,12 @@ void F(or_lowpass_3d)(REAL* vol, int ny, int nz, int nx) {
  * header amin, amax, amean.
  */
 
+void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int n_angles,
+                         const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
+                         const int* lnx, const int* lny, const float* lxs, const float* lys,
+                         const float* ldx, const float* ldy, int y0, int y1, float* vol,
+                         float* stats, const double* means_in);
+
 void F(or_tilt_run)(const or_opts* o, float* P_in, const float* angles, int n_angles,
                     const float* xtilt, int n_xtilt, const float* zfac, int n_zfac,
                     const int* lnx, const int* lny, const float* lxs, const float* lys,

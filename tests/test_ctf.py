@@ -302,3 +302,24 @@ def test_ctf_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
     assert info.label0.decode() == "ctfphaseflip: CTF phase correction (2D strips)"
     assert info.amin == g.min() and info.amax == g.max()
     assert info.xlen == pytest.approx(nx * 3.5)
+
+
+@pytest.mark.gpu
+def test_ctf_forced_allocation_failures(gpu, monkeypatch):
+    P = _stack(3, 40, 60, 3)
+    tilt = np.array([-20.0, 0.0, 20.0], np.float32)
+    ref = gpu.ctf_phaseflip(P, tilt, pixel_a=3.0, tilt_axis_deg=20.0)
+    failed = 0
+    for k in range(1, 30):
+        monkeypatch.setenv("TILTGPU_FAIL_ALLOC_AT", str(k))
+        try:
+            g = gpu.ctf_phaseflip(P, tilt, pixel_a=3.0, tilt_axis_deg=20.0)
+        except gpu.TiltGpuError as e:
+            assert e.code == gpu.ERR_BUDGET
+            failed += 1
+            continue
+        assert np.array_equal(g, ref)
+    monkeypatch.delenv("TILTGPU_FAIL_ALLOC_AT")
+    assert failed >= 5
+    assert np.array_equal(gpu.ctf_phaseflip(P, tilt, pixel_a=3.0, tilt_axis_deg=20.0), ref)
+

@@ -586,3 +586,28 @@ def test_tile_depths(gpu, rel, monkeypatch, tile_z, geom):
         local = O.build_local_grids([(v, 64.0, 6.0, float(sx[v]), float(sy[v])) for v in range(31)], 31)
     g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=150), xtilt=xt, local=local)
     assert rel(g, o) <= TOL
+
+
+def test_forced_allocation_failures_fail_cleanly(gpu, monkeypatch):
+    """TILTGPU_FAIL_ALLOC_AT=k refuses the k-th device allocation: every call
+    that needs it returns TILTGPU_ERR_BUDGET with its message, a run that needs
+    fewer allocations is unaffected, and the device is usable afterwards."""
+    P, ang = case(gpu, nx=64, ny=4, nviews=9, thick=16, seed=5)
+    opts = gpu.TiltOptions(thickness=16, sirt=2)
+    ref, _ = gpu.reconstruct(P, ang, opts)
+    failed = 0
+    for k in range(1, 41):
+        monkeypatch.setenv("TILTGPU_FAIL_ALLOC_AT", str(k))
+        print(f"case: allocation {k} refused", flush=True)
+        try:
+            v, _ = gpu.reconstruct(P, ang, opts)
+        except gpu.TiltGpuError as e:
+            assert e.code == gpu.ERR_BUDGET and "TILTGPU_FAIL_ALLOC_AT" in str(e)
+            failed += 1
+            continue
+        assert np.array_equal(v, ref)
+    monkeypatch.delenv("TILTGPU_FAIL_ALLOC_AT")
+    assert failed >= 5
+    v, _ = gpu.reconstruct(P, ang, opts)
+    assert np.array_equal(v, ref)
+

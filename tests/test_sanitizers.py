@@ -1,0 +1,48 @@
+"""AddressSanitizer, LeakSanitizer and UndefinedBehaviorSanitizer over the host
+side of libtiltgpu (text parsers, MRC I/O, the synthetic series, the launch-shape
+table) and over the oracle, on the CPU.
+
+tests/sanitize/host_asan_driver.cpp calls each entry point on small valid and
+malformed inputs; it is built with g++/gcc -fsanitize=address,undefined and must
+exit 0 with no sanitizer report.  (The device code is not instrumented: GPU
+sanitizers are not available on the target pool.)
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SAN = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-g", "-O1",
+       "-ffp-contract=off"]
+HOST = ["imod-rs_amd/host/textio.cpp", "imod-rs_amd/host/mrc.cpp", "imod-rs_amd/host/synth.cpp",
+        "imod-rs_amd/csrc/tuning.cpp"]
+
+
+def _run(cmd, **kw):
+    return subprocess.run(cmd, cwd=REPO, capture_output=True, text=True, timeout=600, **kw)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or shutil.which("gcc") is None,
+                    reason="no host compiler")
+def test_host_code_and_oracle_clean_under_sanitizers(tmp_path):
+    obj = str(tmp_path / "oracle.o")
+    r = _run(["gcc", *SAN, "-fno-fast-math", "-std=c11", "-D_GNU_SOURCE", "-c",
+              "oracle/tilt_oracle.c", "-o", obj])
+    assert r.returncode == 0, r.stderr
+    exe = str(tmp_path / "host_asan_driver")
+    r = _run(["g++", *SAN, "-std=c++17", "-Iinclude", "tests/sanitize/host_asan_driver.cpp",
+              *HOST, obj, "-o", exe, "-lpthread", "-lm"])
+    if r.returncode != 0 and "asan" in r.stderr.lower():
+        pytest.skip("the sanitizer runtime is not installed")
+    assert r.returncode == 0, r.stderr
+    work = tmp_path / "files"
+    work.mkdir()
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = _run([exe, str(work)], env=env)
+    out = r.stdout + r.stderr
+    assert "Sanitizer" not in out and "runtime error" not in out, out[-4000:]
+    assert r.returncode == 0, out[-4000:]
+    assert "host_asan_driver: ok" in out
