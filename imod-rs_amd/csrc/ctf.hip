@@ -826,12 +826,12 @@ hipError_t launch_rows(void (*k)(CtfStrip, FilterParams), CtfStrip s, FilterPara
 }
 
 // Columns per workgroup of the in-place column kernel for column length n (0:
-// transpose instead): 16 columns (128-byte row segments) while the two LDS
-// buffers stay within 64 KiB, else 8.
+// transpose instead): 8 columns (64-byte row segments) while the two LDS
+// buffers stay within 64 KiB.  Measured at 2048x2048x121 with a tilt axis at
+// 85 deg (195-point columns): 8 columns 118 ms, 16 columns 128 ms (half the
+// workgroups per CU), 4 columns 154 ms (32-byte segments).
 int cols_tile_width(int n) {
-    if (2 * 16 * n * static_cast<int>(sizeof(float2)) <= 65536) return 16;
-    if (2 * 8 * n * static_cast<int>(sizeof(float2)) <= 65536) return 8;
-    return 0;
+    return 2 * 8 * n * static_cast<int>(sizeof(float2)) <= 65536 ? 8 : 0;
 }
 
 hipError_t launch_cols_tile(int C, const CtfStrip& s, const FilterParams& f, int zc,
@@ -840,8 +840,8 @@ hipError_t launch_cols_tile(int C, const CtfStrip& s, const FilterParams& f, int
     CtfStrip a = s;
     FilterParams q = f;
     void* args[] = {&a, &q};
-    const void* k = C == 16 ? reinterpret_cast<const void*>(ctf_cols_tile_kernel<16>)
-                            : reinterpret_cast<const void*>(ctf_cols_tile_kernel<8>);
+    if (C != 8) return hipErrorInvalidValue;
+    const void* k = reinterpret_cast<const void*>(ctf_cols_tile_kernel<8>);
     const size_t lds = static_cast<size_t>(2) * C * f.n * sizeof(float2);
     return hipLaunchKernel(k, dim3(static_cast<unsigned>(zc) * ntile), dim3(kCtfThreads), args,
                            lds, st);
