@@ -170,7 +170,9 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_kernel(CtfStrip s, Filte
 template <int C>
 __global__ void __launch_bounds__(kCtfThreads) ctf_cols_tile_kernel(CtfStrip s, FilterParams P) {
     extern __shared__ float2 sm[];
+    __shared__ float2 tw_lds[512];  // the twiddle table (n <= 512 on this path)
     const int n = P.n;  // column length sh
+    for (int i = threadIdx.x; i < n; i += kCtfThreads) tw_lds[i] = P.tw[i];
     const int ntile = (s.sw + C - 1) / C;
     const int z = blockIdx.x / ntile;
     const int x0 = (blockIdx.x - z * ntile) * C;
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_tile_kernel(CtfStrip s, 
         b0[c * n + y] = x0 + c < s.sw ? A[static_cast<long>(y) * s.sw + c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    float2* X = fft_forward(b0, b1, P, C);
+    float2* X = fft_forward_tw(b0, b1, P, C, tw_lds);
     for (int idx = threadIdx.x; idx < C * n; idx += kCtfThreads) {
         const int c = static_cast<int>(P.n_div.div(static_cast<uint32_t>(idx)));
         const int fy = idx - c * n;
@@ -194,7 +196,7 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_tile_kernel(CtfStrip s, 
         X[idx] = make_float2(v.x, -v.y);
     }
     __syncthreads();
-    const float2* Y = fft_forward(X, X == b0 ? b1 : b0, P, C);
+    const float2* Y = fft_forward_tw(X, X == b0 ? b1 : b0, P, C, tw_lds);
     for (int idx = threadIdx.x; idx < C * n; idx += kCtfThreads) {
         const int y = idx / C, c = idx - y * C;
         if (x0 + c < s.sw) {

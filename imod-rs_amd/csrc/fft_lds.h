@@ -223,6 +223,42 @@ static __device__ float2* fft_forward(float2* b0, float2* b1, const FilterParams
     return in;
 }
 
+// The same with the twiddle table given separately (e.g. staged in LDS).
+static __device__ __forceinline__ float2* fft_forward_tw(float2* b0, float2* b1,
+                                                         const FilterParams& P, int pairs,
+                                                         const float2* tw) {
+    float2* in = b0;
+    float2* out = b1;
+    int Ns = 1;
+    for (int s = 0; s < P.nradix; ++s) {
+        const int R = P.radix[s];
+        const FastDiv& md = P.m_div[s];
+        const FastDiv& nd = P.ns_div[s];
+        if (R == 4)
+            stockham_pass<4>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 2)
+            stockham_pass<2>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 3)
+            stockham_pass<3>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 5)
+            stockham_pass<5>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 7)
+            stockham_pass_odd<7>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 11)
+            stockham_pass_odd<11>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else if (R == 13)
+            stockham_pass_odd<13>(in, out, P.n, pairs, Ns, tw, md, nd);
+        else
+            stockham_pass_generic(in, out, P.n, pairs, Ns, R, tw, md, nd, P.r_div[s]);
+        __syncthreads();
+        Ns *= R;
+        float2* t = in;
+        in = out;
+        out = t;
+    }
+    return in;
+}
+
 // ---------------------------------------------------------------------------
 // n = 2^L, 64 <= n <= 2048: every thread holds 8 complex values of one row
 // (n/8 threads per row).  Radix-8 Stockham passes with the butterfly in
