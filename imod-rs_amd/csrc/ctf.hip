@@ -23,6 +23,7 @@
 // the transforms are single precision with double-precision twiddles.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -615,6 +616,8 @@ struct tiltgpu_ctf {
     double lambda = 0, cs_a = 0, amp = 0, amp_phase = 0, cuton = 0, plate = 0, axis = 0;
     double last_ms = 0;
     int64_t launches = 0;
+    std::vector<char> uploaded;  // [nz]: section uploaded
+    bool done = false;           // a run has completed since the last upload
 };
 
 namespace {
@@ -1065,11 +1068,17 @@ int tiltgpu_ctf_upload(tiltgpu_ctf* c, int32_t z0, int32_t nsec, const float* ho
     HIP_TRY(hipMemcpyAsync(c->in + plane * z0, host, plane * nsec * sizeof(float),
                            hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->uploaded.empty()) c->uploaded.assign(c->nz, 0);
+    for (int z = z0; z < z0 + nsec; ++z) c->uploaded[z] = 1;
+    c->done = false;
     return TILTGPU_OK;
 }
 
 int tiltgpu_ctf_run(tiltgpu_ctf* c) {
     if (!c) return fail(TILTGPU_ERR_ARG, "context is NULL");
+    if (c->uploaded.empty() ||
+        std::find(c->uploaded.begin(), c->uploaded.end(), 0) != c->uploaded.end())
+        return fail(TILTGPU_ERR_STATE, "run before every section was uploaded");
     const long plane = static_cast<long>(c->nx) * c->ny;
     const hipStream_t st = c->stream;
     HIP_TRY(hipEventRecord(c->ev0, st));
@@ -1156,11 +1165,13 @@ int tiltgpu_ctf_run(tiltgpu_ctf* c) {
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->launches = launches;
+    c->done = true;
     return TILTGPU_OK;
 }
 
 int tiltgpu_ctf_download(tiltgpu_ctf* c, int32_t z0, int32_t nsec, float* host) {
     if (!c || !host) return fail(TILTGPU_ERR_ARG, "context or data is NULL");
+    if (!c->done) return fail(TILTGPU_ERR_STATE, "download before a run of the uploaded stack");
     if (z0 < 0 || nsec < 0 || z0 + nsec > c->nz)
         return fail(TILTGPU_ERR_ARG, "sections [%d, %d) outside [0, %d)", z0, z0 + nsec, c->nz);
     const size_t plane = static_cast<size_t>(c->nx) * c->ny;

@@ -358,7 +358,9 @@ int tiltgpu_ctf_strip_width_used(const tiltgpu_ctf* c);
 /* Sections [z0, z0 + nsec) from / to host memory ([nsec][ny][nx] floats). */
 int tiltgpu_ctf_upload(tiltgpu_ctf* c, int32_t z0, int32_t nsec, const float* host);
 int tiltgpu_ctf_download(tiltgpu_ctf* c, int32_t z0, int32_t nsec, float* host);
-/* Corrects the uploaded stack (main.rs:353-691 for every section); blocks. */
+/* Corrects the uploaded stack (main.rs:353-691 for every section); blocks.
+ * TILTGPU_ERR_STATE before every section has been uploaded; a download needs a
+ * run after the last upload. */
 int tiltgpu_ctf_run(tiltgpu_ctf* c);
 /* Device time (ms) and kernel launches of the last tiltgpu_ctf_run. */
 int tiltgpu_ctf_last_run(const tiltgpu_ctf* c, double* ms, int64_t* launches);

@@ -204,8 +204,21 @@ def test_ctf_gpu_rejects_bad_arguments(gpu):
     try:
         with pytest.raises(gpu.TiltGpuError):
             c.upload(np.zeros((2, 16, 16), np.float32), z0=1)
+        with pytest.raises(gpu.TiltGpuError) as e:
+            c.run()  # nothing uploaded yet
+        assert e.value.code == gpu.ERR_STATE
+        c.upload(np.ones((1, 16, 16), np.float32), z0=0)
+        with pytest.raises(gpu.TiltGpuError) as e:
+            c.run()  # section 1 still missing
+        assert e.value.code == gpu.ERR_STATE
+        c.upload(np.ones((1, 16, 16), np.float32), z0=1)
+        with pytest.raises(gpu.TiltGpuError) as e:
+            c.download()  # no run since the upload
+        assert e.value.code == gpu.ERR_STATE
+        c.run()
         with pytest.raises(gpu.TiltGpuError):
             c.download(1, 2)
+        assert c.download().shape == (2, 16, 16)
     finally:
         c.close()
 
