@@ -336,3 +336,32 @@ def test_ctf_forced_allocation_failures(gpu, monkeypatch):
     assert failed >= 5
     assert np.array_equal(gpu.ctf_phaseflip(P, tilt, pixel_a=3.0, tilt_axis_deg=20.0), ref)
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axis", [0.0, 85.0])
+def test_ctf_full_size_properties(gpu, axis):
+    """At the benchmark's image size (2048 x 2048, 4 A pixels, strip width 16):
+    scaling the input by 2 scales the output by 2 bit for bit (every step is
+    linear and a power of two is exact), and the correction is linear to
+    rounding, ctf(P + Q) = ctf(P) + ctf(Q)."""
+    nz, ny, nx = 4, 2048, 2048
+    rng = np.random.default_rng(int(axis))
+    P = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    Q = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    tilt = np.array([-60.0, -20.0, 20.0, 60.0], np.float32)
+    p = gpu.ctf_params(pixel_a=4.0, tilt_axis_deg=axis)
+    c = gpu.CtfCorrector(nx, ny, nz, p, tilt)
+    try:
+        assert c.strip_width == 16
+        out = []
+        for X in (P, 2.0 * P, Q, P + Q):
+            c.upload(X)
+            c.run()
+            out.append(c.download())
+    finally:
+        c.close()
+    a, a2, b, ab = out
+    assert np.array_equal(a2, 2.0 * a)
+    rms = np.sqrt(np.mean(ab.astype(np.float64) ** 2))
+    assert np.max(np.abs(ab.astype(np.float64) - a - b)) <= 1e-4 * rms
