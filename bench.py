@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(REPO, "imod-rs_amd"))
 METRIC = "Gvoxel·view/s WBP 2k²×512×121; served p50/p99 latency vs plain FP32"
 # BP algorithmic work per voxel-view: proj_x (2 adds), floor/frac (1), lerp of
 # two rows (2 FMA = 4 flop counted as the reference's 2 mul + 2 add), y-blend and
-# accumulate; counted as in the survey's roofline: 7 flop per voxel-view.
+# accumulate: 7 flop per voxel-view in all.
 BP_FLOP_PER_VV = 7.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0

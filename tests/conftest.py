@@ -17,7 +17,7 @@ def pytest_configure(config):
 def gpu():
     import tiltgpu
     if tiltgpu.device_count() < 1:
-        pytest.fail("no GPU visible (tests marked gpu must run on the GPU box)")
+        pytest.fail("no GPU visible (tests marked gpu need an MI355X)")
     return tiltgpu
 
 

@@ -1,7 +1,7 @@
 """GPU parity: libtiltgpu (through its C ABI) against the FP32 oracle, which
 restates apps/tilt/src/main.rs in the reference's operation order.
 
-Bar (north star): max |gpu - oracle| <= 1e-4 * RMS(oracle volume) for every FP32
+Bar: max |gpu - oracle| <= 1e-4 * RMS(oracle volume) for every FP32
 result.  FP16/BF16 storage is compared with the FP64 oracle at the tolerance
 stated in its test.
 """
