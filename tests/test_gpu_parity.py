@@ -611,3 +611,26 @@ def test_forced_allocation_failures_fail_cleanly(gpu, monkeypatch):
     v, _ = gpu.reconstruct(P, ang, opts)
     assert np.array_equal(v, ref)
 
+
+
+@pytest.mark.parametrize("nv,ny,nx,thick,kw", [
+    (1, 3, 16, 8, {}),                          # one view
+    (5, 1, 32, 4, {"xtilt": 1.0}),              # one row, two-row geometry at the last row
+    (7, 2, 2, 3, {}),                           # the narrowest width the filter takes
+    (4, 3, 48, 1, {}),                          # thickness 1
+    (6, 3, 48, 16, {"width": 1}),               # one output column
+    (6, 3, 48, 16, {"width": 80}),              # output wider than the projections
+    (5, 1, 24, 6, {"sirt": 2}),                 # SIRT on a one-row series
+])
+def test_degenerate_shapes(gpu, rel, nv, ny, nx, thick, kw):
+    kw = dict(kw)
+    xt = None
+    if "xtilt" in kw:
+        xt = np.full(nv, kw.pop("xtilt"), np.float32)
+    print(f"case: views {nv} rows {ny} width {nx} thickness {thick} {kw}", flush=True)
+    rng = np.random.default_rng(nv * 100 + ny * 10 + nx)
+    P = rng.normal(1.0, 0.3, (nv, ny, nx)).astype(np.float32)
+    ang = np.linspace(-50.0, 50.0, nv).astype(np.float32) if nv > 1 else np.array([10.0], np.float32)
+    g, sg, o, so = both(gpu, P, ang, gpu.TiltOptions(thickness=thick, **kw), xtilt=xt)
+    assert g.shape == o.shape
+    assert rel(g, o) <= TOL
