@@ -365,3 +365,24 @@ def test_ctf_full_size_properties(gpu, axis):
     assert np.array_equal(a2, 2.0 * a)
     rms = np.sqrt(np.mean(ab.astype(np.float64) ** 2))
     assert np.max(np.abs(ab.astype(np.float64) - a - b)) <= 1e-4 * rms
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nz,ny,nx,kw", [
+    (1, 1, 40, {}),                        # one row: columns of length 1
+    (1, 40, 1, {}),                        # one column: every pixel on the zero-weight edge
+    (1, 1, 1, {}),
+    (2, 16, 16, {}),                       # one strip covering the image
+    (2, 7, 9, {"tilt_axis_deg": 30.0}),
+    (3, 5, 3, {"tilt_axis_deg": 90.0}),
+])
+def test_ctf_degenerate_shapes(gpu, rel, nz, ny, nx, kw):
+    print(f"case: {nz}x{ny}x{nx} {kw}", flush=True)
+    P = np.random.default_rng(nz * 100 + ny + nx).normal(size=(nz, ny, nx)).astype(np.float32)
+    tilt = np.linspace(-30.0, 30.0, nz).astype(np.float32)
+    g = gpu.ctf_phaseflip(P, tilt, pixel_a=3.0, **kw)
+    width = gpu.ctf_strip_width(gpu.ctf_params(pixel_a=3.0, **kw), nx)
+    o = O.ctf_phaseflip(P, tilt, 3.0, strip_width=width, **kw)
+    assert rel(g, o) <= TOL
+    same = o == P
+    assert np.array_equal(g[same], P[same])
