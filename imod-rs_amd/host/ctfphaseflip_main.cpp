@@ -1,9 +1,11 @@
 // `ctfphaseflip` -- the drop-in front end of imod-rs apps/ctfphaseflip: its
 // option surface (main.rs:20-89) and file handling (main.rs:212-705), with the
 // strip transforms, the CTF flip and the blend run on the GPU through
-// libtiltgpu (tiltgpu_ctf_*, include/tiltgpu.h).  The stack is read into one
-// page-locked buffer, corrected on the device and written back section by
-// section with the reference's header statistics.
+// libtiltgpu (tiltgpu_ctf_*, include/tiltgpu.h).  The stack streams to the
+// device in chunks of sections (file reads overlapping uploads), is corrected
+// there, and streams back the same way while sections are written with the
+// reference's header statistics.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -14,6 +16,7 @@
 
 #include "../../include/tiltgpu.h"
 #include "mrc.h"
+#include "pipeline.h"
 #include "textio.h"
 
 using namespace tiltgpu_host;
@@ -30,6 +33,7 @@ struct Args {
     bool has_max = false;
     float amp = 0.07f, cuton = 0.0f, plate = 0.0f;
     bool gpu = false;
+    int upload_sections = 0;  // sections per transfer chunk; 0 = about 128 MB
 };
 
 [[noreturn]] void usage(const char* msg) {
@@ -51,7 +55,8 @@ struct Args {
                  "      --amplitude-contrast <w>  [0.07]\n"
                  "      --cuton-freq <1/A>        no correction below this frequency [0]\n"
                  "      --plate-phase <deg>       phase plate shift [0]\n"
-                 "      --gpu                     accepted (the correction always runs on the GPU)\n");
+                 "      --gpu                     accepted (the correction always runs on the GPU)\n"
+                 "      --upload-sections <n>     sections per file<->device transfer [about 128 MB]\n");
     std::exit(2);
 }
 
@@ -87,6 +92,7 @@ Args parse_args(int argc, char** argv) {
         {"cuton-freq", 0, 3, &a.cuton, nullptr},
         {"gpu", 0, 0, &a.gpu, nullptr},
         {"plate-phase", 0, 3, &a.plate, nullptr},
+        {"upload-sections", 0, 2, &a.upload_sections, nullptr},
     };
     for (int i = 1; i < argc; ++i) {
         std::string arg = argv[i];
@@ -238,36 +244,67 @@ int main(int argc, char** argv) {
     else
         std::fprintf(stderr, "ctfphaseflip: defocus=%.0fnm\n", a.defocus);
 
+    // the stack goes to the device in chunks of sections (about 128 MB) read
+    // into two page-locked buffers on a worker thread while the previous chunk
+    // uploads; the corrected sections come back the same way while the previous
+    // chunk is written with the header statistics (main.rs:690-704)
     const double t0 = now_s();
     const size_t plane = static_cast<size_t>(nx) * ny;
-    float* stack = static_cast<float*>(tiltgpu_host_alloc(static_cast<int64_t>(plane * nz * 4)));
-    if (!stack) {
-        std::fprintf(stderr, "Error: %s\n", tiltgpu_last_error());
-        return 1;
-    }
-    for (int z = 0; z < nz; ++z) {
-        size_t n = 0;
-        if (!reader.read_slice_f32_into(z, stack + z * plane, plane, &n, &err) || n != plane) {
-            std::fprintf(stderr, "Error: %s\n", err.empty() ? "unsupported MRC mode" : err.c_str());
+    const int chunk = std::max(
+        1, std::min(nz, a.upload_sections > 0 ? a.upload_sections
+                                              : static_cast<int>((128u << 20) / (plane * 4) + 1)));
+    const int nchunks = (nz + chunk - 1) / chunk;
+    float* bufs[2] = {nullptr, nullptr};
+    for (auto& b : bufs) {
+        b = static_cast<float*>(tiltgpu_host_alloc(static_cast<int64_t>(plane * chunk * 4)));
+        if (!b) {
+            std::fprintf(stderr, "Error: %s\n", tiltgpu_last_error());
             return 1;
         }
     }
-    const double t1 = now_s();
+    auto release = [&] {
+        for (auto& b : bufs) tiltgpu_host_free(b);
+    };
     tiltgpu_ctf* c = nullptr;
-    int rc = tiltgpu_ctf_create(nx, ny, nz, &p, tilts.data(), nt, vptr, n_vdef, &c);
-    if (rc == 0) rc = tiltgpu_ctf_upload(c, 0, nz, stack);
-    if (rc == 0) rc = tiltgpu_ctf_run(c);
-    if (rc == 0) rc = tiltgpu_ctf_download(c, 0, nz, stack);
-    double ms = 0;
-    if (rc == 0) tiltgpu_ctf_last_run(c, &ms, nullptr);
-    if (rc != 0) {
+    if (tiltgpu_ctf_create(nx, ny, nz, &p, tilts.data(), nt, vptr, n_vdef, &c) != 0) {
         std::fprintf(stderr, "ctfphaseflip: correction failed: %s\n", tiltgpu_last_error());
+        release();
         return 1;
     }
-    tiltgpu_ctf_destroy(c);
+    std::string perr;
+    bool ok = pipeline2(
+        nchunks, bufs,
+        [&](int k, float* buf, std::string* e) {
+            const int z0 = k * chunk, z1 = std::min(nz, z0 + chunk);
+            for (int z = z0; z < z1; ++z) {
+                size_t n = 0;
+                if (!reader.read_slice_f32_into(z, buf + (z - z0) * plane, plane, &n, e) ||
+                    n != plane) {
+                    if (e->empty()) *e = "unsupported MRC mode";
+                    return false;
+                }
+            }
+            return true;
+        },
+        [&](int k, float* buf, std::string* e) {
+            const int z0 = k * chunk, z1 = std::min(nz, z0 + chunk);
+            if (tiltgpu_ctf_upload(c, z0, z1 - z0, buf) != 0) {
+                *e = std::string("upload: ") + tiltgpu_last_error();
+                return false;
+            }
+            return true;
+        },
+        &perr);
+    const double t1 = now_s();
+    double ms = 0;
+    if (ok && tiltgpu_ctf_run(c) != 0) {
+        perr = std::string("correction failed: ") + tiltgpu_last_error();
+        ok = false;
+    }
+    if (ok) tiltgpu_ctf_last_run(c, &ms, nullptr);
     const double t2 = now_s();
 
-    // output header (main.rs:318-327) and per-section statistics (main.rs:690-704)
+    // output header (main.rs:318-327)
     MrcHeader oh = MrcHeader::make(nx, ny, nz, 2);
     oh.xlen = h.xlen;
     oh.ylen = h.ylen;
@@ -277,40 +314,54 @@ int main(int argc, char** argv) {
     oh.mz = h.mz;
     oh.add_label("ctfphaseflip: CTF phase correction (2D strips)");
     MrcWriter w;
-    if (!w.create(a.output, oh, &err)) {
-        std::fprintf(stderr, "Error: %s\n", err.c_str());
-        return 1;
-    }
+    if (ok && !w.create(a.output, oh, &perr)) ok = false;
     float gmin = 3.40282347e38f, gmax = -3.40282347e38f;
     double gsum = 0.0;
-    for (int z = 0; z < nz; ++z) {
-        const float* s = stack + z * plane;
-        float mn = 3.40282347e38f, mx = -3.40282347e38f;
-        double sum = 0.0;
-        for (size_t i = 0; i < plane; ++i) {
-            if (s[i] < mn) mn = s[i];
-            if (s[i] > mx) mx = s[i];
-            sum += static_cast<double>(s[i]);
-        }
-        const float mean = static_cast<float>(sum / static_cast<double>(plane));
-        if (mn < gmin) gmin = mn;
-        if (mx > gmax) gmax = mx;
-        gsum += static_cast<double>(mean) * static_cast<double>(plane);
-        if (!w.write_slice_f32(s, plane, &err)) {
-            std::fprintf(stderr, "Error: %s\n", err.c_str());
-            return 1;
-        }
-    }
-    if (!w.finish(gmin, gmax, static_cast<float>(gsum / (static_cast<double>(plane) * nz)), &err)) {
-        std::fprintf(stderr, "Error: %s\n", err.c_str());
+    if (ok)
+        ok = pipeline2(
+            nchunks, bufs,
+            [&](int k, float* buf, std::string* e) {
+                const int z0 = k * chunk, z1 = std::min(nz, z0 + chunk);
+                if (tiltgpu_ctf_download(c, z0, z1 - z0, buf) != 0) {
+                    *e = std::string("download: ") + tiltgpu_last_error();
+                    return false;
+                }
+                return true;
+            },
+            [&](int k, float* buf, std::string* e) {
+                const int z0 = k * chunk, z1 = std::min(nz, z0 + chunk);
+                for (int z = z0; z < z1; ++z) {
+                    const float* sec = buf + (z - z0) * plane;
+                    float mn = 3.40282347e38f, mx = -3.40282347e38f;
+                    double sum = 0.0;
+                    for (size_t i = 0; i < plane; ++i) {
+                        if (sec[i] < mn) mn = sec[i];
+                        if (sec[i] > mx) mx = sec[i];
+                        sum += static_cast<double>(sec[i]);
+                    }
+                    const float mean = static_cast<float>(sum / static_cast<double>(plane));
+                    if (mn < gmin) gmin = mn;
+                    if (mx > gmax) gmax = mx;
+                    gsum += static_cast<double>(mean) * static_cast<double>(plane);
+                    if (!w.write_slice_f32(sec, plane, e)) return false;
+                }
+                return true;
+            },
+            &perr);
+    tiltgpu_ctf_destroy(c);
+    release();
+    if (ok && !w.finish(gmin, gmax, static_cast<float>(gsum / (static_cast<double>(plane) * nz)),
+                        &perr))
+        ok = false;
+    if (!ok) {
+        std::fprintf(stderr, "Error: %s\n", perr.c_str());
         return 1;
     }
-    tiltgpu_host_free(stack);
     const double t3 = now_s();
     std::fprintf(stderr, "ctfphaseflip: corrected %d sections (2D strips)\n", nz);
     std::fprintf(stderr,
-                 "ctfphaseflip: wall time read %.3f s, upload+correction+read-back %.3f s "
-                 "(device %.3f s), write %.3f s\n",
+                 "ctfphaseflip: wall time read+upload %.3f s, correction %.3f s (device %.3f s), "
+                 "read-back+write %.3f s\n",
                  t1 - t0, t2 - t1, ms / 1000.0, t3 - t2);
     return 0;
 }

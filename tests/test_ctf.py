@@ -296,7 +296,8 @@ def test_ctf_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
     (tmp_path / "a.tlt").write_text("".join(f"{t:.2f}\n" for t in tilt))
     (tmp_path / "d.txt").write_text("0 -2500 -2800 20\n5 -3000 -3300 35\n")
     cmd = [_cli(), "-i", src, "-o", out, "-t", str(tmp_path / "a.tlt"), "--defocus-file",
-           str(tmp_path / "d.txt"), "--tilt-axis-angle", "10", "--plate-phase", "45", "--gpu"]
+           str(tmp_path / "d.txt"), "--tilt-axis-angle", "10", "--plate-phase", "45", "--gpu",
+           "--upload-sections", "4"]  # two chunks of 4 and 2 sections each way
     print("cmd:", " ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stderr, flush=True)
