@@ -219,7 +219,51 @@ def section_transforms_section(tg, P, cpu=True, reps=3):
     return res
 
 
-def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0):
+def ctf_end_to_end(P, tilt, pixel=4.0, axis=85.0):
+    """The `ctfphaseflip` command line on the stack from files (chunks of
+    sections read while the previous chunk uploads, read-back overlapped with
+    the MRC write).  Wall times as the command reports them; the files live in
+    a temporary directory that is removed afterwards."""
+    import shutil
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(REPO, "imod-rs_amd"))
+    import tiltgpu as tg  # noqa: E402
+    exe = os.path.join(REPO, "imod-rs_amd", "_build", "ctfphaseflip")
+    if not os.path.exists(exe):
+        return {"skipped": "ctfphaseflip executable not built"}
+    d = tempfile.mkdtemp(prefix="ctf_e2e_")
+    try:
+        need = 2 * P.nbytes + (1 << 30)
+        free = shutil.disk_usage(d).free
+        if free < need:
+            return {"skipped": f"{free / 1e9:.1f} GB free in {d}, {need / 1e9:.1f} GB needed"}
+        nz, ny, nx = P.shape
+        inp, out, tlt = (os.path.join(d, f) for f in ("ts.mrc", "ts_ctf.mrc", "ts.tlt"))
+        tg.mrc_write(inp, P, xlen=nx * pixel, ylen=ny * pixel, zlen=float(nz))
+        with open(tlt, "w") as fh:
+            fh.write("".join(f"{a:.9g}\n" for a in tilt))
+        cmd = [exe, "-i", inp, "-o", out, "-t", tlt, "--tilt-axis-angle", f"{axis:g}"]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        line = [l for l in r.stderr.splitlines() if l.startswith("ctfphaseflip: wall time")]
+        vals = [float(x) for x in re.findall(r"([0-9.]+) s", line[0])] if line else []
+        res = {"command": f"ctfphaseflip -i ts.mrc -o ts_ctf.mrc -t ts.tlt --tilt-axis-angle {axis:g}"
+                          f" ({nx}x{ny}x{nz}, {pixel} A pixels from the header)",
+               "process_wall_s": round(wall, 3), "bytes_each_way": int(os.path.getsize(inp))}
+        if len(vals) == 4:
+            res.update({"read_upload_s": vals[0], "correction_s": vals[1],
+                        "device_s": vals[2], "readback_write_s": vals[3],
+                        "mpixel_per_s_file_to_file": round(float(nx) * ny * nz / wall / 1e6, 1)})
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0, e2e=True):
     """ctfphaseflip on the C1 stack (2048x2048x121): 300 kV, Cs 2.7 mm, defocus
     -3000 nm, 4 A pixels (automatic strip width: 16 px), tilt axis at 0 deg (the
     reference's default: strips are column bands) and at 85 deg (bounding boxes
@@ -263,6 +307,12 @@ def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0):
             r["max_abs_err_over_rms_vs_oracle"] = float(f"{err_vs(g, ref)['max_abs_over_rms']:.3e}")
         out[f"axis_{axis:g}"] = r
         print(f"# ctfphaseflip axis {axis:g}: {r}", file=sys.stderr, flush=True)
+    if e2e:
+        try:
+            out["end_to_end"] = ctf_end_to_end(P, tilt, pixel)
+        except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
+            out["end_to_end"] = {"error": str(e)[:300]}
+        print(f"# ctfphaseflip end to end: {out['end_to_end']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -617,7 +667,8 @@ def main():
     P, ang, xt, local = c1_inputs(tg)
     print(f"# inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if a.ctf_only:
-        print(json.dumps({"ctfphaseflip": ctf_section(tg, P, cpu=not a.no_cpu)}), flush=True)
+        print(json.dumps({"ctfphaseflip": ctf_section(tg, P, cpu=not a.no_cpu, e2e=not a.no_e2e)}),
+              flush=True)
         return
     nx, ny, nv = P.shape[2], P.shape[1], P.shape[0]
     vv = float(nx) * ny * 512 * nv
@@ -708,7 +759,7 @@ def main():
         print(f"# end to end: {line['end_to_end']}", file=sys.stderr, flush=True)
     if not a.no_ctf:
         try:
-            line["ctfphaseflip"] = ctf_section(tg, P, cpu=not a.no_cpu)
+            line["ctfphaseflip"] = ctf_section(tg, P, cpu=not a.no_cpu, e2e=not a.no_e2e)
         except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
             line["ctfphaseflip"] = {"error": str(e)[:300]}
     if not a.no_sirt:
