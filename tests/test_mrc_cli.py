@@ -155,6 +155,46 @@ def test_cli_end_to_end_matches_oracle(gpu, tmp_path, rel):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("args,oopts,zf", [
+    (["--sirt", "2", "--sirt-relax", "0.7", "--fill-mode", "edge"],
+     dict(sirt=2, sirt_relax=0.7, fill_mode="edge"), True),
+    (["--log", "--hamming", "--radial", "0.3,0.1", "--exact-filter-size", "50",
+      "--cosine-interp", "cubic", "--max-cosine-tilt", "45", "-w", "80"],
+     dict(log=True, hamming=True, radial=(0.3, 0.1), exact_filter_size=50,
+          cosine_interp="cubic", max_cosine_tilt=45.0, width=80), False),
+    (["--no-weight", "--no-cosine-stretch", "--fill-mode", "zero", "--fake-sirt", "1",
+      "--outscale", "0.5"],
+     dict(no_weight=True, no_cosine_stretch=True, fill_mode="zero", fake_sirt=1,
+          outscale=0.5), False),
+])
+def test_cli_options_reach_the_library(gpu, tmp_path, rel, args, oopts, zf):
+    """Every option of the `tilt` surface (main.rs:18-120) is parsed and passed
+    on: each option set gives the oracle's reconstruction of the same file."""
+    ang = np.linspace(-55, 55, 17).astype(np.float32)
+    P = gpu.synth_series(64, 6, ang, vol_nz=24, n_ellipsoids=12, n_beads=3, seed=9,
+                         axis=(2.0, 7.0), bead_radius=2.0)
+    P = P - P.min() + 0.5  # positive, for --log
+    inp, out = str(tmp_path / "in.mrc"), str(tmp_path / "out.rec")
+    gpu.mrc_write(inp, P)
+    (tmp_path / "a.tlt").write_text("".join(f"{a:.6f}\n" for a in ang))
+    cmd = [TILT, "-i", inp, "-o", out, "-t", str(tmp_path / "a.tlt"), "-z", "24", *args]
+    zfac = None
+    if zf:
+        zfac = np.linspace(0.9, 1.1, 17).astype(np.float32)
+        (tmp_path / "zf").write_text("".join(f"{v:.6f}\n" for v in zfac))
+        cmd += ["--zfactor-file", str(tmp_path / "zf")]
+        zfac = O.load_per_view_file(str(tmp_path / "zf"))
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    vol = gpu.mrc_read(out)
+    ref, _ = O.tilt_run(P, O.read_tilt_file(str(tmp_path / "a.tlt")),
+                        O.TiltOptions(thickness=24, **oopts), zfactor=zfac)
+    assert vol.shape == ref.shape
+    assert rel(vol, ref) <= 1e-4
+
+
+@pytest.mark.gpu
 def test_cli_section_transforms(gpu, tmp_path, rel):
     """`tilt --xf-file` applies the newstack -x transforms on the GPU before the
     reconstruction: the result equals the oracle's reconstruction of the
