@@ -195,6 +195,29 @@ def test_cli_options_reach_the_library(gpu, tmp_path, rel, args, oopts, zf):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("storage,geometry", [("f16", "f64"), ("bf16", "f32")])
+def test_cli_storage_and_geometry_options(gpu, tmp_path, storage, geometry):
+    """The two additions, --storage and --geometry, reach the library: the file
+    written by `tilt` equals the in-memory reconstruction with the same options
+    bit for bit (chunked upload and slab read-back change nothing)."""
+    ang = np.linspace(-50, 50, 13).astype(np.float32)
+    P = gpu.synth_series(64, 5, ang, vol_nz=20, n_ellipsoids=10, n_beads=2, seed=4,
+                         axis=(2.0, 7.0), bead_radius=2.0)
+    inp, out = str(tmp_path / "in.mrc"), str(tmp_path / "out.rec")
+    gpu.mrc_write(inp, P)
+    (tmp_path / "a.tlt").write_text("".join(f"{a:.6f}\n" for a in ang))
+    cmd = [TILT, "-i", inp, "-o", out, "-t", str(tmp_path / "a.tlt"), "-z", "20",
+           "--storage", storage, "--geometry", geometry, "--upload-views", "5", "--slab-rows", "2"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    angs = np.array(O.read_tilt_file(str(tmp_path / "a.tlt")), np.float32)
+    ref, _ = gpu.reconstruct(P, angs, gpu.TiltOptions(thickness=20, storage=storage,
+                                                      geometry=geometry))
+    assert np.array_equal(gpu.mrc_read(out), ref)
+
+
+@pytest.mark.gpu
 def test_cli_section_transforms(gpu, tmp_path, rel):
     """`tilt --xf-file` applies the newstack -x transforms on the GPU before the
     reconstruction: the result equals the oracle's reconstruction of the
