@@ -175,8 +175,12 @@ __global__ void __launch_bounds__(kCtfThreads) ctf_cols_tile_kernel(CtfStrip s, 
     const int n = P.n;  // column length sh
     for (int i = threadIdx.x; i < n; i += kCtfThreads) tw_lds[i] = P.tw[i];
     const int ntile = (s.sw + C - 1) / C;
-    const int z = blockIdx.x / ntile;
-    const int x0 = (blockIdx.x - z * ntile) * C;
+    // Neighbouring tiles share 128-byte L2 lines (a tile row is C * 8 bytes):
+    // give each XCD a contiguous run of tiles so both halves of a line are
+    // fetched once, into the one L2 that reads them both.
+    const int t = xcd_remap(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    const int z = t / ntile;
+    const int x0 = (t - z * ntile) * C;
     float2* b0 = sm;
     float2* b1 = sm + C * n;
     float2* A = s.a + static_cast<long>(z) * s.sh * s.sw + x0;
