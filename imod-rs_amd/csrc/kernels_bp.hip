@@ -160,7 +160,9 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 // 4 conflict cycles per read.  Giving each group 16 consecutive x positions
 // makes its entries, at stride <= 1, at most 16 consecutive float4s: distinct
 // banks.  Lanes 0 and 63 keep x offsets 0 and 63 (the extremes view_setup
-// reads); a voxel's arithmetic does not depend on its lane.
+// reads); a voxel's arithmetic does not depend on its lane.  The one-row
+// kernel's ds_read_b64 is served in the consecutive halves 0-31 and 32-63,
+// conflict-free at any stride <= 1, so it keeps lane = x.
 __device__ __forceinline__ int b128_lane_x(int lane) {
     const int b = (lane >> 2) & 7;
     return (lane & 32) | ((__builtin_popcount(b) & 1) << 4) | ((b >> 1) << 2) | (lane & 3);
