@@ -485,12 +485,15 @@ __global__ void __launch_bounds__(kBpThreads)
                 // has passed the barrier of view p-1 since, so it is free.
                 const int buf = (p & 1) * P.lds_cap;
                 stage_store<TWO, ST>(P, cur, p, raw, xs, lds4, buf);
-                __syncthreads();
+                // the next view's loads go out before the barrier: they touch only
+                // registers and global memory, and their latency then overlaps the
+                // wait for the other waves' stores
                 Vs nxt = cur;
                 if (p + 1 < P.nviews) {
                     nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
                     stage_load<TWO, ST>(P, nxt, p + 1, raw, xs);
                 }
+                __syncthreads();
                 const TILTGPU_CONSTANT float* zc =
                     cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
                 if constexpr (TWO) {
