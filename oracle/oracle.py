@@ -77,6 +77,9 @@ def _declare(L):
             _fp, _fp, _fp, C.c_int, C.c_int, _fp, C.c_int]
         getattr(L, f"or_forward_project_simple_{suf}").argtypes = [
             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp, C.c_int]
+        getattr(L, f"or_forward_project_full_{suf}").argtypes = [
+            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int, _fp, C.c_int, _fp,
+            C.c_int, _ip, _ip, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int]
         getattr(L, f"or_rweight_float_{suf}").argtypes = [
             _fp, C.c_long, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int]
         getattr(L, f"or_cosine_stretch_float_{suf}").argtypes = [
@@ -484,6 +487,40 @@ def forward_project(V, ptab, in_nx, nz, prec="f32", nthreads=8):
     getattr(lib(), f"or_forward_project_simple_{prec}")(
         in_nx, ny, nz, nviews, out_nx, out_nz, _f(ptab), _f(V), _f(R), nthreads)
     return R
+
+
+def forward_project_full(V, angles, in_nx, nz, xtilt=None, zfactor=None, local=None,
+                         prec="f32", nthreads=8):
+    """forward_project (main.rs:830-896) of V [in_ny][out_nz][out_nx] with the whole
+    geometry: tilt angles, x-axis tilts, z factors and local grids (each optional).
+    Returns R [nz][in_ny][in_nx]."""
+    V = _f32c(V)
+    ny, out_nz, out_nx = V.shape
+    ang = _f32c(angles)
+    xt, zf = _f32c(xtilt), _f32c(zfactor)
+    if local is not None:
+        lnx, lny, lxs, lys, ldx, ldy = local.flat()
+        lp = (lnx.ctypes.data_as(_ip), lny.ctypes.data_as(_ip), _f(lxs), _f(lys),
+              _f(ldx), _f(ldy))
+    else:
+        lp = (None,) * 6
+    R = np.zeros((nz, ny, in_nx), np.float32)
+    getattr(lib(), f"or_forward_project_full_{prec}")(
+        in_nx, ny, nz, out_nx, out_nz, _f(ang), ang.size, _f(xt), 0 if xt is None else xt.size,
+        _f(zf), 0 if zf is None else zf.size, *lp, _f(V), _f(R), nthreads)
+    return R
+
+
+def lowpass_3d(V, prec="f32"):
+    """lowpass_3d_volume (main.rs:493-538): one clamped 3x3x3 box pass over
+    V [ny][nz][nx], computed in the given precision."""
+    dt = np.float32 if prec == "f32" else np.float64
+    a = np.array(V, dtype=dt, copy=True, order="C")
+    ny, nz, nx = a.shape
+    fn = getattr(lib(), f"or_lowpass_3d_{prec}")
+    fn.argtypes = [_fp if prec == "f32" else _dp, C.c_int, C.c_int, C.c_int]
+    fn(a.ctypes.data_as(_fp if prec == "f32" else _dp), ny, nz, nx)
+    return a.astype(np.float64)
 
 
 def local_interp(grids, view, x, y, prec="f32"):

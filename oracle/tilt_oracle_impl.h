@@ -556,6 +556,82 @@ void F(or_lowpass_3d)(REAL* vol, int ny, int nz, int nx) {
     free(prev);
 }
 
+/* The geometry of main.rs:682-705 (ProjParams for the first nviews views) plus
+ * the --local-file grids (main.rs:232-307, as flattened per-view arrays; lnx
+ * NULL = no file), filled into g.  tab: 5*nviews REALs; loff: 3*(nz+1) longs
+ * (the per-view offsets of the flattened grids) when lnx is given.  g->means
+ * is left NULL for the caller. */
+static void F(geom_init)(F(or_geom) * g, int in_nx, int in_ny, int nz, int out_nx, int out_nz,
+                         int fill_mode, const float* angles, int nviews, const float* xtilt,
+                         int n_xtilt, const float* zfac, int n_zfac, const int* lnx,
+                         const int* lny, const float* lxs, const float* lys, const float* ldx,
+                         const float* ldy, REAL* tab, long* loff) {
+    memset(g, 0, sizeof(*g));
+    F(or_proj_params)(angles, nviews, xtilt, n_xtilt, zfac, n_zfac, tab, tab + nviews,
+                      tab + 2 * nviews, tab + 3 * nviews, tab + 4 * nviews);
+    g->in_nx = in_nx;
+    g->in_ny = in_ny;
+    g->nz = nz;
+    g->nviews = nviews;
+    g->out_nx = out_nx;
+    g->out_nz = out_nz;
+    g->fill_mode = fill_mode;
+    g->cos_t = tab;
+    g->sin_t = tab + nviews;
+    g->cos_xt = tab + 2 * nviews;
+    g->sin_xt = tab + 3 * nviews;
+    g->zf = tab + 4 * nviews;
+    if (lnx) {
+        long *lxoff = loff, *lyoff = loff + nz + 1, *lgoff = loff + 2 * (nz + 1);
+        lxoff[0] = lyoff[0] = lgoff[0] = 0;
+        for (int v = 0; v < nz; v++) {
+            lxoff[v + 1] = lxoff[v] + lnx[v];
+            lyoff[v + 1] = lyoff[v] + lny[v];
+            lgoff[v + 1] = lgoff[v] + (long)lnx[v] * lny[v];
+        }
+        g->has_local = 1;
+        g->lnx = lnx;
+        g->lny = lny;
+        g->lxoff = lxoff;
+        g->lyoff = lyoff;
+        g->lgoff = lgoff;
+        g->lxs = lxs;
+        g->lys = lys;
+        g->ldx = ldx;
+        g->ldy = ldy;
+    }
+}
+
+/* forward_project (main.rs:830-896) of a caller-given volume V [in_ny][out_nz]
+ * [out_nx] with the full geometry: angles, x-axis tilts, z factors and local
+ * grids (any may be absent) into R [nz][in_ny][in_nx]. */
+void F(or_forward_project_full)(int in_nx, int in_ny, int nz, int out_nx, int out_nz,
+                                const float* angles, int n_angles, const float* xtilt,
+                                int n_xtilt, const float* zfac, int n_zfac, const int* lnx,
+                                const int* lny, const float* lxs, const float* lys,
+                                const float* ldx, const float* ldy, const float* V_in,
+                                float* R_out, int nthreads) {
+    int nviews = n_angles < nz ? n_angles : nz;
+    long nvol = (long)in_ny * out_nx * out_nz;
+    long total = (long)in_nx * in_ny * nz;
+    REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews > 0 ? nviews : 1));
+    long* loff = lnx ? (long*)calloc(3 * ((size_t)nz + 1), sizeof(long)) : NULL;
+    REAL zero = 0;
+    F(or_geom) g;
+    F(geom_init)(&g, in_nx, in_ny, nz, out_nx, out_nz, 1, angles, nviews, xtilt, n_xtilt,
+                 zfac, n_zfac, lnx, lny, lxs, lys, ldx, ldy, tab, loff);
+    g.means = &zero;
+    REAL* V = (REAL*)malloc(sizeof(REAL) * (size_t)(nvol ? nvol : 1));
+    for (long i = 0; i < nvol; i++) V[i] = (REAL)V_in[i];
+    REAL* R = (REAL*)malloc(sizeof(REAL) * (size_t)(total ? total : 1));
+    F(or_forward_project)(&g, V, R, nthreads);
+    for (long i = 0; i < total; i++) R_out[i] = (float)R[i];
+    free(R);
+    free(V);
+    free(loff);
+    free(tab);
+}
+
 /* ------------------------------------------------------------- pipeline -- */
 
 /*
@@ -636,44 +712,11 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
                              o->max_cosine_tilt);
     int nviews = n_angles < o->nz ? n_angles : o->nz;
     REAL* tab = (REAL*)malloc(sizeof(REAL) * 5 * (size_t)(nviews > 0 ? nviews : 1));
-    F(or_proj_params)(angles, nviews, xtilt, n_xtilt, zfac, n_zfac, tab, tab + nviews,
-                      tab + 2 * nviews, tab + 3 * nviews, tab + 4 * nviews);
+    long* loff = lnx ? (long*)calloc(3 * ((size_t)o->nz + 1), sizeof(long)) : NULL;
     F(or_geom) g;
-    memset(&g, 0, sizeof(g));
-    g.in_nx = o->in_nx;
-    g.in_ny = o->in_ny;
-    g.nz = o->nz;
-    g.nviews = nviews;
-    g.out_nx = o->out_nx;
-    g.out_nz = o->out_nz;
-    g.fill_mode = o->fill_mode;
-    g.cos_t = tab;
-    g.sin_t = tab + nviews;
-    g.cos_xt = tab + 2 * nviews;
-    g.sin_xt = tab + 3 * nviews;
-    g.zf = tab + 4 * nviews;
+    F(geom_init)(&g, o->in_nx, o->in_ny, o->nz, o->out_nx, o->out_nz, o->fill_mode, angles,
+                 nviews, xtilt, n_xtilt, zfac, n_zfac, lnx, lny, lxs, lys, ldx, ldy, tab, loff);
     g.means = means;
-    long *lxoff = NULL, *lyoff = NULL, *lgoff = NULL;
-    if (lnx) {
-        lxoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
-        lyoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
-        lgoff = (long*)calloc((size_t)o->nz + 1, sizeof(long));
-        for (int v = 0; v < o->nz; v++) {
-            lxoff[v + 1] = lxoff[v] + lnx[v];
-            lyoff[v + 1] = lyoff[v] + lny[v];
-            lgoff[v + 1] = lgoff[v] + (long)lnx[v] * lny[v];
-        }
-        g.has_local = 1;
-        g.lnx = lnx;
-        g.lny = lny;
-        g.lxoff = lxoff;
-        g.lyoff = lyoff;
-        g.lgoff = lgoff;
-        g.lxs = lxs;
-        g.lys = lys;
-        g.ldx = ldx;
-        g.ldy = ldy;
-    }
     int out_ny = o->in_ny;
     long nv = (long)o->out_nx * o->out_nz;
     int whole = o->sirt > 0 || o->fake_sirt > 0;
@@ -721,9 +764,7 @@ void F(or_tilt_run_rows)(const or_opts* o, float* P_in, const float* angles, int
     free(means);
     free(P);
     free(O);
-    free(lxoff);
-    free(lyoff);
-    free(lgoff);
+    free(loff);
 }
 
 /* SIRT (main.rs:919-953) on output rows [y0, y1) only, keeping the volume after

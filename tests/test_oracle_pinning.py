@@ -603,3 +603,165 @@ def test_ctf_strip_width_rule():
     assert O.ctf_strip_width(256, 2048, 20.0, lam, -3000.0) == int(20.0 / (lam * 30000 / 2048))
     assert O.ctf_strip_width(256, 2048, 2.0, lam, -3000.0, max_strip_width=100) == 100
     assert O.ctf_strip_width(256, 2048, 2.0, lam, 0.0) == 256
+
+
+# ------------------------------- two-row geometry, local grids, fill modes --
+
+def _geometry_fixture():
+    g = np.load(os.path.join(GOLDEN, "geometry_bruteforce.npz"))
+    return g, [k.split("/")[0] for k in g.files if k.endswith("/angles")]
+
+
+def _case_inputs(g, name):
+    nz = int(g["nz"])
+    pts = g[f"{name}/patches"]
+    grids = None
+    if len(pts):
+        grids = O.build_local_grids([(int(q[0]), q[1], q[2], q[3], q[4]) for q in pts], nz)
+    return g[f"{name}/angles"], g[f"{name}/xtilt"], g[f"{name}/zf"], grids
+
+
+@pytest.mark.parametrize("fill", ["mean", "zero", "edge"])
+def test_two_row_geometry_bp_bruteforce_fixture(fill):
+    """The oracle's backprojection with x-axis tilt, z factors and local grids,
+    under each fill mode, equals an FP64 sparse-operator brute force that shares
+    no code with it (tests/golden/make_golden.py: tent weights, numpy.interp grid
+    interpolation and Edge clamp): the row selection with the last-row fallback
+    and the y-blend (main.rs:1051-1066), the fills (:386-406), the local shifts
+    (:171-228)."""
+    g, cases = _geometry_fixture()
+    assert sorted(cases) == ["grid3x3", "xtilt", "xtilt_zf_local1"]
+    P = g["P"]
+    opts = O.TiltOptions(thickness=int(g["out_nz"]), width=int(g["out_nx"]), no_weight=True,
+                         no_cosine_stretch=True, fill_mode=fill)
+    for name in cases:
+        ang, xt, zf, grids = _case_inputs(g, name)
+        ref = g[f"{name}/bp_{fill}"]
+        rms = np.sqrt(np.mean(ref ** 2))
+        v64, _ = O.tilt_run(P, ang, opts, xtilt=xt, zfactor=zf, local=grids, prec="f64")
+        v32, _ = O.tilt_run(P, ang, opts, xtilt=xt, zfactor=zf, local=grids, prec="f32")
+        assert np.max(np.abs(v64 - ref)) <= 1e-6 * rms, (name, fill)
+        assert np.max(np.abs(v32 - ref)) <= 1e-5 * rms, (name, fill)
+
+
+def test_two_row_geometry_fp_is_the_transposed_operator():
+    """The oracle's forward projection with x-axis tilt, z factors and local grids
+    is the transpose of the zero-fill sampling operator (without 1/nz), entry by
+    entry: the four two-row scatter weights and the one-row fallback of
+    main.rs:866-889."""
+    g, cases = _geometry_fixture()
+    nx, nz = int(g["nx"]), int(g["nz"])
+    for name in cases:
+        ang, xt, zf, grids = _case_inputs(g, name)
+        ref = g[f"{name}/fp"]
+        scale = np.max(np.abs(ref))
+        r64 = O.forward_project_full(g["Vin"], ang, nx, nz, xtilt=xt, zfactor=zf, local=grids,
+                                     prec="f64")
+        r32 = O.forward_project_full(g["Vin"], ang, nx, nz, xtilt=xt, zfactor=zf, local=grids,
+                                     prec="f32")
+        assert np.max(np.abs(r64 - ref)) <= 1e-6 * scale, name
+        assert np.max(np.abs(r32 - ref)) <= 1e-5 * scale, name
+
+
+def test_two_row_fixture_hits_every_rule():
+    """The fixture's cases reach each branch the rules name: both rows, the
+    one-row fallback at the last row, a first row with nothing added, samples
+    left and right of the row, a view with y offset 0 on the x-tilt case and a
+    view with no patches on the grid case."""
+    g, _ = _geometry_fixture()
+    ny, nx, out_nx, out_nz = int(g["ny"]), int(g["nx"]), int(g["out_nx"]), int(g["out_nz"])
+    ang = g["xtilt/angles"].astype(np.float64)
+    xt = np.deg2rad(g["xtilt/xtilt"].astype(np.float64))
+    dx = np.arange(out_nx) - out_nx / 2.0
+    dz = np.arange(out_nz) - out_nz / 2.0
+    sy_last = (ny - 1) + dx * np.sin(xt[0])
+    assert np.any(np.floor(sy_last) == ny - 1) and np.any(np.floor(sy_last) == ny - 2)
+    assert np.any(np.floor(0 + dx * np.sin(xt[0])) == -1)
+    px = (dx[None, :] * np.cos(np.deg2rad(ang[0])) + dz[:, None] * np.sin(np.deg2rad(ang[0]))
+          + nx / 2.0)
+    assert np.any(px < 0) and np.any(np.floor(px) >= nx - 1)
+    assert g["xtilt/xtilt"][3] == 0.0
+    assert not np.any(g["grid3x3/patches"][:, 0] == 2)
+    # the fills differ where samples fall outside the row, so each fill is seen
+    a, b, c = (g[f"xtilt/bp_{f}"] for f in ("mean", "zero", "edge"))
+    assert not np.allclose(a, b) and not np.allclose(a, c) and not np.allclose(b, c)
+
+
+def test_two_row_adjoint_identity_with_local_grid():
+    """<FP V, S> = nz <V, BP_zero-fill S> for the two-row geometry with a local
+    grid, in FP64 (main.rs:866-889 against :1040-1066)."""
+    g, _ = _geometry_fixture()
+    ang, xt, zf, grids = _case_inputs(g, "xtilt_zf_local1")
+    nz, ny, nx = int(g["nz"]), int(g["ny"]), int(g["nx"])
+    rng = np.random.default_rng(77)
+    S = rng.normal(size=(nz, ny, nx)).astype(np.float32)
+    V = g["Vin"]
+    R = O.forward_project_full(V, ang, nx, nz, xtilt=xt, zfactor=zf, local=grids, prec="f64")
+    opts = O.TiltOptions(thickness=int(g["out_nz"]), width=int(g["out_nx"]), no_weight=True,
+                         no_cosine_stretch=True, fill_mode="zero")
+    B, _ = O.tilt_run(S, ang, opts, xtilt=xt, zfactor=zf, local=grids, prec="f64")
+    lhs = float(np.sum(R.astype(np.float64) * S))
+    rhs = nz * float(np.sum(V.astype(np.float64) * B))
+    assert lhs == pytest.approx(rhs, rel=1e-6)
+
+
+# ------------------------------------------------------------- fake-SIRT --
+
+def _box3_clamped(V):
+    """One separable 3x3x3 box pass with clamped (replicated) edges, the textbook
+    form: scipy.ndimage.uniform_filter with mode 'nearest'."""
+    from scipy.ndimage import uniform_filter
+    return uniform_filter(np.asarray(V, np.float64), size=3, mode="nearest")
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_lowpass_impulse_response_and_edges(prec, tol):
+    """lowpass_3d_volume (main.rs:493-538) on unit impulses: 1/27 on the 3x3x3
+    cube around an interior voxel; at a corner the clamped edge gives weights
+    (2/3, 1/3) per axis.  A ramp along each axis is kept inside and pulled in by
+    1/3 of a step at both ends."""
+    ny, nz, nx = 5, 6, 7
+    V = np.zeros((ny, nz, nx))
+    V[2, 3, 3] = 1.0
+    out = O.lowpass_3d(V, prec=prec)
+    exp = np.zeros_like(V)
+    exp[1:4, 2:5, 2:5] = 1.0 / 27.0
+    assert np.max(np.abs(out - exp)) < tol
+    V = np.zeros((ny, nz, nx))
+    V[0, 0, 0] = 1.0
+    w = np.array([2.0 / 3.0, 1.0 / 3.0])
+    out = O.lowpass_3d(V, prec=prec)
+    exp = np.zeros_like(V)
+    exp[:2, :2, :2] = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    assert np.max(np.abs(out - exp)) < tol
+    for axis, n in ((0, ny), (1, nz), (2, nx)):
+        shape = [1, 1, 1]
+        shape[axis] = n
+        ramp = np.broadcast_to(np.arange(n, dtype=np.float64).reshape(shape), (ny, nz, nx))
+        out = O.lowpass_3d(np.array(ramp), prec=prec)
+        exp = np.array(ramp, np.float64)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[axis] = 0
+        hi[axis] = n - 1
+        exp[tuple(lo)] += 1.0 / 3.0
+        exp[tuple(hi)] -= 1.0 / 3.0
+        assert np.max(np.abs(out - exp)) < tol * n, axis
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_fake_sirt_equals_box_filtered_wbp(passes):
+    """--fake-sirt N (main.rs:976-1001) is N clamped 3x3x3 box passes of the WBP
+    volume, before the output transform."""
+    rng = np.random.default_rng(19)
+    P = rng.normal(size=(7, 5, 24)).astype(np.float32)
+    ang = np.linspace(-45, 45, 7).astype(np.float32)
+    wbp, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=9), prec="f64")
+    ref = wbp.astype(np.float64)
+    for _ in range(passes):
+        ref = _box3_clamped(ref)
+    got64, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=9, fake_sirt=passes), prec="f64")
+    got32, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=9, fake_sirt=passes), prec="f32")
+    rms = np.sqrt(np.mean(ref ** 2))
+    assert np.max(np.abs(got64 - ref)) < 1e-6 * rms
+    assert np.max(np.abs(got32 - ref)) < 1e-5 * rms
