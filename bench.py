@@ -105,9 +105,49 @@ def oracle_rows_f64(P, ang, xt, local, rows=ERR_ROWS, halo=24):
     grids = O.build_local_grids(patches, nv)
     opts = O.TiltOptions(thickness=512, radial=(0.35, 0.05))
     vol, _ = O.tilt_run(sub, ang, opts, xtilt=xt, local=grids, prec="f64",
-                        nthreads=min(16, os.cpu_count() or 1), rows=(halo, halo + (y1 - y0)),
+                        nthreads=host_threads(), rows=(halo, halo + (y1 - y0)),
                         means=view_means(P))
     return vol
+
+
+def oracle_rows_f32(P, ang, xt, local, rows=ERR_ROWS):
+    """The FP32 oracle's output rows [rows) of the WBP workload, run on the whole
+    stack so that every coordinate (iy + dx*sin(xt) + shift) rounds as in the
+    whole reconstruction: the parity checker at this size."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402 -- checker only
+    nv = P.shape[0]
+    patches = [(v, float(local[2][v]), float(local[3][v]), float(local[4][v]),
+                float(local[5][v])) for v in range(nv)]
+    grids = O.build_local_grids(patches, nv)
+    opts = O.TiltOptions(thickness=512, radial=(0.35, 0.05))
+    vol, _ = O.tilt_run(P, ang, opts, xtilt=xt, local=grids, prec="f32",
+                        nthreads=host_threads(), rows=rows)
+    return vol
+
+
+def host_threads():
+    """Threads for the host-side oracle work: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when it is set (the machine's share for one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(cap))) if cap.isdigit() and int(cap) > 0 else max(1, n)
+
+
+def host_description():
+    """nproc, the usable CPU count and the CPU model of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"nproc": os.cpu_count(), "usable_cpus": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
 
 def view_means(P):
@@ -121,6 +161,25 @@ def err_vs(x, ref):
     rms = float(np.sqrt(np.mean(ref ** 2)))
     return {"max_abs_over_rms": float(f"{np.max(np.abs(d)) / rms:.3g}"),
             "rms_over_rms": float(f"{np.sqrt(np.mean(d ** 2)) / rms:.3g}")}
+
+
+def evict_from_page_cache(path):
+    """Write the file back and drop its pages from the page cache, so the next
+    read comes from the device (fsync + POSIX_FADV_DONTNEED on the file; no
+    privileges needed).  Returns whether the advice was taken."""
+    fd = os.open(path, os.O_RDONLY)
+    try:
+        os.fsync(fd)
+        if hasattr(os, "posix_fadvise"):
+            os.posix_fadvise(fd, 0, 0, os.POSIX_FADV_DONTNEED)
+            return True
+        return False
+    finally:
+        os.close(fd)
+
+
+PAGE_CACHE_NOTE = ("input written, synced and evicted from the page cache before the run "
+                   "(a cold read); output written through the page cache, not synced")
 
 
 def end_to_end_section(tg, P, ang, xt, local):
@@ -150,6 +209,7 @@ def end_to_end_section(tg, P, ang, xt, local):
         with open(os.path.join(d, "local.txt"), "w") as fh:
             fh.write("".join(f"{v} {local[2][v]:.9g} {local[3][v]:.9g} {local[4][v]:.9g} "
                              f"{local[5][v]:.9g}\n" for v in range(nv)))
+        cold = evict_from_page_cache(inp)
         cmd = [tilt, "-i", inp, "-o", out, "-t", os.path.join(d, "c1.tlt"), "-z", "512",
                "--radial", "0.35,0.05", "--xtilt-file", os.path.join(d, "xtilt.txt"),
                "--local-file", os.path.join(d, "local.txt")]
@@ -163,6 +223,7 @@ def end_to_end_section(tg, P, ang, xt, local):
         res = {"command": "tilt -i c1.mrc -o c1.rec -t c1.tlt -z 512 --radial 0.35,0.05 "
                           "--xtilt-file xtilt.txt --local-file local.txt",
                "process_wall_s": round(wall, 3),
+               "page_cache": PAGE_CACHE_NOTE if cold else "hot (eviction advice unavailable)",
                "input_bytes": int(os.path.getsize(inp)), "output_bytes": int(os.path.getsize(out))}
         if len(vals) == 4:
             res.update({"read_upload_s": vals[0], "reconstruction_s": vals[1],
@@ -243,6 +304,7 @@ def ctf_end_to_end(P, tilt, pixel=4.0, axis=85.0):
         tg.mrc_write(inp, P, xlen=nx * pixel, ylen=ny * pixel, zlen=float(nz))
         with open(tlt, "w") as fh:
             fh.write("".join(f"{a:.9g}\n" for a in tilt))
+        cold = evict_from_page_cache(inp)
         cmd = [exe, "-i", inp, "-o", out, "-t", tlt, "--tilt-axis-angle", f"{axis:g}"]
         t0 = time.perf_counter()
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
@@ -253,7 +315,9 @@ def ctf_end_to_end(P, tilt, pixel=4.0, axis=85.0):
         vals = [float(x) for x in re.findall(r"([0-9.]+) s", line[0])] if line else []
         res = {"command": f"ctfphaseflip -i ts.mrc -o ts_ctf.mrc -t ts.tlt --tilt-axis-angle {axis:g}"
                           f" ({nx}x{ny}x{nz}, {pixel} A pixels from the header)",
-               "process_wall_s": round(wall, 3), "bytes_each_way": int(os.path.getsize(inp))}
+               "process_wall_s": round(wall, 3),
+               "page_cache": PAGE_CACHE_NOTE if cold else "hot (eviction advice unavailable)",
+               "bytes_each_way": int(os.path.getsize(inp))}
         if len(vals) == 4:
             res.update({"read_upload_s": vals[0], "correction_s": vals[1],
                         "device_s": vals[2], "readback_write_s": vals[3],
@@ -316,13 +380,10 @@ def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0, e2e=True):
     return out
 
 
-def cpu_baseline(P, ang, xt, local, rows=32):
-    """The FP32 oracle (the reference's algorithm restated in C, its threading
-    model) on a bounded slab: output rows [1008, 1040) of the same workload,
-    from the projection rows they can touch."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import oracle as O  # noqa: E402 -- CPU-baseline leg only
-    nthreads = min(16, os.cpu_count() or 1)
+def _cpu_band(O, P, ang, xt, local, rows, nthreads):
+    """One timed FP32-oracle run: output rows [ny/2 - rows/2, +rows) of the WBP
+    workload from the projection rows they can touch (+-24 halo), with the whole
+    images' means.  Returns Gvoxel*view/s and seconds."""
     ny = P.shape[1]
     y0 = ny // 2 - rows // 2
     halo = 24  # |dx*sin(0.5 deg)| <= 9 rows plus |shift| of a few rows
@@ -338,11 +399,27 @@ def cpu_baseline(P, ang, xt, local, rows=32):
     O.tilt_run(sub, ang, opts, xtilt=xt, local=grids, prec="f32", nthreads=nthreads,
                rows=(halo, halo + rows), means=means)
     dt = time.perf_counter() - t0
-    vv = rows * 512 * P.shape[2] * nv
-    return {"value": vv / dt / 1e9, "unit": "Gvoxel·view/s", "cores": nthreads,
+    return rows * 512 * P.shape[2] * nv / dt / 1e9, dt
+
+
+def cpu_baseline(P, ang, xt, local, rows=32, rows_1t=4):
+    """The FP32 oracle (the reference's algorithm restated in C, its threading
+    model: rows of a chunk spread over the threads) on bounded slabs of the same
+    workload: `rows` output rows on every thread this process may use, and
+    `rows_1t` rows on one thread."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O  # noqa: E402 -- CPU-baseline leg only
+    nthreads = host_threads()
+    v, dt = _cpu_band(O, P, ang, xt, local, rows, nthreads)
+    v1, dt1 = _cpu_band(O, P, ang, xt, local, rows_1t, 1)
+    return {"value": v, "unit": "Gvoxel·view/s", "cores": nthreads,
             "kind": "port",
             "sample": f"{rows} output rows (of 2048) of the same WBP workload, "
-                      f"projection rows +-{halo} halo, filter+stretch+BP, {dt:.1f} s"}
+                      f"projection rows +-24 halo, filter+stretch+BP, {dt:.1f} s on "
+                      f"{nthreads} threads",
+            "one_thread": {"value": v1, "cores": 1,
+                           "sample": f"{rows_1t} output rows, {dt1:.1f} s"},
+            "host": host_description()}
 
 
 # --------------------------------------------------------------------- SIRT
@@ -382,7 +459,7 @@ def sirt_section(tg, iters=20, rows=(504, 520), oracle=True):
         import oracle as O  # noqa: E402 -- checker for the error figures only
         t1 = time.perf_counter()
         ref, ref_res = O.sirt_trace(P, ang, O.TiltOptions(thickness=th, sirt=iters), rows,
-                                    prec="f64", nthreads=min(16, os.cpu_count() or 1))
+                                    prec="f64", nthreads=host_threads())
         out["oracle_f64_seconds"] = round(time.perf_counter() - t1, 2)
         print(f"# FP64 oracle trace: {out['oracle_f64_seconds']} s", file=sys.stderr, flush=True)
     for storage in ("f32", "bf16"):
@@ -706,8 +783,9 @@ def main():
         },
         "roofline": {
             "kernel": "bp_kernel (two-row, local)",
-            "bound": "mfma",
-            "pipe": "FP32 vector ALU (gather/interpolation, no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
+            "bound": "fp32_valu",
+            "pipe": "FP32 vector ALU: gather/interpolation, no matrix work, no MFMA; "
+                    "peak = MI355X FP32 vector 157.3 TFLOP/s",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -743,6 +821,16 @@ def main():
             err[f"{st}_storage_f64_geometry"] = err_vs(r, ref64)
         line["error_vs_f64_oracle"] = err
         print(f"# error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        ref32 = oracle_rows_f32(P, ang, xt, local)
+        err32 = {"subset": f"output rows [{ERR_ROWS[0]}, {ERR_ROWS[1]}) of 2048, FP32 oracle "
+                           f"on the whole stack {time.perf_counter() - t1:.1f} s",
+                 "bar": "max_abs_over_rms <= 1e-4 for FP32 storage",
+                 "f32_storage": err_vs(rows32, ref32)}
+        if rows16 is not None:
+            err32["f16_storage"] = err_vs(rows16, ref32)
+        line["error_vs_f32_oracle"] = err32
+        print(f"# error vs FP32 oracle: {err32}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(P, ang, xt, local)
     if not a.no_xf:
@@ -767,7 +855,17 @@ def main():
     if not a.no_serve:
         line["serving"] = serving_section(tg, a.serve_requests, a.stream_requests,
                                           c1=(P, ang, xt, local))
+    line["child_processes_at_exit"] = child_processes()
     print(json.dumps(line), flush=True)
+
+
+def child_processes():
+    """Processes this run started that are still alive (should be none)."""
+    try:
+        import psutil
+        return [f"{c.pid} {c.name()}" for c in psutil.Process().children(recursive=True)]
+    except Exception as e:  # noqa: BLE001 -- diagnostic only
+        return [f"unknown: {e}"[:80]]
 
 
 if __name__ == "__main__":
