@@ -73,24 +73,10 @@ float stretch_factor(float deg, float max_tilt) {
 
 int pick_pairs_per_wg(int n) { return std::max(1, 2048 / std::max(1, n)); }
 
-// Upper bound on the LDS entries one backprojection tile needs for a view.
-void tile_bounds(const tiltgpu_session* s, int view, const tiltgpu_view_params& g, int tile_z,
+// Upper bound on the LDS entries one backprojection tile needs for a view whose
+// local shifts span ldx_span x ldy_span pixels.
+void tile_bounds(const tiltgpu_view_params& g, int tile_z, double ldx_span, double ldy_span,
                  int* seg, int* rows) {
-    double ldx_span = 0, ldy_span = 0;
-    if (s->has_local && s->lnx[view] > 0 && s->lny[view] > 0) {
-        const int ng = s->lnx[view] * s->lny[view];
-        const float* gx = s->ldx.data() + s->lgoff[view];
-        const float* gy = s->ldy.data() + s->lgoff[view];
-        float xmn = gx[0], xmx = gx[0], ymn = gy[0], ymx = gy[0];
-        for (int i = 1; i < ng; ++i) {
-            xmn = std::min(xmn, gx[i]);
-            xmx = std::max(xmx, gx[i]);
-            ymn = std::min(ymn, gy[i]);
-            ymx = std::max(ymx, gy[i]);
-        }
-        ldx_span = static_cast<double>(xmx) - xmn;
-        ldy_span = static_cast<double>(ymx) - ymn;
-    }
     const double sx = std::fabs(static_cast<double>(g.cos_t) * g.cos_xt) * (kBpTileX - 1) +
                       std::fabs(static_cast<double>(g.sin_t) * g.zfactor) * (tile_z - 1) +
                       ldx_span + 4.0;
@@ -99,59 +85,102 @@ void tile_bounds(const tiltgpu_session* s, int view, const tiltgpu_view_params& 
     *rows = static_cast<int>(std::min(sy, 1e9));
 }
 
-int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& vp) {
-    const int nv = static_cast<int>(vp.size());
-    s->nviews = nv;
-    s->two_row = s->has_local;
+// Extent of a view's local-alignment shifts (0 without a grid).
+static void local_spans(const tiltgpu_session* s, int view, double* ldx_span, double* ldy_span) {
+    *ldx_span = *ldy_span = 0;
+    if (!s || !s->has_local || s->lnx[view] <= 0 || s->lny[view] <= 0) return;
+    const int ng = s->lnx[view] * s->lny[view];
+    const float* gx = s->ldx.data() + s->lgoff[view];
+    const float* gy = s->ldy.data() + s->lgoff[view];
+    float xmn = gx[0], xmx = gx[0], ymn = gy[0], ymx = gy[0];
+    for (int i = 1; i < ng; ++i) {
+        xmn = std::min(xmn, gx[i]);
+        xmx = std::max(xmx, gx[i]);
+        ymn = std::min(ymn, gy[i]);
+        ymx = std::max(ymx, gy[i]);
+    }
+    *ldx_span = static_cast<double>(xmx) - xmn;
+    *ldy_span = static_cast<double>(ymx) - ymn;
+}
+
+bool needs_two_rows(const tiltgpu_view_params* vp, int nv, bool has_local) {
     // (sin_xt == 0 implies cos_xt == 1 exactly; the second test keeps the
     // single-row kernels, which rely on it, safe for any table)
-    for (const auto& g : vp)
-        if (g.sin_xt != 0.0f || g.cos_xt != 1.0f) s->two_row = true;
+    bool two = has_local;
+    for (int v = 0; v < nv; ++v)
+        if (vp[v].sin_xt != 0.0f || vp[v].cos_xt != 1.0f) two = true;
+    return two;
+}
+
+int choose_bp_tile(const tiltgpu_session* local_src, const tiltgpu_view_params* vp, int nv,
+                   bool two_row, bool has_local, bool half, int in_nx, int out_nz, int forced,
+                   BpTileChoice* out) {
     // tile depth: minimise padded depth x (1 + per-view overhead / voxels per lane);
     // LDS capacity from the per-view bounds
     const int max_lds_bytes = 64 * 1024;
-    int chosen = 0;
-    const int out_nz0 = s->d.out_nz;
-    std::vector<int> order = {256, 128, 64, 32};
+    int order[4] = {256, 128, 64, 32};
+    int norder = 4;
     auto cost = [&](int tz) {
-        const double pad = static_cast<double>((out_nz0 + tz - 1) / tz * tz);
+        const double pad = static_cast<double>((out_nz + tz - 1) / tz * tz);
         return pad * (1.0 + 32.0 / tz);
     };
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
-    s->tile_source = kTuneHeuristic;
-    const int tuned = tuned_tile_z(s->two_row ? 1 : 0, s->has_local ? 1 : 0,
-                                   s->o.storage != TILTGPU_STORE_F32 ? 1 : 0, s->d.in_nx, out_nz0);
+    for (int i = 1; i < 4; ++i)  // stable insertion sort by cost (no temporary buffer)
+        for (int j = i; j > 0 && cost(order[j]) < cost(order[j - 1]); --j)
+            std::swap(order[j], order[j - 1]);
+    out->source = kTuneHeuristic;
+    const int tuned = tuned_tile_z(two_row ? 1 : 0, has_local ? 1 : 0, half ? 1 : 0, in_nx, out_nz);
     if (tuned > 0) {
         // the measured depth first; the others remain as LDS fallbacks
-        order.erase(std::remove(order.begin(), order.end(), tuned), order.end());
-        order.insert(order.begin(), tuned);
-        s->tile_source = kTuneTable;
+        int k = 0;
+        int rest[4];
+        for (int i = 0; i < 4; ++i)
+            if (order[i] != tuned) rest[k++] = order[i];
+        order[0] = tuned;
+        for (int i = 0; i < k; ++i) order[1 + i] = rest[i];
+        norder = 1 + k;
+        out->source = kTuneTable;
     }
-    const int forced = s->tune_tile_z > 0 ? s->tune_tile_z : s->tile_z_override;
     if (forced > 0) {
-        order = {forced};
-        s->tile_source = kTuneForced;
+        order[0] = forced;
+        norder = 1;
+        out->source = kTuneForced;
     }
-    for (int tz : order) {
+    for (int i = 0; i < norder; ++i) {
+        const int tz = order[i];
         int cap = 1;
         for (int v = 0; v < nv; ++v) {
             int seg, rows;
-            tile_bounds(s, v, vp[v], tz, &seg, &rows);
-            const int need = s->two_row ? seg * rows : seg;
+            double lx, ly;
+            local_spans(has_local ? local_src : nullptr, v, &lx, &ly);
+            tile_bounds(vp[v], tz, lx, ly, &seg, &rows);
+            const int need = two_row ? seg * rows : seg;
             cap = std::max(cap, need);
         }
         cap = (cap + 63) / 64 * 64;
-        if (bp_lds_bytes(cap, s->two_row) <= static_cast<size_t>(max_lds_bytes)) {
-            s->tile_z = tz;
-            s->lds_cap = cap;
-            chosen = 1;
-            break;
+        if (bp_lds_bytes(cap, two_row) <= static_cast<size_t>(max_lds_bytes)) {
+            out->tile_z = tz;
+            out->lds_cap = cap;
+            return TILTGPU_OK;
         }
     }
-    if (!chosen)
-        return fail(TILTGPU_ERR_ARG,
-                    "geometry needs more LDS per backprojection tile than supported "
-                    "(local-alignment shifts vary too much within 64 pixels)");
+    return fail(TILTGPU_ERR_ARG,
+                "geometry needs more LDS per backprojection tile than supported "
+                "(local-alignment shifts vary too much within 64 pixels)");
+}
+
+int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& vp) {
+    const int nv = static_cast<int>(vp.size());
+    s->nviews = nv;
+    s->two_row = needs_two_rows(vp.data(), nv, s->has_local);
+    const int forced = s->tune_tile_z > 0 ? s->tune_tile_z : s->tile_z_override;
+    BpTileChoice tc{};
+    int rc0 = choose_bp_tile(s, vp.data(), nv, s->two_row, s->has_local,
+                             s->o.storage != TILTGPU_STORE_F32, s->d.in_nx, s->d.out_nz, forced,
+                             &tc);
+    if (rc0) return rc0;
+    s->tile_z = tc.tile_z;
+    s->lds_cap = tc.lds_cap;
+    s->tile_source = tc.source;
     const int out_nz = s->d.out_nz;
     s->nz_pad = (out_nz + s->tile_z - 1) / s->tile_z * s->tile_z;
     std::vector<float4> g4(std::max(nv, 1));
@@ -916,6 +945,37 @@ void* tiltgpu_host_alloc(int64_t bytes) {
 
 void tiltgpu_host_free(void* p) {
     if (p) (void)hipHostFree(p);
+}
+
+void* tiltgpu_device_alloc(int64_t bytes) {
+    if (bytes <= 0) {
+        fail(TILTGPU_ERR_ARG, "bad size");
+        return nullptr;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, static_cast<size_t>(bytes));
+    if (e != hipSuccess) {
+        fail(TILTGPU_ERR_HIP, "hipMalloc(%lld bytes) failed: %s", static_cast<long long>(bytes),
+             hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+
+void tiltgpu_device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int tiltgpu_copy_to_device(void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+    return TILTGPU_OK;
+}
+
+int tiltgpu_copy_to_host(void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    HIP_TRY(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+    return TILTGPU_OK;
 }
 
 int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {

@@ -228,6 +228,19 @@ inline void collect_timing(tiltgpu_session* s) {
 
 
 int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& vp);
+// Backprojection tile depth and LDS capacity of a geometry (allocates nothing):
+// the tuned table's depth for the kernel variant, else the padded-depth rule,
+// falling back to shallower tiles until the per-view LDS bound fits 64 KiB;
+// forced > 0 pins the depth.  local_src supplies the local-shift extents when
+// has_local.
+struct BpTileChoice {
+    int tile_z, lds_cap, source;
+};
+int choose_bp_tile(const tiltgpu_session* local_src, const tiltgpu_view_params* vp, int nv,
+                   bool two_row, bool has_local, bool half, int in_nx, int out_nz, int forced,
+                   BpTileChoice* out);
+// Whether a view table needs the two-row backprojection (x-axis tilt or local shifts).
+bool needs_two_rows(const tiltgpu_view_params* vp, int nv, bool has_local);
 int upload_local(tiltgpu_session* s);
 int prepare_pipeline_geometry(tiltgpu_session* s);
 int prepare_filter(tiltgpu_session* s);

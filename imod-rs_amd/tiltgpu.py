@@ -121,6 +121,10 @@ SIGNATURES = [
     ("tiltgpu_upload_projections", C.c_int, [_vp, _fp, C.c_int64]),
     ("tiltgpu_upload_views", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
     ("tiltgpu_host_alloc", C.c_void_p, [C.c_int64]),
+    ("tiltgpu_device_alloc", C.c_void_p, [C.c_int64]),
+    ("tiltgpu_device_free", None, [C.c_void_p]),
+    ("tiltgpu_copy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("tiltgpu_copy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("tiltgpu_host_free", None, [C.c_void_p]),
     ("tiltgpu_run", C.c_int, [_vp]),
     ("tiltgpu_run_rows", C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -146,6 +150,10 @@ SIGNATURES = [
                                          C.c_int32, C.c_int32, C.POINTER(_vp)]),
     ("tiltgpu_reconstruct_row", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_reconstruct_rows", C.c_int, [_vp, C.c_int32, C.c_int32, _fp]),
+    ("tiltgpu_reconstruct_rows_noalloc", C.c_int, [_vp, C.POINTER(ViewParams), _fp, C.c_int32,
+                                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                   C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                   _vp, _vp]),
     ("tiltgpu_server_create", C.c_int, [C.c_int64, C.POINTER(_vp)]),
     ("tiltgpu_server_submit", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
                                         C.c_int32, _fp, _fp, C.POINTER(C.c_int64)]),
@@ -554,6 +562,49 @@ class GpuSession:
             self.close()
         except Exception:
             pass
+
+
+class DeviceArray:
+    """A float32 device buffer from tiltgpu_device_alloc (for the device-pointer
+    entry points): upload from / download into numpy arrays."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = lib().tiltgpu_device_alloc(max(4, self.nbytes))
+        if not self.ptr:
+            raise TiltGpuError(ERR_HIP, lib().tiltgpu_last_error().decode(errors="replace"))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.nbytes == self.nbytes
+        _check(lib().tiltgpu_copy_to_device(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, np.float32)
+        _check(lib().tiltgpu_copy_to_host(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().tiltgpu_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def reconstruct_rows_noalloc(d_proj, params, proj_means, in_nx, in_ny, n_projs, out_nx, out_nz,
+                             y0, y1, d_out, fill_mode="mean"):
+    """tiltgpu_reconstruct_rows_noalloc on DeviceArrays: d_proj [n_projs][in_ny][in_nx],
+    d_out [y1-y0][out_nz][out_nx]; params a ViewParams array (proj_params)."""
+    m = _f32(proj_means)
+    _check(lib().tiltgpu_reconstruct_rows_noalloc(
+        d_proj.ptr, params, _f(m), len(params), in_nx, in_ny, n_projs, out_nx, out_nz,
+        FILL[fill_mode], y0, y1, d_out.ptr, None))
 
 
 def read_tilt_file(path, cap=100000):

@@ -161,6 +161,15 @@ int tiltgpu_upload_projections(tiltgpu_session* s, const float* host, int64_t co
  * pinned-host rate.  No counterpart in the reference: added for streaming. */
 int tiltgpu_upload_views(tiltgpu_session* s, int32_t v0, int32_t nv, const float* host);
 
+/* Device buffers for callers without an allocator of their own (and for the
+ * device-pointer entry points' tests): tiltgpu_device_alloc returns NULL on
+ * failure (tiltgpu_last_error says why).  The copies are synchronous.  No
+ * counterpart in the reference. */
+void* tiltgpu_device_alloc(int64_t bytes);
+void tiltgpu_device_free(void* p);
+int tiltgpu_copy_to_device(void* dst, const void* src, int64_t bytes);
+int tiltgpu_copy_to_host(void* dst, const void* src, int64_t bytes);
+
 /* Page-locked host memory for upload and read-back buffers (NULL on failure,
  * tiltgpu_last_error says why); free with tiltgpu_host_free.  No counterpart
  * in the reference. */
@@ -260,6 +269,26 @@ int tiltgpu_bp_session_new(const float* projections, const tiltgpu_view_params* 
 int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out);
 /* The same for rows [y0, y1) in one launch. */
 int tiltgpu_reconstruct_rows(tiltgpu_session* s, int32_t y0, int32_t y1, float* out);
+
+/* Allocation-free backprojection.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The computation of
+ * tiltgpu_bp_session_new + tiltgpu_reconstruct_rows (bit-identical results) on
+ * caller-owned device buffers: d_projections [nz][in_ny][in_nx] (filtered and
+ * stretched) and d_out [y1-y0][out_nz][out_nx]; params [n_views] and proj_means
+ * [nz] are small host tables.  stream is a hipStream_t (NULL: the default
+ * stream); the call returns when the backprojection has finished.  The per-view
+ * tables live in a static scratch array of TILTGPU_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends: geometries needing more (about n_views x
+ * padded thickness x 4 bytes) are rejected with TILTGPU_ERR_ARG.  It calls no
+ * hipMalloc or hipFree.  Calls on one GPU share that scratch: calls that overlap
+ * on one GPU (from several threads or streams) race. */
+#define TILTGPU_NOALLOC_SCRATCH_BYTES (8 << 20)
+int tiltgpu_reconstruct_rows_noalloc(const float* d_projections,
+                                     const tiltgpu_view_params* params, const float* proj_means,
+                                     int32_t n_views, int32_t in_nx, int32_t in_ny, int32_t nz,
+                                     int32_t out_nx, int32_t out_nz, int32_t fill_mode, int32_t y0,
+                                     int32_t y1, float* d_out, void* stream);
 
 /* ------------------------------------------------- continuous batching */
 /* A server owns the device through one worker thread.  Requests from any host
