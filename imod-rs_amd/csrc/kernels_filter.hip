@@ -15,6 +15,9 @@
 // (-ffp-contract=off) and writes the row coalesced.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include <cmath>
 
 #include "fft_lds.h"
@@ -310,8 +313,33 @@ __global__ void __launch_bounds__(256) filter_batch_kernel(FilterBatch B) {
     filter_body(B.descs[a], b - B.block_start[a]);
 }
 
+// Rows wider than 4096 on the mixed-radix path need more than the default 64 KiB
+// of dynamic LDS per workgroup (two buffers of n complex values); a kernel opts
+// in to the size a launch needs the first time it is needed.
+static hipError_t allow_large_lds(const void* kernel, size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
+    if (lds > kFilterMaxLds) return hipErrorInvalidValue;
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> granted;
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& g : granted)
+        if (g.first == kernel && g.second >= lds) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    for (auto& g : granted)
+        if (g.first == kernel) {
+            g.second = lds;
+            return hipSuccess;
+        }
+    granted.emplace_back(kernel, lds);
+    return hipSuccess;
+}
+
 hipError_t launch_filter_batch(const FilterBatch& b, int total_blocks, size_t lds, hipStream_t st) {
     if (total_blocks <= 0) return hipSuccess;
+    const hipError_t e = allow_large_lds(reinterpret_cast<const void*>(filter_batch_kernel), lds);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(filter_batch_kernel, dim3(total_blocks), dim3(256), lds, st, b);
     return hipGetLastError();
 }
@@ -364,6 +392,8 @@ hipError_t launch_filter(const FilterParams& p, hipStream_t st) {
     const int nblk = (total_pairs + p.pairs_per_wg - 1) / p.pairs_per_wg;
     if (nblk <= 0) return hipSuccess;
     const size_t lds = filter_lds_bytes(p.n, p.pairs_per_wg);
+    const hipError_t e = allow_large_lds(reinterpret_cast<const void*>(filter_kernel), lds);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(filter_kernel, dim3(nblk), dim3(256), lds, st, p);
     return hipGetLastError();
 }

@@ -343,8 +343,17 @@ int prepare_filter(tiltgpu_session* s) {
     const int n = s->d.in_nx;
     if (filter_plan(n, s->radix, &s->nradix) != 0)
         return fail(TILTGPU_ERR_ARG, "projection width %d is not supported by the LDS FFT", n);
-    if (filter_lds_bytes(n, pick_pairs_per_wg(n)) > 160 * 1024)
-        return fail(TILTGPU_ERR_ARG, "projection width %d too large for the LDS FFT", n);
+    size_t lds_limit = kFilterMaxLds;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, s->device) ==
+            hipSuccess &&
+        optin > 0)
+        lds_limit = std::min(lds_limit, static_cast<size_t>(optin));
+    if (filter_lds_bytes(n, pick_pairs_per_wg(n)) > lds_limit)
+        return fail(TILTGPU_ERR_ARG,
+                    "projection width %d too large for the LDS FFT (its rows need %zu bytes of "
+                    "LDS per workgroup, the device allows %zu)",
+                    n, filter_lds_bytes(n, pick_pairs_per_wg(n)), lds_limit);
     std::vector<float> w(n);
     int rc = tiltgpu_radial_weights(n, s->o.exact_filter_size, s->o.radial_cutoff,
                                     s->o.radial_falloff, s->o.hamming, w.data());

@@ -192,6 +192,10 @@ hipError_t launch_filter(const FilterParams& p, hipStream_t st);
 hipError_t launch_filter_batch(const FilterBatch& b, int total_blocks, size_t lds, hipStream_t st);
 int filter_blocks(const FilterParams& p);
 size_t filter_lds_bytes(int n, int pairs_per_wg);
+// Largest dynamic LDS a filter workgroup may take: the 160 KiB of a gfx950 CU
+// less 256 bytes for the kernels' own static LDS (rows up to 10224 wide on the
+// mixed-radix path).
+constexpr size_t kFilterMaxLds = 160 * 1024 - 256;
 int filter_plan(int n, int* radix, int* nradix);  // 0 ok, -1 unsupported n
 
 hipError_t launch_log_means(const float* proj, float* logd, int64_t nxy, int nz, int do_log,

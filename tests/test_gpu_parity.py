@@ -80,6 +80,22 @@ def test_wbp_mixed_radix_widths(gpu, rel, nx):
     assert rel(g, o) <= TOL
 
 
+@pytest.mark.parametrize("nx", [4100, 5760, 10200])
+def test_wbp_wide_rows(gpu, rel, nx):
+    """Rows wider than 4096 take the mixed-radix filter with more than 64 KiB of
+    LDS per workgroup (159.4 KiB at 10200)."""
+    P, ang = case(gpu, nx=nx, ny=3, nviews=7, thick=16, seed=nx, n_ell=40)
+    g, _, o, _ = both(gpu, P, ang, gpu.TiltOptions(thickness=16))
+    assert rel(g, o) <= TOL
+
+
+def test_wbp_width_beyond_lds_is_refused(gpu):
+    P = np.zeros((2, 1, 10240), np.float32)  # 160 KiB of dynamic LDS: over the limit
+    with pytest.raises(gpu.TiltGpuError) as ei:
+        gpu.reconstruct(P, np.zeros(2, np.float32), gpu.TiltOptions(thickness=4))
+    assert ei.value.code == gpu.ERR_ARG and "too large" in str(ei.value)
+
+
 def test_wbp_odd_rows_and_fewer_angles(gpu, rel):
     # in_ny odd: FFT row pairs straddle views; 25 angles for 27 sections
     P, ang = case(gpu, ny=7, nviews=27, thick=40, seed=3)
