@@ -619,33 +619,46 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
     print(f"# serving inputs generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     out = {"config": {"requests": n_req, "distinct_series": len(series),
                       "mix": "even WBP / odd SIRT-10; widths 256-1024, 41-121 views, z 64-300",
-                      "budget_bytes": budget}}
-    # batched capacity (all requests at once), FP32 and FP16 storage
-    for storage in ("f32", "f16"):
+                      "budget_bytes": budget,
+                      "scheduler": "one round in flight; arrivals during it join the next round; "
+                                   "backprojection launches merge tile depths"}}
+    # batched capacity (all requests at once), FP32 and FP16 storage; FP32 also with
+    # one backprojection launch per tile depth (the launches before merging)
+    runs = [("f32", None), ("f16", None), ("f32", "0")]
+    for storage, merge in runs:
         reqs = c3_requests(series, n_req, storage)
         work = sum(r[3] for r in reqs)
-        dt, st = serve_batch(tg, reqs, budget)
-        out[f"batched_{storage}"] = {
+        if merge is not None:
+            os.environ["TILTGPU_SERVER_MERGE_TILES"] = merge
+        try:
+            dt, st = serve_batch(tg, reqs, budget)
+        finally:
+            os.environ.pop("TILTGPU_SERVER_MERGE_TILES", None)
+        key = f"batched_{storage}" + ("_launch_per_tile_depth" if merge == "0" else "")
+        out[key] = {
             "seconds": round(dt, 3), "requests_per_s": round(n_req / dt, 3),
             "gvoxel_view_per_s": round(work / dt / 1e9, 2),
             "rounds": st["rounds"], "bp_launches": st["bp_launches"],
             "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2),
             "max_active": st["max_active"]}
-        print(f"# batched {storage}: {dt:.2f} s", file=sys.stderr, flush=True)
+        print(f"# {key}: {dt:.2f} s", file=sys.stderr, flush=True)
     reqs = c3_requests(series, n_req, "f32")
     work = sum(r[3] for r in reqs)
     t1 = time.perf_counter()
     for P, a, o, _ in reqs:
         plain_run(tg, P, a, o)
     dtp = time.perf_counter() - t1
-    out["plain_f32_sequential"] = {"seconds": round(dtp, 3), "requests_per_s": round(n_req / dtp, 3),
+    plain_cap = n_req / dtp
+    out["plain_f32_sequential"] = {"seconds": round(dtp, 3), "requests_per_s": round(plain_cap, 3),
                                    "gvoxel_view_per_s": round(work / dtp / 1e9, 2)}
     print(f"# plain sequential: {dtp:.2f} s", file=sys.stderr, flush=True)
     if not streams:
         return out
     cap = out["batched_f32"]["requests_per_s"]
-    # served request streams at 20 / 60 / 90 % of batched capacity, with slab
-    # requests of the resident C1 series mixed in when it is given
+    # served request streams at 30 / 60 / 90 % of the batched capacity, with slab
+    # requests of the resident C1 series mixed in when it is given; the plain
+    # baseline at the 30 % rate (below its own capacity) and at 90 % of its own
+    # capacity, where its queue is still stable
     slab_session = None
     if c1 is not None:
         P1, ang1, xt1, loc1 = c1
@@ -656,32 +669,57 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
         slab_session.set_local(*loc1)
         slab_session.upload(P1)
         slab_session.run_rows(0, 64)  # filters the resident series once
-    stream = []
-    for load in (0.2, 0.6, 0.9):
-        jobs = stream_jobs(c3_requests(series, n_stream, "f32"), c1 is not None, seed=int(load * 100))
-        n_recon = sum(1 for j in jobs if j[0] == "recon")
-        # offered load counts reconstructions; slabs ride along
-        arr = poisson_times(load * cap * len(jobs) / n_recon, len(jobs), seed=int(load * 100))
-        lat_s, st = serve_stream(tg, jobs, arr, budget, slab_series=c1)
-        lat_p = plain_stream(tg, jobs, arr, slab_session=slab_session)
-        kinds = np.array([j[0] for j in jobs])
-        entry = {"load": load, "offered_requests_per_s": round(load * cap, 3),
-                 "jobs": {"reconstructions": n_recon, "slabs": int(np.sum(kinds == "slab"))},
-                 "served": {"p50_ms": round(pct(lat_s, 50), 2), "p99_ms": round(pct(lat_s, 99), 2),
-                            "mean_requests_per_bp_launch": round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2)},
-                 "plain_f32": {"p50_ms": round(pct(lat_p, 50), 2), "p99_ms": round(pct(lat_p, 99), 2)}}
+
+    def summary(lat, kinds, st=None):
+        e = {"p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+             "p99_over_p50": round(pct(lat, 99) / pct(lat, 50), 2)}
+        rec = kinds == "recon"
+        e["reconstruction_p50_ms"] = round(pct(lat[rec], 50), 2)
+        e["reconstruction_p99_ms"] = round(pct(lat[rec], 99), 2)
+        if st is not None:
+            e["mean_requests_per_bp_launch"] = round(st["bp_batched_requests"] / max(1, st["bp_launches"]), 2)
+            e["rounds"] = st["rounds"]
         if c1 is not None:
             sl = kinds == "slab"
-            entry["served"]["slab_p50_ms"] = round(pct(lat_s[sl], 50), 2)
-            entry["served"]["slab_p99_ms"] = round(pct(lat_s[sl], 99), 2)
-            entry["plain_f32"]["slab_p50_ms"] = round(pct(lat_p[sl], 50), 2)
-            entry["plain_f32"]["slab_p99_ms"] = round(pct(lat_p[sl], 99), 2)
-        stream.append(entry)
-        print(f"# stream load {load}: served p50 {pct(lat_s, 50):.1f} p99 {pct(lat_s, 99):.1f} ms; "
-              f"plain p50 {pct(lat_p, 50):.1f} p99 {pct(lat_p, 99):.1f} ms", file=sys.stderr, flush=True)
+            e["slab_p50_ms"] = round(pct(lat[sl], 50), 2)
+            e["slab_p99_ms"] = round(pct(lat[sl], 99), 2)
+        return e
+
+    def stream(rate, seed):
+        jobs = stream_jobs(c3_requests(series, n_stream, "f32"), c1 is not None, seed=seed)
+        n_recon = sum(1 for j in jobs if j[0] == "recon")
+        # offered rate counts reconstructions; slabs ride along
+        arr = poisson_times(rate * len(jobs) / n_recon, len(jobs), seed=seed)
+        return jobs, arr, np.array([j[0] for j in jobs])
+
+    served = []
+    for load in (0.3, 0.6, 0.9):
+        jobs, arr, kinds = stream(load * cap, int(load * 100))
+        lat_s, st = serve_stream(tg, jobs, arr, budget, slab_series=c1)
+        entry = {"load": load, "offered_requests_per_s": round(load * cap, 3),
+                 "jobs": {"reconstructions": int(np.sum(kinds == "recon")),
+                          "slabs": int(np.sum(kinds == "slab"))},
+                 "served": summary(lat_s, kinds, st)}
+        served.append(entry)
+        print(f"# served stream load {load}: {entry['served']}", file=sys.stderr, flush=True)
+    out["stream"] = served
+    plain = []
+    for name, rate, seed in (("at_served_30pct_rate", 0.3 * cap, 30),
+                             ("at_90pct_of_plain_capacity", 0.9 * plain_cap, 91)):
+        if rate >= plain_cap:
+            continue
+        jobs, arr, kinds = stream(rate, seed)
+        lat_p = plain_stream(tg, jobs, arr, slab_session=slab_session)
+        entry = {"name": name, "offered_requests_per_s": round(rate, 3),
+                 "fraction_of_plain_capacity": round(rate / plain_cap, 3),
+                 "plain_f32": summary(lat_p, kinds)}
+        if name == "at_served_30pct_rate":
+            entry["served_same_jobs"] = served[0]["served"]
+        plain.append(entry)
+        print(f"# plain stream {name}: {entry['plain_f32']}", file=sys.stderr, flush=True)
+    out["plain_stream"] = plain
     if slab_session is not None:
         slab_session.close()
-    out["stream"] = stream
     out["matched_accuracy"] = "served results are bit-identical to plain per-request FP32 runs"
     return out
 
@@ -698,7 +736,8 @@ def main():
     ap.add_argument("--serve-batch-only", action="store_true",
                     help="only the batched and plain runs of the serving section (for profiles)")
     ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--stream-requests", type=int, default=48)
+    ap.add_argument("--stream-requests", type=int, default=150,
+                    help="reconstructions per stream (a slab request rides along every third)")
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
     ap.add_argument("--no-xf", action="store_true",
                     help="skip the newstack -x section-transform measurement")

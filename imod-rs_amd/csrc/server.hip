@@ -8,13 +8,14 @@
 //   forward projection (SIRT iterations) -> filter (new series: R-weighting +
 //   stretch; SIRT: residual O - R) -> backprojection (persistent, per-XCD
 //   tile queues over the tiles of all requests).
-// Requests that arrive while rounds are on the device join the next round.  A
-// WBP request needs one round, a SIRT-k request k+1, a slab request one.
+// Requests that arrive while a round is on the device join the next round: a
+// new round is issued only when fewer than max_inflight rounds are in flight
+// (one by default), so rounds coalesce arrivals.  A WBP request needs one
+// round, a SIRT-k request k+1, a slab request one.
 //
-// The host never waits for a round before issuing the next one (up to kSlots
-// rounds are in flight): admission work -- buffers from a device-buffer cache
-// that never calls hipFree on the hot path, geometry tables, the projection
-// upload on a stream of its own -- overlaps the rounds already on the device.
+// Admission work -- buffers from a device-buffer cache that never calls hipFree
+// on the hot path, geometry tables, the projection upload on a stream of its
+// own -- overlaps the round on the device.
 // Finisher threads wait for each round, copy finished results to the callers'
 // buffers (one copy stream each) and return their buffers to the cache.
 //
@@ -25,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -45,7 +47,7 @@ using Clock = std::chrono::steady_clock;
 
 enum { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3 };
 enum { kRecon = 0, kSlab = 1, kSeries = 2 };
-constexpr int kSlots = 3;  // rounds in flight
+constexpr int kSlots = 3;  // round slots: at most this many rounds in flight
 constexpr size_t kAdmitMax = 8;              // admissions per round issued
 constexpr int64_t kAdmitBytes = 2ll << 30;   // projection bytes uploaded per round issued
 
@@ -227,6 +229,16 @@ constexpr int kFinishers = 4;  // result copies of pageable buffers proceed in p
 
 struct tiltgpu_server {
     int device = 0;
+    // Rounds allowed on the device at once (1..kSlots).  A new round is issued
+    // only when fewer are in flight; requests arriving meanwhile are admitted
+    // and join it, so rounds carry every request that arrived while the device
+    // was busy.  TILTGPU_SERVER_ROUNDS overrides.
+    int max_inflight = 1;
+    // Backprojection launches group requests by storage and geometry variant and
+    // run at the group's smallest tile depth (the depths are powers of two, so a
+    // request prepared for a deeper tile divides evenly; results do not depend
+    // on the depth).  TILTGPU_SERVER_MERGE_TILES=0 keeps one launch per depth.
+    bool merge_tiles = true;
     int n_cu = 256;
     int64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;     // rounds
@@ -244,6 +256,7 @@ struct tiltgpu_server {
     int finishing = 0;         // requests handed to the finisher, not yet released
     Slot slots[kSlots];
     uint64_t round_seq = 0;
+    uint64_t completed_seq = 0;  // newest round known to have completed (scheduler)
     std::deque<FinItem> fin;
     std::vector<hipEvent_t> events;  // spare events (under mu)
     DevCache cache;
@@ -536,19 +549,28 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
             Req* sr = S->series[r->series];
             add(r, sr->s, r->y0, r->y1, 1, r->slab);
         }
-        std::stable_sort(items.begin(), items.end(), [](const BpItem& a, const BpItem& b) {
-            return std::make_tuple(a.storage, a.two_row, a.local, a.tile_z) <
-                   std::make_tuple(b.storage, b.two_row, b.local, b.tile_z);
-        });
+        const bool merge = S->merge_tiles;
+        auto key = [merge](const BpItem& a) {
+            return std::make_tuple(a.storage, a.two_row, a.local, merge ? 0 : a.tile_z);
+        };
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const BpItem& a, const BpItem& b) { return key(a) < key(b); });
         std::vector<std::vector<BpItem>> groups;
         size_t i = 0;
         while (i < items.size()) {
             size_t j = i;
             std::vector<BpItem> grp;
-            while (j < items.size() && items[j].storage == items[i].storage &&
-                   items[j].two_row == items[i].two_row && items[j].local == items[i].local &&
-                   items[j].tile_z == items[i].tile_z)
+            int tz = items[i].tile_z;
+            while (j < items.size() && key(items[j]) == key(items[i])) {
+                tz = std::min(tz, items[j].tile_z);
                 grp.push_back(items[j++]);
+            }
+            // every item runs at the group's depth: its padded thickness is a
+            // multiple of it, and its LDS bound for a deeper tile covers it
+            for (BpItem& it : grp) {
+                it.p.ntz = it.p.nz_pad / tz;
+                it.tile_z = tz;
+            }
             groups.push_back(std::move(grp));
             i = j;
         }
@@ -665,6 +687,28 @@ void worker_main(tiltgpu_server* S) {
             S->max_active = std::max<int>(S->max_active, static_cast<int>(act.size()));
         }
         if (act.empty()) continue;
+        // Coalescing: while max_inflight rounds are on the device, wait for the
+        // oldest, then admit what arrived meanwhile before issuing the next round.
+        {
+            int inflight = 0;
+            Slot* oldest = nullptr;
+            for (Slot& q : S->slots) {
+                if (!q.seq || q.seq <= S->completed_seq) continue;
+                if (hipEventQuery(q.done) == hipSuccess) {
+                    S->completed_seq = std::max(S->completed_seq, q.seq);
+                    S->cache.set_done(q.seq);
+                    continue;
+                }
+                ++inflight;
+                if (!oldest || q.seq < oldest->seq) oldest = &q;
+            }
+            if (inflight >= S->max_inflight && oldest) {
+                (void)hipEventSynchronize(oldest->done);
+                S->completed_seq = std::max(S->completed_seq, oldest->seq);
+                S->cache.set_done(oldest->seq);
+                continue;
+            }
+        }
         Slot& sl = S->slots[S->round_seq % kSlots];
         if (sl.seq) {
             (void)hipEventSynchronize(sl.done);
@@ -806,6 +850,38 @@ int fill_recon(Req* r, const tiltgpu_dims* d, const tiltgpu_options* o, const fl
     return TILTGPU_OK;
 }
 
+// Copies a request's x-axis tilts, z factors and local grids (each optional).
+int fill_geometry(Req* r, int nz, const tiltgpu_view_geometry* g) {
+    if (!g) return TILTGPU_OK;
+    if (g->xtilt && g->n_xtilt > 0) {
+        r->xtilt.assign(g->xtilt, g->xtilt + g->n_xtilt);
+        r->has_xtilt = true;
+    }
+    if (g->zfactor && g->n_zfactor > 0) {
+        r->zfac.assign(g->zfactor, g->zfactor + g->n_zfactor);
+        r->has_zfac = true;
+    }
+    if (g->lnx && g->lny) {
+        r->lnx.assign(g->lnx, g->lnx + nz);
+        r->lny.assign(g->lny, g->lny + nz);
+        int64_t nxs = 0, nys = 0, ng = 0;
+        for (int v = 0; v < nz; ++v) {
+            if (g->lnx[v] < 0 || g->lny[v] < 0) return fail(TILTGPU_ERR_ARG, "negative grid size");
+            nxs += g->lnx[v];
+            nys += g->lny[v];
+            ng += static_cast<int64_t>(g->lnx[v]) * g->lny[v];
+        }
+        if ((nxs && !g->lxs) || (nys && !g->lys) || (ng && (!g->ldx || !g->ldy)))
+            return fail(TILTGPU_ERR_ARG, "local grid arrays missing");
+        r->lxs.assign(g->lxs, g->lxs + nxs);
+        r->lys.assign(g->lys, g->lys + nys);
+        r->ldx.assign(g->ldx, g->ldx + ng);
+        r->ldy.assign(g->ldy, g->ldy + ng);
+        r->has_local = true;
+    }
+    return TILTGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -823,6 +899,11 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, S->device) == hipSuccess) S->n_cu = prop.multiProcessorCount;
     S->cache.cap = budget_bytes + budget_bytes / 4;
+    if (const char* e = std::getenv("TILTGPU_SERVER_ROUNDS")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= kSlots) S->max_inflight = v;
+    }
+    if (const char* e = std::getenv("TILTGPU_SERVER_MERGE_TILES")) S->merge_tiles = std::atoi(e) != 0;
     bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&S->up_stream, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& st : S->down_stream)
@@ -848,10 +929,19 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out) {
 int tiltgpu_server_submit(tiltgpu_server* S, const tiltgpu_dims* d, const tiltgpu_options* o,
                           const float* angles, int32_t n_angles, const float* projections,
                           float* out, int64_t* id) {
+    return tiltgpu_server_submit_geometry(S, d, o, angles, n_angles, nullptr, projections, out,
+                                          id);
+}
+
+int tiltgpu_server_submit_geometry(tiltgpu_server* S, const tiltgpu_dims* d,
+                                   const tiltgpu_options* o, const float* angles,
+                                   int32_t n_angles, const tiltgpu_view_geometry* geom,
+                                   const float* projections, float* out, int64_t* id) {
     if (!S || !out) return fail(TILTGPU_ERR_ARG, "bad arguments");
     std::unique_ptr<Req> tmp(new Req());
     int rc = fill_recon(tmp.get(), d, o, angles, n_angles, projections);
     if (rc) return rc;
+    if ((rc = fill_geometry(tmp.get(), d->nz, geom))) return rc;
     if (!tmp->o.no_cosine_stretch && n_angles < d->nz)
         return fail(TILTGPU_ERR_ARG, "cosine stretch needs one tilt angle per section");
     std::lock_guard<std::mutex> lk(S->mu);
@@ -877,26 +967,8 @@ int tiltgpu_server_add_series(tiltgpu_server* S, const tiltgpu_dims* d, const ti
     if (rc) return rc;
     if (tmp->o.sirt != 0 || tmp->o.fake_sirt != 0)
         return fail(TILTGPU_ERR_ARG, "a resident series serves WBP slabs (sirt = fake_sirt = 0)");
-    if (xtilt && n_xtilt > 0) {
-        tmp->xtilt.assign(xtilt, xtilt + n_xtilt);
-        tmp->has_xtilt = true;
-    }
-    if (lnx && lny) {
-        const int nz = d->nz;
-        tmp->lnx.assign(lnx, lnx + nz);
-        tmp->lny.assign(lny, lny + nz);
-        int64_t nxs = 0, nys = 0, ng = 0;
-        for (int v = 0; v < nz; ++v) {
-            nxs += lnx[v];
-            nys += lny[v];
-            ng += static_cast<int64_t>(lnx[v]) * lny[v];
-        }
-        tmp->lxs.assign(lxs, lxs + nxs);
-        tmp->lys.assign(lys, lys + nys);
-        tmp->ldx.assign(ldx, ldx + ng);
-        tmp->ldy.assign(ldy, ldy + ng);
-        tmp->has_local = true;
-    }
+    const tiltgpu_view_geometry g{xtilt, n_xtilt, nullptr, 0, lnx, lny, lxs, lys, ldx, ldy};
+    if ((rc = fill_geometry(tmp.get(), d->nz, &g))) return rc;
     // a resident series keeps projections and their filtered copy, not a volume
     tmp->need -= static_cast<int64_t>(d->in_ny) * d->out_nz * d->out_nx * 4;
     int64_t id = 0;

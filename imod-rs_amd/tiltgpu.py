@@ -89,6 +89,14 @@ class ServerStats(C.Structure):
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double)]
 
 
+class ViewGeometry(C.Structure):
+    _fields_ = [("xtilt", C.POINTER(C.c_float)), ("n_xtilt", C.c_int32),
+                ("zfactor", C.POINTER(C.c_float)), ("n_zfactor", C.c_int32),
+                ("lnx", C.POINTER(C.c_int32)), ("lny", C.POINTER(C.c_int32)),
+                ("lxs", C.POINTER(C.c_float)), ("lys", C.POINTER(C.c_float)),
+                ("ldx", C.POINTER(C.c_float)), ("ldy", C.POINTER(C.c_float))]
+
+
 class CtfParams(C.Structure):
     _fields_ = [("voltage_kv", C.c_float), ("cs_mm", C.c_float), ("pixel_a", C.c_float),
                 ("amplitude_contrast", C.c_float), ("cuton_freq", C.c_float),
@@ -157,6 +165,9 @@ SIGNATURES = [
     ("tiltgpu_server_create", C.c_int, [C.c_int64, C.POINTER(_vp)]),
     ("tiltgpu_server_submit", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
                                         C.c_int32, _fp, _fp, C.POINTER(C.c_int64)]),
+    ("tiltgpu_server_submit_geometry", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
+                                                 C.c_int32, C.POINTER(ViewGeometry), _fp, _fp,
+                                                 C.POINTER(C.c_int64)]),
     ("tiltgpu_server_add_series", C.c_int, [_vp, C.POINTER(Dims), C.POINTER(Options), _fp,
                                             C.c_int32, _fp, C.c_int32, _ip, _ip, _fp, _fp,
                                             _fp, _fp, _fp, _ip]),
@@ -817,6 +828,26 @@ def ctf_phaseflip(P, tilt_deg, pixel_a, voltage_kv=300.0, cs_mm=2.7, defocus_nm=
     return out
 
 
+def _view_geometry(xtilt, zfactor, local):
+    """A ViewGeometry over numpy copies (returned too: they must outlive the call)."""
+    xt, zf = _f32(xtilt), _f32(zfactor)
+    g = ViewGeometry()
+    keep = [xt, zf]
+    if xt is not None:
+        g.xtilt, g.n_xtilt = _f(xt), xt.size
+    if zf is not None:
+        g.zfactor, g.n_zfactor = _f(zf), zf.size
+    if local is not None:
+        parts = local.flat() if hasattr(local, "flat") else local
+        lnx = np.ascontiguousarray(parts[0], np.int32)
+        lny = np.ascontiguousarray(parts[1], np.int32)
+        fl = [_f32(v) if np.size(v) else np.zeros(1, np.float32) for v in parts[2:]]
+        keep += [lnx, lny] + fl
+        g.lnx, g.lny = lnx.ctypes.data_as(_ip), lny.ctypes.data_as(_ip)
+        g.lxs, g.lys, g.ldx, g.ldy = [_f(v) for v in fl]
+    return g, keep
+
+
 class Server:
     """Continuous-batching server (tiltgpu_server_*): requests from any thread are
     admitted by device-memory budget and batched into shared launches."""
@@ -829,8 +860,10 @@ class Server:
         self._series_shape = {}
         self._series_keep = []
 
-    def submit(self, P, angles, opts: TiltOptions | None = None):
-        """Queue a whole reconstruction; returns the request id."""
+    def submit(self, P, angles, opts: TiltOptions | None = None, xtilt=None, zfactor=None,
+               local=None):
+        """Queue a whole reconstruction (with optional x-axis tilts, z factors and
+        local grids); returns the request id."""
         P = _f32(P)
         ang = _f32(angles)
         opts = opts or TiltOptions()
@@ -841,8 +874,15 @@ class Server:
         o = opts.to_c()
         out = np.empty((ny, out_nz, out_nx), np.float32)
         rid = C.c_int64()
-        _check(lib().tiltgpu_server_submit(self.h, C.byref(d), C.byref(o), _f(ang), ang.size,
-                                           _f(P), _f(out), C.byref(rid)))
+        if xtilt is None and zfactor is None and local is None:
+            _check(lib().tiltgpu_server_submit(self.h, C.byref(d), C.byref(o), _f(ang),
+                                               ang.size, _f(P), _f(out), C.byref(rid)))
+        else:
+            g, keep = _view_geometry(xtilt, zfactor, local)
+            _check(lib().tiltgpu_server_submit_geometry(self.h, C.byref(d), C.byref(o), _f(ang),
+                                                        ang.size, C.byref(g), _f(P), _f(out),
+                                                        C.byref(rid)))
+            del keep  # copied at submit
         self._pending[rid.value] = (out, P)  # the server reads P until the request completes
         return rid.value
 

@@ -314,6 +314,31 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out);
 int tiltgpu_server_submit(tiltgpu_server* s, const tiltgpu_dims* dims,
                           const tiltgpu_options* opts, const float* angles, int32_t n_angles,
                           const float* projections, float* out, int64_t* id);
+/* Per-view geometry of a served reconstruction beyond the tilt angles: the
+ * --xtilt-file and --zfactor-file contents (main.rs:664-680) and the
+ * --local-file grids laid out as for tiltgpu_set_local_alignment (lnx, lny: one
+ * entry per section).  Any part may be NULL / 0 (absent). */
+typedef struct {
+    const float* xtilt;
+    int32_t n_xtilt;
+    const float* zfactor;
+    int32_t n_zfactor;
+    const int32_t* lnx;
+    const int32_t* lny;
+    const float* lxs;
+    const float* lys;
+    const float* ldx;
+    const float* ldy;
+} tiltgpu_view_geometry;
+/* tiltgpu_server_submit with the x-axis tilts, z factors and local grids of the
+ * reconstruction (geom may be NULL): the served form of `tilt ... --xtilt-file
+ * --zfactor-file --local-file`, bit-identical to a session given the same
+ * tiltgpu_set_xtilt / set_zfactor / set_local_alignment.  The arrays are copied
+ * at submit. */
+int tiltgpu_server_submit_geometry(tiltgpu_server* s, const tiltgpu_dims* dims,
+                                   const tiltgpu_options* opts, const float* angles,
+                                   int32_t n_angles, const tiltgpu_view_geometry* geom,
+                                   const float* projections, float* out, int64_t* id);
 /* Keep a series resident (filtered once) for slab requests; returns when ready.
  * xtilt and the local grids (as tiltgpu_set_local_alignment) may be NULL. */
 int tiltgpu_server_add_series(tiltgpu_server* s, const tiltgpu_dims* dims,

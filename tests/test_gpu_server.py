@@ -113,3 +113,44 @@ def test_server_rejects_bad_requests(gpu):
             srv.submit(P, ang[:5], o)  # stretch needs an angle per section
         ok = srv.submit(P, ang, o)
         assert srv.wait(ok)[0].shape == (32, 48, 128)
+
+
+def test_server_requests_with_geometry(gpu, rel):
+    """Served reconstructions carrying x-axis tilts, z factors and local grids
+    (tiltgpu_server_submit_geometry): WBP bit for bit the plain session's, in a
+    mix with plain requests; SIRT within rounding of the session's (the
+    two-row reprojection sums with atomics) and at the bar against the oracle."""
+    ang = np.linspace(-60, 60, 31).astype(np.float32)
+    rng = np.random.default_rng(21)
+    sx, sy = rng.normal(0, 1.5, 31).astype(np.float32), rng.normal(0, 1.5, 31).astype(np.float32)
+    xt = np.linspace(-1.0, 1.0, 31).astype(np.float32)
+    zf = np.linspace(0.95, 1.05, 31).astype(np.float32)
+    P = gpu.synth_series(128, 32, ang, vol_nz=48, n_ellipsoids=25, n_beads=4, seed=21, xtilt=xt,
+                         shift_x=sx, shift_y=sy, axis=(2.0, 9.0), bead_radius=2.0)
+    patches = []
+    for v in range(31):
+        for x in (20.0, 100.0):
+            for y in (6.0, 26.0):
+                patches.append((v, x, y, float(sx[v] + rng.normal(0, 0.3)),
+                                float(sy[v] + rng.normal(0, 0.3))))
+    grids = O.build_local_grids(patches, 31)
+    wbp = gpu.TiltOptions(thickness=48)
+    sirt = gpu.TiltOptions(thickness=48, sirt=2)
+    plain = mix(gpu)[:3]
+    print("case: served WBP and SIRT-2 with x-tilt, z factor, 2x2 local grids, in a mix", flush=True)
+    with gpu.Server(8 << 30) as srv:
+        ids = [srv.submit(P, ang, wbp, xtilt=xt, zfactor=zf, local=grids)]
+        ids += [srv.submit(Q, a, o) for Q, a, o in plain]
+        ids.append(srv.submit(P, ang, sirt, xtilt=xt, zfactor=zf, local=grids))
+        got = [srv.wait(i)[0] for i in ids]
+    ref_w, _ = gpu.reconstruct(P, ang, wbp, xtilt=xt, zfactor=zf, local=grids)
+    assert np.array_equal(got[0], ref_w)
+    for (Q, a, o), g in zip(plain, got[1:4]):
+        assert np.array_equal(g, gpu.reconstruct(Q, a, o)[0])
+    ref_s, _ = gpu.reconstruct(P, ang, sirt, xtilt=xt, zfactor=zf, local=grids)
+    assert rel(got[4], ref_s) <= 1e-5
+    o_w, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=48), xtilt=xt, zfactor=zf, local=grids)
+    assert rel(got[0], o_w) <= 1e-4
+    o_s, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=48, sirt=2), xtilt=xt, zfactor=zf,
+                        local=grids)
+    assert rel(got[4], o_s) <= 1e-4
