@@ -33,6 +33,11 @@ METRIC = "Gvoxel·view/s WBP 2k²×512×121; served p50/p99 latency vs plain FP3
 # accumulate: 7 flop per voxel-view in all.
 BP_FLOP_PER_VV = 7.0
 PEAK_FP32_TFLOPS = 157.3
+# VALU lane-instructions per voxel*view of the C1 backprojection launch
+# (SQ_INSTS_VALU x 64 / voxel*views, profiles/bp_c1_sq_counters.md) and the
+# SIMDs of an MI355X (256 CUs x 4)
+BP_VALU_PER_VV = 12.09
+N_SIMDS = 1024
 PEAK_HBM_GBS = 8000.0
 
 
@@ -794,7 +799,8 @@ def main():
     value = vv / (dt / a.steps) / 1e9
     bp_ms = tm["bp_ms"] / max(1, tm["bp_launches"])
     achieved = BP_FLOP_PER_VV * vv / (bp_ms * 1e-3) / 1e12
-    fma_tflops = tg.fma_calibration()  # attainable non-packed FP32 rate, measured
+    fma_tflops = tg.fma_calibration()  # attainable non-packed FP32 rate (both VALU pipes), measured
+    issue = tg.issue_calibration()     # ns per wave instruction per SIMD of the calibration mixes
     traffic = load_traffic()
     line = {
         "metric": METRIC,
@@ -834,6 +840,9 @@ def main():
             "launch_ms": round(bp_ms, 3),
             "fp32_fma_attainable_tflops": round(fma_tflops, 1),
             "frac_of_attainable": round(achieved / fma_tflops, 4),
+            "valu_issue_ns_per_wave_instr": {k: round(v, 4) for k, v in issue.items()},
+            "bp_launch_if_issue_bound_ms": round(
+                vv / 64.0 * BP_VALU_PER_VV * issue["bp_two_row_voxel_step"] / N_SIMDS / 1e6, 2),
         },
     }
     rows16 = None

@@ -11,9 +11,14 @@
 // once per lane window instead of once per bin it touches, and the loop has no
 // per-bin range search.
 //
-// Two-row geometry (x-axis tilt / local alignment): scatter with float atomics,
-// same coordinates and weights as main.rs:840-889; the per-bin summation order
-// is not the reference's, so results agree to rounding, not bit for bit.
+// Two-row geometry (x-axis tilt / local alignment): a scatter with the
+// coordinates and weights of main.rs:840-889.  Each term is rounded to a
+// multiple of 2^-k (k from the volume's largest magnitude, so that no bin sum
+// can reach 2^62) and summed with 64-bit integer atomics, which are
+// order-independent: the sinogram is the same bits on every run and launch
+// shape.  The fixed-point sums are exact up to those term roundings (2^-31 of
+// the volume's largest magnitude or finer), so they differ from the reference's
+// sequential FP32 sums by the rounding of the latter.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -388,6 +393,59 @@ __device__ bool fp_local_shift(const FpScatterParams& P, int view, float x, floa
     return true;
 }
 
+// term * 2^k rounded to nearest as a 64-bit two's-complement integer, added with
+// an integer atomic (the sum is independent of the order of the additions).
+__device__ __forceinline__ void fixed_add(unsigned long long* a, float term, double scale) {
+    const long long q = __double2ll_rn(static_cast<double>(term) * scale);
+    atomicAdd(a, static_cast<unsigned long long>(q));
+}
+
+// k such that count_bound * max|V| * 2^k < 2^62 (0 for an all-zero volume).
+__global__ void fp_fixed_exponent_kernel(const unsigned* absmax_bits, double count_bound,
+                                         int* k_out) {
+    const double bound = static_cast<double>(__uint_as_float(*absmax_bits)) * count_bound;
+    int k = 0;
+    if (bound > 0.0) {
+        int e;
+        (void)frexp(bound, &e);  // bound < 2^e
+        k = 62 - e;
+    }
+    *k_out = max(-1000, min(1000, k));
+}
+
+__global__ void __launch_bounds__(256) fp_fixed_finalize_kernel(const unsigned long long* acc,
+                                                               float* sino, int64_t n,
+                                                               const int* k) {
+    const double inv = ldexp(1.0, -*k);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        sino[i] = static_cast<float>(static_cast<double>(static_cast<long long>(acc[i])) * inv);
+}
+
+__global__ void __launch_bounds__(256) fp_zero_u64_kernel(unsigned long long* a, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        a[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) fp_absmax_kernel(const float* v, int64_t n,
+                                                       unsigned* absmax_bits) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        m = fmaxf(m, fabsf(v[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(absmax_bits, __float_as_uint(red[0]));
+}
+
+__global__ void fp_zero_u32_kernel(unsigned* p) { *p = 0u; }
+
 __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
     const int ox = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y;
@@ -417,8 +475,9 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
     const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
     const float fn = static_cast<float>(P.in_nx);
     const float* vcol = P.vol + static_cast<int64_t>(iy) * P.out_nz * P.out_nx + ox;
-    float* r0 = P.sino + (static_cast<int64_t>(p) * P.in_ny + sy0) * P.in_nx;
-    float* r1 = r0 + P.in_nx;
+    unsigned long long* a0 = P.acc + (static_cast<int64_t>(p) * P.in_ny + sy0) * P.in_nx;
+    unsigned long long* a1 = a0 + P.in_nx;
+    const double scale = ldexp(1.0, *P.scale_exp);
     const float w0 = 1.0f - fy, w1 = fy;
     for (int oz = 0; oz < P.out_nz; ++oz) {
         float px = (a + zc[oz]) + P.cx;
@@ -429,13 +488,13 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
         const float frac = px - fl;
         const float val = vcol[static_cast<int64_t>(oz) * P.out_nx];
         if (mode == 2) {
-            atomicAdd(r0 + px0, val * (1.0f - frac) * w0);
-            atomicAdd(r0 + px0 + 1, val * frac * w0);
-            atomicAdd(r1 + px0, val * (1.0f - frac) * w1);
-            atomicAdd(r1 + px0 + 1, val * frac * w1);
+            fixed_add(a0 + px0, val * (1.0f - frac) * w0, scale);
+            fixed_add(a0 + px0 + 1, val * frac * w0, scale);
+            fixed_add(a1 + px0, val * (1.0f - frac) * w1, scale);
+            fixed_add(a1 + px0 + 1, val * frac * w1, scale);
         } else {
-            atomicAdd(r0 + px0, val * (1.0f - frac));
-            atomicAdd(r0 + px0 + 1, val * frac);
+            fixed_add(a0 + px0, val * (1.0f - frac), scale);
+            fixed_add(a0 + px0 + 1, val * frac, scale);
         }
     }
 }
@@ -467,10 +526,27 @@ hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_
     return hipGetLastError();
 }
 
-hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st) {
-    if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
-    dim3 grid((p.out_nx + 255) / 256, p.in_ny, p.nviews);
-    hipLaunchKernelGGL(fp_scatter_kernel, grid, dim3(256), 0, st, p);
+hipError_t launch_forward_scatter(const FpScatterParams& p, unsigned* absmax_scratch,
+                                  int* scale_exp, hipStream_t st) {
+    // every section of the sinogram is written (views >= nviews stay zero)
+    const int64_t nsino = static_cast<int64_t>(p.nz_sections) * p.in_ny * p.in_nx;
+    if (nsino <= 0) return hipSuccess;
+    const int64_t nvol = static_cast<int64_t>(p.in_ny) * p.out_nz * p.out_nx;
+    const unsigned gs = static_cast<unsigned>(std::min<int64_t>(4096, (nsino + 255) / 256));
+    hipLaunchKernelGGL(fp_zero_u64_kernel, dim3(gs), dim3(256), 0, st, p.acc, nsino);
+    hipLaunchKernelGGL(fp_zero_u32_kernel, dim3(1), dim3(1), 0, st, absmax_scratch);
+    const unsigned gv = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(2048, (nvol + 255) / 256)));
+    hipLaunchKernelGGL(fp_absmax_kernel, dim3(gv), dim3(256), 0, st, p.vol, nvol, absmax_scratch);
+    // a bin receives at most one term per voxel of the volume
+    const double count_bound = static_cast<double>(std::max<int64_t>(nvol, 1));
+    hipLaunchKernelGGL(fp_fixed_exponent_kernel, dim3(1), dim3(1), 0, st, absmax_scratch,
+                       count_bound, scale_exp);
+    if (p.nviews > 0 && p.in_ny > 0) {
+        dim3 grid((p.out_nx + 255) / 256, p.in_ny, p.nviews);
+        hipLaunchKernelGGL(fp_scatter_kernel, grid, dim3(256), 0, st, p);
+    }
+    hipLaunchKernelGGL(fp_fixed_finalize_kernel, dim3(gs), dim3(256), 0, st, p.acc, p.sino, nsino,
+                       static_cast<const int*>(scale_exp));
     return hipGetLastError();
 }
 

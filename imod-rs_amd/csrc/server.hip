@@ -942,6 +942,9 @@ int tiltgpu_server_submit_geometry(tiltgpu_server* S, const tiltgpu_dims* d,
     int rc = fill_recon(tmp.get(), d, o, angles, n_angles, projections);
     if (rc) return rc;
     if ((rc = fill_geometry(tmp.get(), d->nz, geom))) return rc;
+    // the two-row reprojection of SIRT keeps 64-bit fixed-point sums
+    if (tmp->o.sirt > 0 && (tmp->has_xtilt || tmp->has_local))
+        tmp->need += static_cast<int64_t>(d->nz) * d->in_ny * d->in_nx * 8;
     if (!tmp->o.no_cosine_stretch && n_angles < d->nz)
         return fail(TILTGPU_ERR_ARG, "cosine stretch needs one tilt angle per section");
     std::lock_guard<std::mutex> lk(S->mu);

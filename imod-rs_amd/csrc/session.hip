@@ -563,8 +563,15 @@ int run_forward(tiltgpu_session* s) {
     Scope sc(s, kClsFp);
     const int64_t total = static_cast<int64_t>(s->d.nz) * s->d.in_ny * s->d.in_nx;
     if (s->two_row) {
-        HIP_TRY(launch_fill_f32(s->sino, total, 0.f, s->stream));
+        if (!s->fp_acc) {
+            int rc = dev_alloc(s, &s->fp_acc, static_cast<size_t>(total));
+            if (rc) return rc;
+            if ((rc = dev_alloc(s, &s->fp_fixed, 2))) return rc;
+        }
         FpScatterParams p{};
+        p.nz_sections = s->d.nz;
+        p.acc = s->fp_acc;
+        p.scale_exp = reinterpret_cast<const int*>(s->fp_fixed + 1);
         p.vol = s->vol;
         p.sino = s->sino;
         p.vgeom = s->vgeom;
@@ -587,7 +594,8 @@ int run_forward(tiltgpu_session* s) {
         p.lys = s->d_lys;
         p.ldx = s->d_ldx;
         p.ldy = s->d_ldy;
-        HIP_TRY(launch_forward_scatter(p, s->stream));
+        HIP_TRY(launch_forward_scatter(p, s->fp_fixed, reinterpret_cast<int*>(s->fp_fixed + 1),
+                                       s->stream));
     } else {
         // views >= nviews are never projected (main.rs:835 iterates proj_params)
         if (s->nviews < s->d.nz) {
@@ -1320,8 +1328,9 @@ int tiltgpu_reconstruct_row(tiltgpu_session* s, int32_t iy, float* out) {
     return tiltgpu_reconstruct_rows(s, iy, iy + 1, out);
 }
 
-int tiltgpu_fma_calibration(int32_t iters, double* tflops) {
-    if (iters < 1 || !tflops) return fail(TILTGPU_ERR_ARG, "bad arguments");
+// One timed launch (after a warm-up) of an issue-rate mix over 8 waves per SIMD;
+// ms and the wave instructions issued per SIMD.
+static int run_issue_mix(int mix, int32_t iters, double* ms_out, double* instr_per_simd) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
         return fail(TILTGPU_ERR_NODEV, "no GPU visible to HIP");
@@ -1329,7 +1338,7 @@ int tiltgpu_fma_calibration(int32_t iters, double* tflops) {
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         cus = prop.multiProcessorCount;
-    const int blocks = cus * 32;  // 8 waves per SIMD
+    const int blocks = cus * 8;  // 8 workgroups of 4 waves per CU: 8 waves per SIMD
     float* out = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1337,22 +1346,48 @@ int tiltgpu_fma_calibration(int32_t iters, double* tflops) {
     if (hipMalloc(&out, static_cast<size_t>(blocks) * 256 * 4) != hipSuccess ||
         hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        rc = fail(TILTGPU_ERR_HIP, "FMA calibration: allocation failed");
+        rc = fail(TILTGPU_ERR_HIP, "issue calibration: allocation failed");
     } else {
         float ms = 0.f;
-        if (launch_fma_rate(out, blocks, iters, st) != hipSuccess ||  // warm-up
-            hipEventRecord(e0, st) != hipSuccess || launch_fma_rate(out, blocks, iters, st) != hipSuccess ||
+        if (launch_issue_rate(mix, out, blocks, iters, st) != hipSuccess ||  // warm-up
+            hipEventRecord(e0, st) != hipSuccess ||
+            launch_issue_rate(mix, out, blocks, iters, st) != hipSuccess ||
             hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-            rc = fail(TILTGPU_ERR_HIP, "FMA calibration: launch failed");
-        const double fma = static_cast<double>(blocks) * 256 * 8 * iters;
-        *tflops = 2.0 * fma / (static_cast<double>(ms) * 1e-3) / 1e12;
+            rc = fail(TILTGPU_ERR_HIP, "issue calibration: launch failed");
+        *ms_out = ms;
+        // waves per SIMD (8) x lane-loop iterations x instructions per iteration
+        *instr_per_simd = 8.0 * iters * issue_rate_instructions(mix);
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
     if (out) (void)hipFree(out);
     return rc;
+}
+
+int tiltgpu_fma_calibration(int32_t iters, double* tflops) {
+    if (iters < 1 || !tflops) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    double ms = 0, n = 0;
+    int rc = run_issue_mix(0, iters, &ms, &n);
+    if (rc) return rc;
+    int dev = 0, cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    // n wave FMAs per SIMD, 4 SIMDs per CU, 64 lanes, 2 FLOP each
+    *tflops = n * 4.0 * cus * 64.0 * 2.0 / (ms * 1e-3) / 1e12;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_issue_calibration(int32_t mix, int32_t iters, double* ns_per_instr) {
+    if (mix < 0 || mix > 4 || iters < 1 || !ns_per_instr)
+        return fail(TILTGPU_ERR_ARG, "bad arguments");
+    double ms = 0, n = 0;
+    int rc = run_issue_mix(mix, iters, &ms, &n);
+    if (rc) return rc;
+    *ns_per_instr = ms * 1e6 / n;
+    return TILTGPU_OK;
 }
 
 int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy) {

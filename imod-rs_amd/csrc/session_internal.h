@@ -83,6 +83,8 @@ struct tiltgpu_session {
     float* vol = nullptr;      // [out_ny][out_nz][out_nx]
     float* vol_tmp = nullptr;  // fake-SIRT ping-pong
     float* sino = nullptr;     // SIRT reprojection [nz][in_ny][in_nx]
+    unsigned long long* fp_acc = nullptr;  // two-row reprojection fixed-point sums
+    unsigned* fp_fixed = nullptr;          // [0]: volume absmax bits, [1]: fixed-point exponent
     float4* vgeom = nullptr;
     BpView* views = nullptr;
     float* zc = nullptr;

@@ -155,6 +155,7 @@ struct FpScatterParams {
     const float4* vgeom;
     const float* zc_tab;
     int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
+    int nz_sections;  // sections of the sinogram (>= nviews)
     float cx, ocx;
     int has_local;
     const int* lnx;
@@ -166,6 +167,10 @@ struct FpScatterParams {
     const float* lys;
     const float* ldx;
     const float* ldy;
+    // fixed-point accumulation: acc [nz][in_ny][in_nx] 64-bit sums of the terms
+    // rounded to multiples of 2^-(*scale_exp); scale_exp is set on the device
+    unsigned long long* acc;
+    const int* scale_exp;
 };
 
 // Several series in one filter / forward-projection launch: workgroups
@@ -208,7 +213,11 @@ hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_
                                        hipStream_t st);
 size_t fp_lds_bytes(const FpParams& p);  // dynamic LDS of one series' workgroups
 int fp_gather_blocks(const FpParams& p);
-hipError_t launch_forward_scatter(const FpScatterParams& p, hipStream_t st);
+// Two-row forward projection, deterministic: the terms of main.rs:866-889 as
+// 64-bit fixed point summed with integer atomics (order-independent), then
+// converted to FP32.  absmax_scratch: one unsigned; scale_exp: one int.
+hipError_t launch_forward_scatter(const FpScatterParams& p, unsigned* absmax_scratch,
+                                  int* scale_exp, hipStream_t st);
 hipError_t launch_sum_partials(const double* partial, int n, double* out, int sqrt_result,
                                hipStream_t st);
 hipError_t launch_transform(float* v, int64_t n, float dw, float os, float oa,
@@ -223,7 +232,10 @@ hipError_t launch_slice_stats(const float* v, int rows, int64_t per_row, float* 
                               hipStream_t st);
 hipError_t launch_fill_f32(float* p, int64_t n, float value, hipStream_t st);
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
-hipError_t launch_fma_rate(float* out, int blocks, int iters, hipStream_t st);
+// VALU issue-rate calibration kernels (kernels_misc.hip): mix 0..4, and the
+// wave instructions one lane-loop iteration of a mix issues.
+hipError_t launch_issue_rate(int mix, float* out, int blocks, int iters, hipStream_t st);
+int issue_rate_instructions(int mix);
 hipError_t launch_geometry_f64(const float* angles, int n_angles, const float* xtilt,
                                int n_xtilt, const float* zfac, int n_zfac, int nviews, int nsec,
                                float max_tilt, int stretch, float* view_out, float* stretch_out,

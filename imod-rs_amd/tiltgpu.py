@@ -180,6 +180,7 @@ SIGNATURES = [
     ("tiltgpu_server_destroy", None, [_vp]),
     ("tiltgpu_copy_calibration", C.c_int, [C.c_int64, C.c_int32, _dp]),
     ("tiltgpu_fma_calibration", C.c_int, [C.c_int32, _dp]),
+    ("tiltgpu_issue_calibration", C.c_int, [C.c_int32, C.c_int32, _dp]),
     ("tiltgpu_ctf_default_params", None, [C.POINTER(CtfParams)]),
     ("tiltgpu_electron_wavelength", C.c_double, [C.c_float]),
     ("tiltgpu_ctf_strip_width", C.c_int, [C.POINTER(CtfParams), C.c_int32, _fp, C.c_int32]),
@@ -698,10 +699,26 @@ def copy_calibration(nbytes, iters=1):
 
 
 def fma_calibration(iters=1 << 14):
-    """Attainable non-packed FP32 FMA rate of the device, TFLOP/s."""
+    """Attainable non-packed FP32 FMA rate of the device (VGPR operands, both
+    VALU pipes), TFLOP/s."""
     t = C.c_double()
     _check(lib().tiltgpu_fma_calibration(iters, C.byref(t)))
     return t.value
+
+
+ISSUE_MIXES = {0: "fp32_fma", 1: "fract_alone", 2: "fract_with_fp32_fma",
+               3: "fract_with_int_add", 4: "bp_two_row_voxel_step"}
+
+
+def issue_calibration(iters=1 << 13):
+    """ns per wave instruction per SIMD of each calibration mix (8 waves/SIMD)."""
+    out = {}
+    for mix, name in ISSUE_MIXES.items():
+        t = C.c_double()
+        n = iters if mix != 4 else max(1, iters // 8)
+        _check(lib().tiltgpu_issue_calibration(mix, n, C.byref(t)))
+        out[name] = t.value
+    return out
 
 
 def read_defocus_file(path):

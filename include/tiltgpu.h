@@ -364,9 +364,18 @@ void tiltgpu_server_destroy(tiltgpu_server* s);
  * HBM bandwidth of the device. */
 int tiltgpu_copy_calibration(int64_t bytes, int32_t iters, double* ms_per_copy);
 /* Diagnostic (no counterpart in the reference): the attainable FP32 VALU rate of
- * the device with non-packed FMAs (the form every kernel here is built with),
- * from `iters` iterations of eight independent FMA chains per lane; TFLOP/s. */
+ * the device with non-packed FMAs (the form every kernel here is built with)
+ * whose operands are all in VGPRs, so that the VALU issues them on both of its
+ * pipes, from `iters` iterations of eight independent FMA chains per lane at 8
+ * waves per SIMD; TFLOP/s. */
 int tiltgpu_fma_calibration(int32_t iters, double* tflops);
+/* Diagnostic (no counterpart in the reference): time per wave instruction per
+ * SIMD (ns) of an instruction mix at 8 waves per SIMD -- 0: FP32 FMA (VGPR
+ * operands); 1: v_fract_f32 alone; 2: v_fract_f32 + FP32 FMA pairs; 3:
+ * v_fract_f32 + integer add pairs; 4: the two-row backprojection voxel step
+ * (10 instructions, no LDS read).  Which instructions share the VALU's second
+ * pipe sets the rate a kernel's instruction mix can reach. */
+int tiltgpu_issue_calibration(int32_t mix, int32_t iters, double* ns_per_instr);
 
 /* ------------------------------------------------------------ ctfphaseflip
  * CTF phase flip of a tilt series by 2D strips: the CPU path of

@@ -219,17 +219,32 @@ def test_reproject_bitwise(gpu, in_nx, ny, out_nx, thick, nviews):
     assert np.array_equal(R.view(np.uint32), R2.view(np.uint32))
 
 
-def test_reproject_xtilt_scatter(gpu, rel):
+@pytest.mark.parametrize("local", [False, True])
+def test_reproject_xtilt_scatter(gpu, rel, local):
+    """Two-row reprojection (x-axis tilt, optionally a 3x2 local grid): the same
+    bits on repeated runs (fixed-point sums), within FP32 summation rounding of
+    the FP32 oracle's serial scatter and closer still to the FP64 oracle's."""
     rng = np.random.default_rng(3)
     ang = np.linspace(-60, 60, 15).astype(np.float32)
-    xt = np.full(15, 0.7, np.float32)
+    xt = np.linspace(-0.7, 0.9, 15).astype(np.float32)
     V = rng.standard_normal((8, 30, 96)).astype(np.float32)
+    grids = None
+    if local:
+        pts = [(v, x, y, float(rng.normal(0, 1.2)), float(rng.normal(0, 0.8)))
+               for v in range(15) for x in (10.0, 48.0, 90.0) for y in (1.0, 6.0)]
+        grids = O.build_local_grids(pts, 15)
     with gpu.Session(96, 8, 15, gpu.TiltOptions(thickness=30, no_cosine_stretch=True)) as s:
         s.set_angles(ang)
         s.set_xtilt(xt)
+        if grids is not None:
+            s.set_local(*grids.flat())
         R = s.reproject(V)
-    ref = O.forward_project(V, O.proj_params(ang, xtilt=xt), 96, 15)
+        R2 = s.reproject(V)
+    assert np.array_equal(R.view(np.uint32), R2.view(np.uint32))
+    ref = O.forward_project_full(V, ang, 96, 15, xtilt=xt, local=grids)
+    ref64 = O.forward_project_full(V, ang, 96, 15, xtilt=xt, local=grids, prec="f64")
     assert rel(R, ref) <= 1e-5
+    assert rel(R, ref64) <= rel(ref, ref64) + 1e-7
 
 
 def test_sirt_trace_per_iteration(gpu, rel):
@@ -303,10 +318,8 @@ def test_launch_shapes_do_not_change_results(gpu, xtilt):
             else:
                 assert t["tile_source"] == 1 and t["fp_source"] == 1  # the tuned table
     for o in outs[1:]:
-        if xt is None:
-            assert np.array_equal(o, outs[0])
-        else:  # the two-row FP scatters with float atomics: equal to rounding
-            assert np.max(np.abs(o - outs[0])) <= 1e-5 * np.sqrt(np.mean(outs[0] ** 2))
+        # the two-row reprojection sums in fixed point: order-independent too
+        assert np.array_equal(o, outs[0])
 
 
 def test_sirt_xtilt_scatter(gpu, rel):
