@@ -22,7 +22,8 @@ ARCH = "gfx950"
 HIP_SOURCES = ["csrc/kernels_bp.hip", "csrc/kernels_filter.hip", "csrc/kernels_fp.hip",
                "csrc/kernels_misc.hip", "csrc/session.hip", "csrc/server.hip", "csrc/ctf.hip",
                "csrc/noalloc.hip"]
-HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp", "csrc/tuning.cpp"]
+HOST_SOURCES = ["host/textio.cpp", "host/synth.cpp", "host/mrc.cpp", "host/localalign.cpp",
+                "csrc/tuning.cpp"]
 # drop-in executables: name -> sources (linked against libtiltgpu)
 CLIS = {"tilt": ["host/tilt_main.cpp"], "ctfphaseflip": ["host/ctfphaseflip_main.cpp"]}
 

@@ -97,6 +97,11 @@ class ViewGeometry(C.Structure):
                 ("ldx", C.POINTER(C.c_float)), ("ldy", C.POINTER(C.c_float))]
 
 
+class AlignView(C.Structure):
+    _fields_ = [("dx", C.c_double), ("dy", C.c_double), ("rotation", C.c_double),
+                ("mag", C.c_double), ("tilt", C.c_double)]
+
+
 class CtfParams(C.Structure):
     _fields_ = [("voltage_kv", C.c_float), ("cs_mm", C.c_float), ("pixel_a", C.c_float),
                 ("amplitude_contrast", C.c_float), ("cuton_freq", C.c_float),
@@ -200,6 +205,12 @@ SIGNATURES = [
     ("tiltgpu_interpolate_defocus", C.c_int, [_ip, _fp, C.c_int32, C.c_int32, _fp]),
     ("tiltgpu_read_tilt_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
     ("tiltgpu_read_per_view_file", C.c_int, [C.c_char_p, _fp, C.c_int32]),
+    ("tiltgpu_local_corrections", C.c_int, [_dp, _dp, C.POINTER(AlignView), C.c_int32,
+                                            C.c_int32, C.c_int32, C.c_float, C.c_double,
+                                            C.c_double, C.c_int32, C.c_double, C.c_double,
+                                            C.c_int32, _ip, _ip, _ip, _dp, _dp]),
+    ("tiltgpu_write_local_corrections", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _dp,
+                                                  _dp]),
     ("tiltgpu_parse_local_file", C.c_int, [C.c_char_p, C.c_int32, _ip, _ip, _ip, _fp, _fp,
                                            _fp, _fp]),
     ("tiltgpu_mrc_info_read", C.c_int, [C.c_char_p, C.POINTER(MrcInfo)]),
@@ -617,6 +628,41 @@ def reconstruct_rows_noalloc(d_proj, params, proj_means, in_nx, in_ny, n_projs, 
     _check(lib().tiltgpu_reconstruct_rows_noalloc(
         d_proj.ptr, params, _f(m), len(params), in_nx, in_ny, n_projs, out_nx, out_nz,
         FILL[fill_mode], y0, y1, d_out.ptr, None))
+
+
+def local_corrections(obs, beads, views, patch_size=500, overlap=0.5, cx=0.0, cy=0.0,
+                      skip_edge=0, img_w=0.0, img_h=0.0):
+    """tiltalign's --output-local corrections (apps/tiltalign/src/main.rs:864-940,
+    1374-1402).  obs [n_beads][n_views][2] centred observations (NaN x: none),
+    beads [n_beads][3] centred, views: sequence of (dx, dy, rotation, mag, tilt).
+    Returns a list of (view, patch_x, patch_y, dx, dy)."""
+    obs = np.ascontiguousarray(obs, np.float64)
+    beads = np.ascontiguousarray(beads, np.float64)
+    nb, nv = obs.shape[0], obs.shape[1]
+    av = (AlignView * max(1, nv))(*[AlignView(*map(float, v)) for v in views])
+    cap = 1024
+    while True:
+        ov, opx, opy = (np.zeros(cap, np.int32) for _ in range(3))
+        odx, ody = np.zeros(cap), np.zeros(cap)
+        n = _check(lib().tiltgpu_local_corrections(
+            obs.ctypes.data_as(_dp), beads.ctypes.data_as(_dp), av, nb, nv, patch_size, overlap,
+            cx, cy, skip_edge, img_w, img_h, cap, ov.ctypes.data_as(_ip), opx.ctypes.data_as(_ip),
+            opy.ctypes.data_as(_ip), odx.ctypes.data_as(_dp), ody.ctypes.data_as(_dp)))
+        if n <= cap:
+            return [(int(ov[i]), int(opx[i]), int(opy[i]), float(odx[i]), float(ody[i]))
+                    for i in range(n)]
+        cap = n
+
+
+def write_local_corrections(path, corrections):
+    """The --output-local file of tiltalign (main.rs:1404-1409)."""
+    n = len(corrections)
+    cols = list(zip(*corrections)) if n else [[]] * 5
+    v, px, py = (np.ascontiguousarray(c, np.int32) if n else np.zeros(1, np.int32) for c in cols[:3])
+    dx, dy = (np.ascontiguousarray(c, np.float64) if n else np.zeros(1) for c in cols[3:])
+    _check(lib().tiltgpu_write_local_corrections(
+        str(path).encode(), n, v.ctypes.data_as(_ip), px.ctypes.data_as(_ip),
+        py.ctypes.data_as(_ip), dx.ctypes.data_as(_dp), dy.ctypes.data_as(_dp)))
 
 
 def read_tilt_file(path, cap=100000):

@@ -455,6 +455,35 @@ int tiltgpu_read_defocus_file(const char* path, int32_t* views, float* entries, 
 int tiltgpu_interpolate_defocus(const int32_t* views, const float* entries, int32_t n,
                                 int32_t n_views, float* out);
 
+/* tiltalign's solved per-view alignment (ViewParams, apps/tiltalign/src/main.rs:
+ * 147-154): shift, in-plane rotation (radians), magnification, tilt (radians). */
+typedef struct {
+    double dx, dy, rotation, mag, tilt;
+} tiltgpu_align_view;
+/* The local-alignment corrections tiltalign writes with --output-local
+ * (compute_local_alignment, apps/tiltalign/src/main.rs:864-940, and the
+ * --skip-edge-patches filter, :1374-1402): patches of patch_size pixels at a
+ * step of patch_size * (1 - overlap) over the beads' bounding box; for every
+ * patch and view, the mean residual (observed - projected) of the beads observed
+ * inside the patch, where at least two are.  obs [n_beads][n_views][2]: centred
+ * observed positions, NaN x for no observation; beads [n_beads][3]: solved
+ * centred (x, y, z); cx, cy: the image centre; skip_edge 0 keeps every patch,
+ * img_w / img_h <= 0 mean 2*cx / 2*cy.  Writes up to cap corrections (view,
+ * patch index x, patch index y, dx, dy) in the reference's order and returns
+ * their count.  The patch fields are grid INDICES, which `tilt --local-file`
+ * (main.rs:232-307) reads as image coordinates -- kept as the reference does. */
+int tiltgpu_local_corrections(const double* obs, const double* beads,
+                              const tiltgpu_align_view* views, int32_t n_beads, int32_t n_views,
+                              int32_t patch_size, float overlap, double cx, double cy,
+                              int32_t skip_edge, double img_w, double img_h, int32_t cap,
+                              int32_t* out_view, int32_t* out_px, int32_t* out_py,
+                              double* out_dx, double* out_dy);
+/* The --output-local file (main.rs:1404-1409): a comment line, then
+ * "view patch_x patch_y dx dy" with dx, dy to 4 decimals. */
+int tiltgpu_write_local_corrections(const char* path, int32_t n, const int32_t* view,
+                                    const int32_t* px, const int32_t* py, const double* dx,
+                                    const double* dy);
+
 /* --local-file (load_local_alignment, main.rs:232-307) parsed into per-view
  * grids for tiltgpu_set_local_alignment.  sizes[3] receives the total lengths of
  * xs, ys and dx/dy; with nx == NULL only sizes are written.  nx, ny: nz ints. */

@@ -14,6 +14,7 @@ It also restates the text parsers the path uses: ``read_tilt_file``
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -404,6 +405,63 @@ def parse_local_file(path):
 
 
 # ----------------------------------------------------------------- numerics --
+
+def align_project(bead, view):
+    """project, apps/tiltalign/src/main.rs:169-184 (f64)."""
+    dx, dy, rot, mag, tilt = view
+    ct, st, cr, sr = math.cos(tilt), math.sin(tilt), math.cos(rot), math.sin(rot)
+    xp = bead[0] * ct + bead[2] * st
+    yp = bead[1]
+    return mag * (cr * xp - sr * yp) + dx, mag * (sr * xp + cr * yp) + dy
+
+
+def local_corrections(obs, beads, views, patch_size=500, overlap=0.5, cx=0.0, cy=0.0,
+                      skip_edge=0, img_w=0.0, img_h=0.0):
+    """compute_local_alignment (apps/tiltalign/src/main.rs:864-940) and the
+    --skip-edge-patches filter (:1374-1402), restated with Python floats (f64).
+    obs[b][v] = (x, y) or None; returns [(view, px, py, dx, dy)]."""
+    if not beads:
+        return []
+    xs = [b[0] + cx for b in beads]
+    ys = [b[1] + cy for b in beads]
+    min_x, max_x, min_y, max_y = min(xs), max(xs), min(ys), max(ys)
+    ps = float(patch_size)
+    step = ps * (1.0 - float(np.float32(overlap)))
+    if step < 1.0:
+        return []
+    npx = max(0, math.ceil((max_x - min_x) / step)) + 1
+    npy = max(0, math.ceil((max_y - min_y) / step)) + 1
+    out = []
+    for py in range(npy):
+        for px in range(npx):
+            pcx = min_x + px * step + ps / 2.0
+            pcy = min_y + py * step + ps / 2.0
+            half = ps / 2.0
+            for vi, view in enumerate(views):
+                sx = sy = 0.0
+                count = 0
+                for bi, track in enumerate(obs):
+                    o = track[vi]
+                    if o is None:
+                        continue
+                    if abs(o[0] + cx - pcx) <= half and abs(o[1] + cy - pcy) <= half:
+                        qx, qy = align_project(beads[bi], view)
+                        sx += o[0] - qx
+                        sy += o[1] - qy
+                        count += 1
+                if count >= 2:
+                    out.append((vi, px, py, sx / count, sy / count))
+    if skip_edge > 0:
+        w = img_w if img_w > 0 else cx * 2.0
+        h = img_h if img_h > 0 else cy * 2.0
+        m = float(skip_edge)
+
+        def keep(c):
+            ex, ey = c[1] * step + ps / 2.0, c[2] * step + ps / 2.0
+            return ex >= m and ex <= w - m and ey >= m and ey <= h - m
+        out = [c for c in out if keep(c)]
+    return out
+
 
 def min_max_mean(a):
     a = np.ascontiguousarray(a, dtype=np.float32).ravel()

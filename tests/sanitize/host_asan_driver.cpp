@@ -41,6 +41,12 @@ void or_ctf_phaseflip_f64(const float*, int, int, int, const float*, int, const 
                           const double*, int, float*, int);
 }
 
+// libtiltgpu's error recorder, which the host sources call; the device part of
+// the library (where it lives) is not built here.
+namespace tiltgpu_sess {
+int fail(int code, const char*, ...) { return code; }
+}  // namespace tiltgpu_sess
+
 static int g_fail = 0;
 #define CHECK(c)                                                       \
     do {                                                               \
@@ -189,6 +195,36 @@ int main(int argc, char** argv) {
     const float tilt3[3] = {-30, 0, 30};
     or_ctf_phaseflip_f32(vol.data(), nx, ny, nz, tilt3, 3, vdef, -3000.f, phys, 8, xo.data(), 2);
     or_ctf_phaseflip_f64(vol.data(), nx, ny, nz, tilt3, 2, nullptr, -3000.f, phys, 16, xo.data(), 1);
+    // ---- tiltalign local corrections: a small case, an empty one, bad arguments
+    {
+        const int nb = 4, nvw = 3;
+        std::vector<double> obs(static_cast<size_t>(nb) * nvw * 2), beads(nb * 3);
+        for (int b = 0; b < nb; ++b) {
+            beads[3 * b] = 10.0 * b - 15.0;
+            beads[3 * b + 1] = 7.0 * b - 10.0;
+            beads[3 * b + 2] = 3.0 * b;
+            for (int v = 0; v < nvw; ++v) {
+                obs[(b * nvw + v) * 2] = (b == 2 && v == 1) ? NAN : beads[3 * b] + 0.3 * v;
+                obs[(b * nvw + v) * 2 + 1] = beads[3 * b + 1] - 0.2 * v;
+            }
+        }
+        tiltgpu_align_view av[3] = {{0.1, -0.2, 0.01, 1.0, -0.5}, {0, 0, 0, 1, 0}, {0.2, 0.1, -0.02, 1.01, 0.5}};
+        int32_t ov[64], opx[64], opy[64];
+        double odx[64], ody[64];
+        const int n = tiltgpu_local_corrections(obs.data(), beads.data(), av, nb, nvw, 40, 0.5f,
+                                                50.0, 40.0, 0, 0.0, 0.0, 64, ov, opx, opy, odx, ody);
+        CHECK(n > 0 && n <= 64);
+        CHECK(tiltgpu_local_corrections(obs.data(), beads.data(), av, nb, nvw, 40, 0.5f, 50.0, 40.0,
+                                        5, 0.0, 0.0, 2, ov, opx, opy, odx, ody) >= 0);
+        CHECK(tiltgpu_local_corrections(nullptr, nullptr, av, 0, nvw, 40, 0.5f, 50.0, 40.0, 0, 0.0,
+                                        0.0, 0, nullptr, nullptr, nullptr, nullptr, nullptr) == 0);
+        CHECK(tiltgpu_local_corrections(nullptr, beads.data(), av, nb, nvw, 40, 0.5f, 50.0, 40.0, 0,
+                                        0.0, 0.0, 4, ov, opx, opy, odx, ody) == TILTGPU_ERR_ARG);
+        CHECK(tiltgpu_write_local_corrections((dir + "/local.txt").c_str(), n > 64 ? 64 : n, ov,
+                                              opx, opy, odx, ody) == TILTGPU_OK);
+        CHECK(tiltgpu_write_local_corrections((dir + "/no/such/dir/l.txt").c_str(), 0, ov, opx,
+                                              opy, odx, ody) == TILTGPU_ERR_IO);
+    }
     std::fprintf(stderr, "host_asan_driver: %s\n", g_fail ? "FAILED" : "ok");
     return g_fail;
 }

@@ -17,7 +17,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SAN = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-g", "-O1",
        "-ffp-contract=off"]
 HOST = ["imod-rs_amd/host/textio.cpp", "imod-rs_amd/host/mrc.cpp", "imod-rs_amd/host/synth.cpp",
-        "imod-rs_amd/csrc/tuning.cpp"]
+        "imod-rs_amd/host/localalign.cpp", "imod-rs_amd/csrc/tuning.cpp"]
 
 
 def _run(cmd, **kw):
