@@ -240,6 +240,81 @@ def end_to_end_section(tg, P, ang, xt, local):
         shutil.rmtree(d, ignore_errors=True)
 
 
+# The `tilt` WBP rows the reference publishes whole-process times for
+# (BENCHMARKS.md:42-43, Apple Silicon): (width = rows, views, thickness,
+# seconds on 1 thread, seconds on 8 threads).
+PUBLISHED_TILT = [(256, 41, 128, 2.8, 0.9), (512, 61, 256, 33.3, 9.1)]
+
+
+def published_sizes_section(tg, cpu=True, reps=3):
+    """The reference's two published `tilt` sizes: the `tilt` command on files
+    (process wall including MRC read and write, median of `reps`, the files just
+    written and so in the page cache), the reconstruction alone on the resident
+    series (HIP events of the session), and the FP32 oracle on this host's
+    threads (the whole job) -- beside the reference's published seconds."""
+    import shutil
+    import subprocess
+    import tempfile
+    tilt = os.path.join(REPO, "imod-rs_amd", "_build", "tilt")
+    if not os.path.exists(tilt):
+        return {"skipped": "tilt executable not built"}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O  # noqa: E402 -- CPU-baseline leg only
+    out = []
+    for nx, nv, thick, ref1, ref8 in PUBLISHED_TILT:
+        ang = np.linspace(-60, 60, nv).astype(np.float32)
+        P = tg.synth_series(nx, nx, ang, vol_nz=thick, n_ellipsoids=60, n_beads=10, seed=nx)
+        vv = float(nx) * nx * thick * nv
+        res = {"workload": f"tilt WBP {nx}x{nx}x{nv} -> z={thick}, default options",
+               "reference_published_s": {"1_thread": ref1, "8_threads": ref8,
+                                         "host": "Apple Silicon, 8 cores (BENCHMARKS.md:3,42-43)"}}
+        d = tempfile.mkdtemp(prefix="tilt_pub_")
+        try:
+            inp, rec, tlt = (os.path.join(d, f) for f in ("ts.mrc", "ts.rec", "ts.tlt"))
+            tg.mrc_write(inp, P)
+            with open(tlt, "w") as fh:
+                fh.write("".join(f"{a:.9g}\n" for a in ang))
+            walls = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                r = subprocess.run([tilt, "-i", inp, "-o", rec, "-t", tlt, "-z", str(thick)],
+                                   capture_output=True, text=True, timeout=300)
+                walls.append(time.perf_counter() - t0)
+                if r.returncode != 0:
+                    res["error"] = r.stderr[-300:]
+                    break
+            if "error" not in res:
+                w = float(np.median(walls))
+                res["tilt_process_wall_s"] = round(w, 3)
+                res["speedup_vs_reference_8_threads"] = round(ref8 / w, 1)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        with tg.Session(nx, nx, nv, tg.TiltOptions(thickness=thick)) as s:
+            s.set_angles(ang)
+            s.upload(P)
+            s.run()
+            s.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                s.run()
+            s.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+        res["gpu_resident_ms"] = round(ms, 3)
+        res["gpu_resident_gvoxel_view_per_s"] = round(vv / ms / 1e6, 1)
+        if cpu:
+            nthreads = host_threads()
+            t0 = time.perf_counter()
+            O.tilt_run(P, ang, O.TiltOptions(thickness=thick), prec="f32", nthreads=nthreads)
+            dt = time.perf_counter() - t0
+            res["cpu_oracle"] = {"seconds": round(dt, 3), "cores": nthreads,
+                                 "gvoxel_view_per_s": round(vv / dt / 1e9, 3),
+                                 "note": "compute only (no file I/O)"}
+        out.append(res)
+        print(f"# published size: {res}", file=sys.stderr, flush=True)
+    return out
+
+
 def section_transforms_section(tg, P, cpu=True, reps=3):
     """newstack -x on the C1 stack (2048x2048x121): one seeded small rotation +
     shift per section applied on the device (xf_apply_kernel: 4 B read + 4 B
@@ -626,7 +701,10 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
                       "mix": "even WBP / odd SIRT-10; widths 256-1024, 41-121 views, z 64-300",
                       "budget_bytes": budget,
                       "scheduler": "one round in flight; arrivals during it join the next round; "
-                                   "backprojection launches merge tile depths"}}
+                                   "requests of <= 5e8 voxel-views join their variant's "
+                                   "shallowest backprojection launch",
+                      "slowdown": "latency / the same job's time alone on the device "
+                                  "(plain FP32 session; slabs: run_rows + read-back)"}}
     # batched capacity (all requests at once), FP32 and FP16 storage; FP32 also with
     # one backprojection launch per tile depth (the launches before merging)
     runs = [("f32", None), ("f16", None), ("f32", "0")]
@@ -649,9 +727,12 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
         print(f"# {key}: {dt:.2f} s", file=sys.stderr, flush=True)
     reqs = c3_requests(series, n_req, "f32")
     work = sum(r[3] for r in reqs)
+    alone = []  # each request's time alone on the device, for the slowdowns below
     t1 = time.perf_counter()
     for P, a, o, _ in reqs:
+        t2 = time.perf_counter()
         plain_run(tg, P, a, o)
+        alone.append((time.perf_counter() - t2) * 1e3)
     dtp = time.perf_counter() - t1
     plain_cap = n_req / dtp
     out["plain_f32_sequential"] = {"seconds": round(dtp, 3), "requests_per_s": round(plain_cap, 3),
@@ -674,10 +755,37 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
         slab_session.set_local(*loc1)
         slab_session.upload(P1)
         slab_session.run_rows(0, 64)  # filters the resident series once
+    slab_alone = None
+    if slab_session is not None:
+        ts = []
+        for y0 in (0, 640, 1280, 1920, 960):
+            t2 = time.perf_counter()
+            slab_session.run_rows(y0, y0 + 64)
+            slab_session.read_rows(y0, y0 + 64, stats=False)
+            ts.append((time.perf_counter() - t2) * 1e3)
+        slab_alone = float(np.median(ts))
+        out["slab_alone_ms"] = round(slab_alone, 2)
+
+    def slowdowns(lat, kinds):
+        # latency / the job's own time alone: stream request k repeats request
+        # k % n_req of the batched mix (same series, same SIRT count: n_req is even)
+        own, k = [], 0
+        for kind in kinds:
+            if kind == "recon":
+                own.append(alone[k % n_req])
+                k += 1
+            else:
+                own.append(slab_alone)
+        return lat / np.array(own, np.float64)
 
     def summary(lat, kinds, st=None):
         e = {"p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
              "p99_over_p50": round(pct(lat, 99) / pct(lat, 50), 2)}
+        if n_req % len(series) == 0 and len(series) % 2 == 0 and (
+                slab_alone is not None or not np.any(kinds == "slab")):
+            sd = slowdowns(lat, kinds)
+            e["slowdown_p50"] = round(pct(sd, 50), 2)
+            e["slowdown_p99"] = round(pct(sd, 99), 2)
         rec = kinds == "recon"
         e["reconstruction_p50_ms"] = round(pct(lat[rec], 50), 2)
         e["reconstruction_p99_ms"] = round(pct(lat[rec], 99), 2)
@@ -741,7 +849,7 @@ def main():
     ap.add_argument("--serve-batch-only", action="store_true",
                     help="only the batched and plain runs of the serving section (for profiles)")
     ap.add_argument("--serve-requests", type=int, default=64)
-    ap.add_argument("--stream-requests", type=int, default=150,
+    ap.add_argument("--stream-requests", type=int, default=225,
                     help="reconstructions per stream (a slab request rides along every third)")
     ap.add_argument("--no-sirt", action="store_true", help="skip the SIRT section")
     ap.add_argument("--no-xf", action="store_true",
@@ -893,6 +1001,11 @@ def main():
         except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
             line["end_to_end"] = {"error": str(e)[:300]}
         print(f"# end to end: {line['end_to_end']}", file=sys.stderr, flush=True)
+    if not a.no_e2e:
+        try:
+            line["published_sizes"] = published_sizes_section(tg, cpu=not a.no_cpu)
+        except Exception as e:  # noqa: BLE001 -- reported, the device figures stand
+            line["published_sizes"] = {"error": str(e)[:300]}
     if not a.no_ctf:
         try:
             line["ctfphaseflip"] = ctf_section(tg, P, cpu=not a.no_cpu, e2e=not a.no_e2e)

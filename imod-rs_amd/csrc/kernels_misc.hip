@@ -357,8 +357,6 @@ hipError_t launch_zc_f64(const float* angles, const float* zfac, int n_zfac, int
     return hipGetLastError();
 }
 
-// Independent FP32 FMA chains (eight per lane, one v_fma_f32 each, no packed
-// forms): the attainable non-packed FP32 VALU rate of the device.
 // VALU issue-rate calibration (diagnostics).  Each kernel runs 8 independent
 // chains per lane of one instruction mix, so that issue, not latency, bounds
 // it.  The instruction text is fixed with inline asm (no memory operations);

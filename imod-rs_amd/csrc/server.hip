@@ -51,6 +51,15 @@ constexpr int kSlots = 3;  // round slots: at most this many rounds in flight
 constexpr size_t kAdmitMax = 8;              // admissions per round issued
 constexpr int64_t kAdmitBytes = 2ll << 30;   // projection bytes uploaded per round issued
 
+// An item of at most this many voxel-views runs at the smallest tile depth of
+// its kernel variant in the round, in that depth's launch, instead of a launch
+// of its own depth.  A shallower tile costs a deep-tuned item up to about 40%
+// more per slice, while a launch of its own costs about a tile's time at the
+// tail plus the launch gap (tens of microseconds): at the backprojection's
+// 4 Gvoxel-view/ms, 5e8 voxel-views is where the two meet
+// (TILTGPU_SERVER_MERGE_WORK overrides it).
+constexpr double kMergeWork = 5e8;
+
 struct Req {
     int64_t id = 0;
     int kind = kRecon;
@@ -234,11 +243,13 @@ struct tiltgpu_server {
     // and join it, so rounds carry every request that arrived while the device
     // was busy.  TILTGPU_SERVER_ROUNDS overrides.
     int max_inflight = 1;
-    // Backprojection launches group requests by storage and geometry variant and
-    // run at the group's smallest tile depth (the depths are powers of two, so a
-    // request prepared for a deeper tile divides evenly; results do not depend
-    // on the depth).  TILTGPU_SERVER_MERGE_TILES=0 keeps one launch per depth.
+    // Backprojection launches group requests by storage, geometry variant and
+    // tile depth; requests of at most merge_work voxel-views join the launch of
+    // their variant's smallest depth (the depths are powers of two, so a request
+    // prepared for a deeper tile divides evenly; results do not depend on the
+    // depth).  TILTGPU_SERVER_MERGE_TILES=0 keeps one launch per depth.
     bool merge_tiles = true;
+    double merge_work = kMergeWork;
     int n_cu = 256;
     int64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;     // rounds
@@ -342,6 +353,7 @@ struct BpItem {
     BpParams p;
     int storage, two_row, local, tile_z;
     size_t lds;
+    double work;  // voxel-views of the item's rows
 };
 
 // One persistent launch per kernel variant; all groups' descriptors go to the
@@ -537,6 +549,7 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
             it.local = s->has_local ? 1 : 0;
             it.tile_z = s->tile_z;
             it.lds = bp_lds_bytes(s->lds_cap, it.two_row);
+            it.work = static_cast<double>(y1 - y0) * s->d.out_nz * s->d.out_nx * s->d.nz;
             items.push_back(it);
         };
         for (Req* r : prep) {
@@ -549,9 +562,26 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
             Req* sr = S->series[r->series];
             add(r, sr->s, r->y0, r->y1, 1, r->slab);
         }
-        const bool merge = S->merge_tiles;
-        auto key = [merge](const BpItem& a) {
-            return std::make_tuple(a.storage, a.two_row, a.local, merge ? 0 : a.tile_z);
+        // small items join the launch of their variant's smallest depth; every
+        // other item runs at its own depth
+        if (S->merge_tiles) {
+            std::map<std::tuple<int, int, int>, int> dmin;
+            for (const BpItem& it : items) {
+                auto k = std::make_tuple(it.storage, it.two_row, it.local);
+                auto f = dmin.find(k);
+                if (f == dmin.end() || it.tile_z < f->second) dmin[k] = it.tile_z;
+            }
+            for (BpItem& it : items) {
+                if (it.work > S->merge_work) continue;
+                // the padded thickness is a multiple of every smaller depth, and
+                // the LDS bound of a deeper tile covers a shallower one
+                const int tz = dmin[std::make_tuple(it.storage, it.two_row, it.local)];
+                it.p.ntz = it.p.nz_pad / tz;
+                it.tile_z = tz;
+            }
+        }
+        auto key = [](const BpItem& a) {
+            return std::make_tuple(a.storage, a.two_row, a.local, a.tile_z);
         };
         std::stable_sort(items.begin(), items.end(),
                          [&](const BpItem& a, const BpItem& b) { return key(a) < key(b); });
@@ -560,17 +590,7 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
         while (i < items.size()) {
             size_t j = i;
             std::vector<BpItem> grp;
-            int tz = items[i].tile_z;
-            while (j < items.size() && key(items[j]) == key(items[i])) {
-                tz = std::min(tz, items[j].tile_z);
-                grp.push_back(items[j++]);
-            }
-            // every item runs at the group's depth: its padded thickness is a
-            // multiple of it, and its LDS bound for a deeper tile covers it
-            for (BpItem& it : grp) {
-                it.p.ntz = it.p.nz_pad / tz;
-                it.tile_z = tz;
-            }
+            while (j < items.size() && key(items[j]) == key(items[i])) grp.push_back(items[j++]);
             groups.push_back(std::move(grp));
             i = j;
         }
@@ -904,6 +924,7 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out) {
         if (v >= 1 && v <= kSlots) S->max_inflight = v;
     }
     if (const char* e = std::getenv("TILTGPU_SERVER_MERGE_TILES")) S->merge_tiles = std::atoi(e) != 0;
+    if (const char* e = std::getenv("TILTGPU_SERVER_MERGE_WORK")) S->merge_work = std::atof(e);
     bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&S->up_stream, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& st : S->down_stream)
