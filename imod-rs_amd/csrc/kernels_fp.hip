@@ -9,7 +9,11 @@
 // in the reference's order (oz, then ox) and with the reference's roundings:
 // the sinogram equals the serial scatter bit for bit.  Every voxel is evaluated
 // once per lane window instead of once per bin it touches, and the loop has no
-// per-bin range search.
+// per-bin range search.  For volume rows staged in LDS, each lane's voxel
+// window of a z row is exact (trimmed by evaluating px at its ends), so a voxel
+// needs no clamp and no validity test, and the sums of the two bins the lane is
+// on ride in registers: a voxel costs one LDS store and one LDS load instead of
+// a read-modify-write of two sums (the walk is then bound by VALU issue).
 //
 // Two-row geometry (x-axis tilt / local alignment): a scatter with the
 // coordinates and weights of main.rs:840-889.  Each term is rounded to a
@@ -74,12 +78,15 @@ __device__ void fp_bin_scan(const FpParams& P, int x, int iy, int p, float* out)
 }
 
 // Per-lane constants of one walk window: bins x0 .. x0+B-1 of (view p, row iy).
-// The lane's LDS slot s is at byte s * 4 * kFpWG + 4 * threadIdx.x; slots
-// 2..B+1 hold the bins, slots 0, 1, B+2 and B+3 absorb the terms of voxels that
-// land outside the window.
+// The lane's LDS slot s is at byte s * 4 * kFpWG + 4 * threadIdx.x.  In the
+// clamped walk slots 2..B+1 hold the bins, and slots 0, 1, B+2 and B+3 absorb
+// the terms of voxels that land outside the window; with exact windows slots
+// 1..B hold the bins, slots 0 and B+1 take the terms of bins x0-1 and x0+B, and
+// slot B+2 is the register walk's look-ahead past the window.
 struct WalkLane {
-    float ct, cxt, inv_c, cx, ocx, fnx;
+    float ct, cxt, inv_c, cx, ocx, fnx, fin;
     float k_lo, k_hi;  // px = x0-1 and px = x0+B bound the voxels that reach the window
+    float b_lo, b_hi;  // exact windows: max(x0-1, 0) <= px < min(x0+B, in_nx-1)
     float mbase;
     unsigned nvalid;   // in_nx - 1
     int lo, hi;        // x0 - 2, x0 + B: clamp of floor(px)
@@ -87,7 +94,10 @@ struct WalkLane {
     int last;
 };
 
-__device__ __forceinline__ WalkLane walk_lane(const FpParams& P, float4 g, int x0, int B) {
+// first: the slot of bin x0 (2 with the clamped walk's two dummies below the
+// window, 1 with an exact window's one).
+__device__ __forceinline__ WalkLane walk_lane(const FpParams& P, float4 g, int x0, int B,
+                                              int first) {
     WalkLane L;
     L.ct = g.x;
     L.cxt = g.z;
@@ -95,15 +105,18 @@ __device__ __forceinline__ WalkLane walk_lane(const FpParams& P, float4 g, int x
     L.cx = P.cx;
     L.ocx = P.ocx;
     L.fnx = static_cast<float>(P.out_nx);
+    L.fin = static_cast<float>(P.in_nx);
     L.k_lo = static_cast<float>(x0 - 1) - P.cx;
     L.k_hi = static_cast<float>(x0 + B) - P.cx;
+    L.b_lo = static_cast<float>(max(x0 - 1, 0));
+    L.b_hi = static_cast<float>(min(x0 + B, P.in_nx - 1));
     // Bound on |computed - exact| of px and of the inverted window ends, in px
     // units: a few roundings of magnitudes below in_nx + out_nx + |zc|.
     L.mbase = 3e-7f * (4.f * L.fnx + 5.f * static_cast<float>(P.in_nx) + 64.f);
     L.nvalid = static_cast<unsigned>(P.in_nx - 1);
     L.lo = x0 - 2;
     L.hi = x0 + B;
-    L.qoff = 4u * threadIdx.x + static_cast<unsigned>(2 - x0) * (4u * kFpWG);
+    L.qoff = 4u * threadIdx.x + static_cast<unsigned>(first - x0) * (4u * kFpWG);
     L.last = P.out_nx - 1;
     return L;
 }
@@ -117,6 +130,107 @@ __device__ __forceinline__ void walk_window(const WalkLane& L, float zcv, int& e
     const float thi = (L.k_hi - zcv) * L.inv_c + L.ocx;
     e0 = static_cast<int>(fminf(fmaxf(ceilf(tlo - margin), 0.f), L.fnx));
     e1 = static_cast<int>(fminf(fmaxf(floorf(thi + margin) + 1.f, 0.f), L.fnx));
+}
+
+// px of the voxel at dx = ox - ocx in the reference's operation order
+// (main.rs:858-860; single-row geometry, so (dx*cos_t)*cos_xt is dx*cos_t).
+__device__ __forceinline__ float walk_px(const WalkLane& L, float dx, float zcv) {
+    return (dx * L.ct + zcv) + L.cx;
+}
+
+// The exact voxel window of z row zc: [e0, e1) holds exactly the voxels with
+// b_lo <= px < b_hi, i.e. max(x0-1, 0) <= floor(px) <= min(x0+B-1, in_nx-2):
+// the valid voxels (0 <= px0 < in_nx-1) that add a term to a bin of the lane's
+// window.  px is non-decreasing in ox (each rounding step is monotonic and
+// cos_t > 0).  The inverted bounds t(b_lo), t(b_hi) carry the walk_window
+// margin m, so ceil(t(b_lo) - m) is the first voxel of the window or the one
+// before it, and floor(t(b_hi) + m) + 1 its end or one past it, whenever
+// 2m < 1: one evaluation of px, exactly as the walk computes it, settles each
+// end.  (m stays far below 1/2 for every staged shape; the loops cover the
+// rest.)
+__device__ __forceinline__ void walk_window_exact(const WalkLane& L, float zcv, int& e0,
+                                                  int& e1) {
+    const float margin = (L.mbase + 1.8e-6f * fabsf(zcv)) * L.inv_c + 1e-3f;
+    const float tlo = ((L.b_lo - L.cx) - zcv) * L.inv_c + L.ocx;
+    const float thi = ((L.b_hi - L.cx) - zcv) * L.inv_c + L.ocx;
+    e0 = static_cast<int>(fminf(fmaxf(ceilf(tlo - margin), 0.f), L.fnx));
+    e1 = static_cast<int>(fminf(fmaxf(floorf(thi + margin) + 1.f, 0.f), L.fnx));
+    if (margin < 0.5f) {  // uniform over the wave (one view, one z row)
+        e0 += walk_px(L, static_cast<float>(e0) - L.ocx, zcv) < L.b_lo ? 1 : 0;
+        e1 -= walk_px(L, static_cast<float>(e1 - 1) - L.ocx, zcv) >= L.b_hi ? 1 : 0;
+    } else {
+        while (e0 < e1 && walk_px(L, static_cast<float>(e0) - L.ocx, zcv) < L.b_lo) ++e0;
+        while (e1 > e0 && walk_px(L, static_cast<float>(e1 - 1) - L.ocx, zcv) >= L.b_hi) --e1;
+    }
+}
+
+// One voxel of an exact window: floor(px) lies in [x0-1, x0+B-1], so the pair
+// of slots it adds to is in [0, B+1] with no clamp, and the voxel is valid.
+template <int B>
+__device__ __forceinline__ void walk_voxel_exact(const WalkLane& L, char* lds, float dx,
+                                                 float zcv, float v) {
+    const float px = walk_px(L, dx, zcv);
+    const int fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    float* q = reinterpret_cast<float*>(lds + (static_cast<unsigned>(fi) * (4u * kFpWG) + L.qoff));
+    q[0] += v * (1.0f - frac);
+    q[kFpWG] += v * frac;
+}
+
+// Register-carried sums for exact windows.  While floor(px) advances by at most
+// one per voxel, the lane keeps the running sums of bins fi and fi+1 of its last
+// voxel in registers (r0, r1) and the stored sum of bin fi+2 loaded one voxel
+// ahead (la).  A voxel that stays on bin fi adds to r0 and r1; one that moves
+// to fi+1 makes r1 and la the new pair.  Each voxel stores r0 -- the final sum
+// of bin fi once the walk moves on, an intermediate one that a later store
+// replaces otherwise -- so a voxel costs one LDS store and one LDS load instead
+// of reading and writing two sums.  The terms and their order are those of
+// walk_voxel_exact: each bin's sum is S + t1 + t2 + ... in voxel order.
+// (Slot B+2, the look-ahead of the window's last bin, is loaded but never used.)
+struct RegWalk {
+    float r0, r1, la;
+    float* q;  // slot of bin fi
+    int fi;
+};
+
+// qb: the lane's slot of bin 0 (lds + qoff; LDS addresses wrap at 2^32)
+__device__ __forceinline__ void reg_begin(const WalkLane& L, char* qb, float dx, float zcv,
+                                          float v, RegWalk& W) {
+    const float px = walk_px(L, dx, zcv);
+    W.fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    W.q = reinterpret_cast<float*>(qb + W.fi * (4 * kFpWG));
+    W.r0 = W.q[0] + v * (1.0f - frac);
+    W.r1 = W.q[kFpWG] + v * frac;
+    W.la = W.q[2 * kFpWG];
+}
+
+__device__ __forceinline__ void reg_step(const WalkLane& L, char* qb, float dx, float zcv,
+                                         float v, RegWalk& W) {
+    const float px = walk_px(L, dx, zcv);
+    const int fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    const float a = v * (1.0f - frac);
+    const float b = v * frac;
+    W.q[0] = W.r0;
+    float* q = reinterpret_cast<float*>(qb + fi * (4 * kFpWG));
+    const float la = q[2 * kFpWG];
+    const bool adv = fi != W.fi;  // fi == W.fi + 1
+    const float r0 = adv ? W.r1 : W.r0;
+    const float r1 = adv ? W.la : W.r1;
+    W.r0 = r0 + a;
+    W.r1 = r1 + b;
+    W.la = la;
+    W.q = q;
+    W.fi = fi;
+}
+
+// floor(px) advances by at most one per voxel when cos_t < 1 - M*2^-21, M
+// bounding |px| and its partial results (in_nx + out_nx + |zc|): each of px's
+// three roundings is within half an ulp of M, so consecutive px differ by at
+// most cos_t + 3 ulp(M) < 1.  Uniform over the wave.
+__device__ __forceinline__ bool reg_walk_ok(const WalkLane& L, float zcv) {
+    return L.ct < 1.0f - (L.fnx + L.fin + fabsf(zcv)) * 4.76837158203125e-7f;
 }
 
 // One voxel, main.rs:860-887 with px0 = floor(px): v*(1-frac) to bin px0 and
@@ -147,7 +261,10 @@ template <int B, bool STAGED>
 __device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
                                          const float* __restrict__ vz, float zcv) {
     int e0, e1;
-    walk_window(L, zcv, e0, e1);
+    if constexpr (STAGED)
+        walk_window_exact(L, zcv, e0, e1);
+    else
+        walk_window(L, zcv, e0, e1);
     // dx = ox - ocx is exact in f32 and advances exactly by 1 per voxel
     float dx = static_cast<float>(e0) - L.ocx;
     const float dx_end = static_cast<float>(e1) - L.ocx;
@@ -155,16 +272,35 @@ __device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
     if constexpr (STAGED) {
         // two voxels per step (one paired LDS read), the odd one after; dx + 1
         // is exact, so every voxel sees the same dx as in a one-by-one walk
-        (void)dx_end;
-        const int n = e1 - e0;
-        int i = 0;
-        for (; i + 1 < n; i += 2) {
-            const float v0 = vp[i], v1 = vp[i + 1];
-            walk_voxel<B>(L, lds, dx, zcv, v0);
-            walk_voxel<B>(L, lds, dx + 1.0f, zcv, v1);
-            dx += 2.0f;
+        const float* vend = vz + e1;
+        const float* vlast = vend - 1;  // a pair starts before the last voxel
+        if (reg_walk_ok(L, zcv)) {
+            if (vp < vend) {
+                char* const qb = lds + L.qoff;
+                RegWalk W;
+                reg_begin(L, qb, dx, zcv, vp[0], W);
+                ++vp;
+                dx += 1.0f;
+                for (; vp < vlast; vp += 2) {
+                    const float v0 = vp[0], v1 = vp[1];
+                    reg_step(L, qb, dx, zcv, v0, W);
+                    reg_step(L, qb, dx + 1.0f, zcv, v1, W);
+                    dx += 2.0f;
+                }
+                if (vp < vend) reg_step(L, qb, dx, zcv, vp[0], W);
+                W.q[0] = W.r0;
+                W.q[kFpWG] = W.r1;
+            }
+        } else {
+            for (; vp < vlast; vp += 2) {
+                const float v0 = vp[0], v1 = vp[1];
+                walk_voxel_exact<B>(L, lds, dx, zcv, v0);
+                walk_voxel_exact<B>(L, lds, dx + 1.0f, zcv, v1);
+                dx += 2.0f;
+            }
+            if (vp < vend) walk_voxel_exact<B>(L, lds, dx, zcv, vp[0]);
         }
-        if (i < n) walk_voxel<B>(L, lds, dx, zcv, vp[i]);
+        (void)dx_end;
     } else {
         const float* vlast = vz + L.last;
         float vn = *(vp < vlast ? vp : vlast);
@@ -179,11 +315,11 @@ __device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
 
 template <int B>
 __device__ __forceinline__ void walk_store(const FpParams& P, const float* a, int x0, int iy,
-                                           int p) {
+                                           int p, int first) {
     float* row = P.sino + (static_cast<int64_t>(p) * P.in_ny + iy) * P.in_nx;
 #pragma unroll
     for (int j = 0; j < B; ++j)
-        if (x0 + j < P.in_nx) row[x0 + j] = a[(j + 2) * kFpWG];
+        if (x0 + j < P.in_nx) row[x0 + j] = a[(j + first) * kFpWG];
 }
 
 // Unstaged form: each wave walks its own (view, row) reading the volume from
@@ -199,7 +335,7 @@ __device__ __forceinline__ void fp_walk_global(const FpParams& P, float* lds, in
             if (x0 + j < P.in_nx) fp_bin_scan(P, x0 + j, iy, p, row + x0 + j);
         return;
     }
-    const WalkLane L = walk_lane(P, g, x0, B);
+    const WalkLane L = walk_lane(P, g, x0, B, 2);
 #pragma unroll
     for (int s = 2; s < B + 2; ++s) a[s * kFpWG] = 0.f;
     const float* zc = P.zc_tab + static_cast<int64_t>(p) * P.nz_pad;
@@ -211,7 +347,7 @@ __device__ __forceinline__ void fp_walk_global(const FpParams& P, float* lds, in
                            vrow + static_cast<int64_t>(oz) * P.out_nx, zcv);
         zcv = zc_next;
     }
-    walk_store<B>(P, a, x0, iy, p);
+    walk_store<B>(P, a, x0, iy, p, 2);
 }
 
 // Rows of at most kStageNx voxels are staged in LDS: the 4 waves of a workgroup
@@ -256,14 +392,15 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
     const bool live = wave_live && x0 < P.in_nx;
     const int pv = wave_live ? p : 0;
     const float4 g = P.vgeom[pv];
-    const WalkLane L = walk_lane(P, g, x0, B);
+    // exact windows: slot 0 and slot B+1 take the terms of bins x0-1 and x0+B
+    const WalkLane L = walk_lane(P, g, x0, B, 1);
 #pragma unroll
-    for (int s = 2; s < B + 2; ++s) a[s * kFpWG] = 0.f;
+    for (int s = 1; s < B + 1; ++s) a[s * kFpWG] = 0.f;
     const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
     const int nx = P.out_nx;
     const int nz = P.out_nz;
     const int t = static_cast<int>(threadIdx.x);
-    float* stg = lds + (B + 4) * kFpWG;
+    float* stg = lds + (B + 3) * kFpWG;
     const int ns = fp_stage_stride(nx);
     const float* vrow = P.vol + static_cast<int64_t>(iy) * nz * nx;
     float r[kStageRegs];
@@ -292,7 +429,7 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
         zcv = zc_next;
         __syncthreads();
     }
-    if (live) walk_store<B>(P, a, x0, iy, p);
+    if (live) walk_store<B>(P, a, x0, iy, p, 1);
 }
 
 __device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk) {
@@ -303,7 +440,7 @@ __device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk)
     }
 }
 
-__global__ void __launch_bounds__(kFpWG) fp_walk_kernel(FpParams P, int nblk) {
+__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_kernel(FpParams P, int nblk) {
     extern __shared__ float fp_lds[];
     fp_block(P, fp_lds, xcd_remap(blockIdx.x, nblk));
 }
@@ -320,7 +457,7 @@ __host__ __device__ inline int fp_blocks(const FpParams& p) {
 // a range that starts at a multiple of 8 and is padded to one, so that its
 // workgroups spread evenly over the 8 XCDs (a contiguous run of each series per
 // XCD); padding workgroups exit at once.
-__global__ void __launch_bounds__(kFpWG) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
+__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
     extern __shared__ float fp_lds[];
     const int b = blockIdx.x;
     if (b >= nblk) return;
@@ -502,8 +639,12 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
 }  // namespace
 
 size_t fp_lds_bytes(const FpParams& p) {
+    // unstaged workgroups: B + 4 slots per lane; staged: B + 3 slots and the
+    // two staged volume rows
     size_t b = static_cast<size_t>(p.bins_per_lane + 4) * kFpWG * sizeof(float);
-    if (p.out_nx <= kStageNx) b += 2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float);
+    if (p.out_nx <= kStageNx)
+        b = std::max(b, static_cast<size_t>(p.bins_per_lane + 3) * kFpWG * sizeof(float) +
+                            2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float));
     return b;
 }
 
