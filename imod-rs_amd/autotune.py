@@ -2,8 +2,10 @@
 
 Measures, for every backprojection kernel variant (single-row; two-row with
 x-axis tilt; two-row with local shifts) x projection storage (f32, f16) x width
-x thickness class, the backprojection time at each tile depth (32, 64, 128, 256),
-and, per width, the forward-projection time at 4, 8 and 16 bins per lane.  The
+x thickness class, the backprojection time at each tile depth (32, 64, 128, 256)
+and at the other built launch shapes (two views staged per barrier at depths
+32-128; two x positions per lane, tile x 128, at depths 32 and 64), and, per
+width, the forward-projection time at 4, 8 and 16 bins per lane.  The
 measurements are written as DIR/autotune.json and turned into the C++ table
 csrc/tile_table.inc (compiled into the library; sessions and the server's
 batched launches choose their shapes from it) and a markdown summary.
@@ -34,13 +36,16 @@ VARIANTS = [  # (name, two_row, local)
 WIDTHS = [256, 512, 1024, 2048]
 THICK = [64, 128, 256, 512]
 TILES = [32, 64, 128, 256]
+# (tile depth, views per stage, x per lane) beyond shape (1, 1)
+SHAPES = [(32, 2, 1), (64, 2, 1), (128, 2, 1), (32, 1, 2), (64, 1, 2)]
 FP_WIDTHS = [256, 384, 512, 768, 1024, 1536, 2048]
 FP_BINS = [4, 8, 16]
 
 
-def bp_time(P, th, two_row, local, storage, tz, reps=3):
+def bp_time(P, th, two_row, local, storage, tz, reps=3, vps=1, xpl=1):
     """Median backprojection ms of a square series (as many rows as columns, so
-    that the launch fills the device as a real one does) at tile depth tz."""
+    that the launch fills the device as a real one does) at tile depth tz and
+    launch shape (vps views per stage, xpl x positions per lane)."""
     nviews, rows, w = P.shape
     ang = np.linspace(-60.0, 60.0, nviews).astype(np.float32)
     rng = np.random.default_rng(w + th)
@@ -54,6 +59,7 @@ def bp_time(P, th, two_row, local, storage, tz, reps=3):
             s.set_local(one, one, np.full(nviews, w / 2, np.float32), np.full(nviews, rows / 2, np.float32),
                         rng.normal(0, 2, nviews).astype(np.float32), rng.normal(0, 2, nviews).astype(np.float32))
         s.set_tuning(tz, 0)
+        s.set_bp_shape(vps, xpl)
         s.upload(P)
         s.run()
         ts = []
@@ -91,44 +97,64 @@ def pad(th, tz):
     return (th + tz - 1) // tz * tz
 
 
+def shape_key(tz, vps, xpl):
+    return f"{tz}/{vps}/{xpl}"
+
+
 def emit(meas, out):
     """Write tile_table.inc and autotune.md from the measurements.
 
-    Backprojection time is proportional to the padded depth: t = pad(th, tz) *
-    k(tz) for a variant, storage and width.  The table stores k(tz) (the median
-    over the measured thicknesses of t / pad, relative to k(128) = 1000); the
-    library picks the depth that minimises pad(out_nz, tz) * k(tz) at the
-    nearest measured width."""
+    Per kernel variant, storage, width and thickness, each launch shape's time is
+    stored as a per-padded-slice cost k = t / pad(th, tz), relative to shape
+    (128, 1, 1) of the same measurement = 1000.  The library takes the
+    measurement nearest in width, then thickness, and picks the shape that
+    minimises pad(out_nz, tz) * k."""
     md = ["# Autotuned launch shapes (gfx950)", "",
-          "Written by `python imod-rs_amd/autotune.py --out DIR`.  Backprojection:",
-          "median ms per launch of a w x w series at 61 views (w output rows), and the",
-          "per-padded-slice cost k(tz) derived from them (relative, k(128) = 1000);",
-          "forward projection: median ms of one reprojection of a w x w x 128 volume",
-          "at 61 views.", "",
-          "## Backprojection tile depth", "",
+          "Written by `python imod-rs_amd/autotune.py --out DIR`.  Backprojection: median",
+          "ms per launch of a w x w series at 61 views (w output rows) for every launch",
+          "shape -- tile depth tz, views staged per barrier vps, x positions per lane xpl",
+          "(tile x 64 * xpl); forward projection: median ms of one reprojection of a",
+          "w x w x 128 volume at 61 views.", "",
+          "## Backprojection tile depth (1 view per stage, 1 x per lane)", "",
           "| variant | storage | width | thickness | tz=32 | tz=64 | tz=128 | tz=256 | fastest |",
           "|---|---|---|---|---|---|---|---|---|"]
-    rows = []
-    groups = {}
+    cost_rows = []
     for m in meas["bp"]:
-        groups.setdefault((m["two_row"], m["local"], m["half"], m["width"]), []).append(m)
         ms = m["ms"]
         ok = {int(k): v for k, v in ms.items() if v is not None}
         best = min(ok, key=ok.get)
         cells = [f"{ms[str(tz)]:.3f}" if ms.get(str(tz)) is not None else "-" for tz in TILES]
         md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | {m['thickness']} | "
                   + " | ".join(cells) + f" | {best} |")
-    md += ["", "Per-padded-slice cost k(tz), relative to k(128) = 1000:", "",
-           "| two_row | local | half | width | k32 | k64 | k128 | k256 |", "|---|---|---|---|---|---|---|---|"]
-    for key, ms in sorted(groups.items()):
-        k = {}
-        for tz in TILES:
-            vals = [m["ms"][str(tz)] / pad(m["thickness"], tz) for m in ms if m["ms"].get(str(tz)) is not None]
-            k[tz] = float(np.median(vals)) if vals else None
-        base = k[128] if k[128] else min(v for v in k.values() if v)
-        kk = [int(round(1000 * k[tz] / base)) if k[tz] else 0 for tz in TILES]
-        rows.append((*key, *kk))
-        md.append("| %d | %d | %d | %d | %d | %d | %d | %d |" % (*key, *kk))
+    md += ["", "## Other launch shapes", "",
+           "| variant | storage | width | thickness | " + " | ".join(
+               f"tz={tz} vps={v} xpl={x}" for tz, v, x in SHAPES) + " |",
+           "|---|---|---|---|" + "---|" * len(SHAPES)]
+    for m in meas["bp"]:
+        sh = m.get("shapes", {})
+        md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | "
+                  f"{m['thickness']} | " + " | ".join(
+                      f"{sh[shape_key(*c)]:.3f}" if sh.get(shape_key(*c)) is not None else "-"
+                      for c in SHAPES) + " |")
+    md += ["", "## The library's choice at the measured thickness", "",
+           "The shape minimising pad(thickness, tz) * k, and its measured ms against the",
+           "fastest tile depth of shape (1, 1).", "",
+           "| variant | storage | width | thickness | chosen (tz, vps, xpl) | ms | best (1, 1) ms |",
+           "|---|---|---|---|---|---|---|"]
+    for m in meas["bp"]:
+        th = m["thickness"]
+        cand = [((tz, 1, 1), m["ms"].get(str(tz))) for tz in TILES]
+        cand += [(c, m.get("shapes", {}).get(shape_key(*c))) for c in SHAPES]
+        cand = [(c, t) for c, t in cand if t is not None]
+        base_t = m["ms"].get("128") or min(t / pad(th, c[0]) * pad(th, 128) for c, t in cand)
+        base = base_t / pad(th, 128)
+        for (tz, v, x), t in cand:
+            cost_rows.append((m["two_row"], m["local"], m["half"], m["width"], th, tz, v, x,
+                              int(round(1000 * (t / pad(th, tz)) / base))))
+        ch = min(cand, key=lambda ct: ct[1])  # pad(th, tz) * k is proportional to t
+        b11 = min(t for c, t in cand if c[1:] == (1, 1))
+        md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | {th} | "
+                  f"({ch[0][0]}, {ch[0][1]}, {ch[0][2]}) | {ch[1]:.3f} | {b11:.3f} |")
     md += ["", "## Forward-projection bins per lane", "",
            "| width | 4 | 8 | 16 | chosen |", "|---|---|---|---|---|"]
     fp_rows = []
@@ -139,12 +165,15 @@ def emit(meas, out):
         md.append(f"| {m['width']} | " + " | ".join(f"{ms[b]:.3f}" for b in FP_BINS) + f" | {best} |")
     with open(os.path.join(out, "tile_table.inc"), "w") as fh:
         fh.write("// Measured launch shapes for gfx950, written by imod-rs_amd/autotune.py.\n")
-        fh.write("// Backprojection: per-padded-slice cost of tile depth 32, 64, 128, 256 (relative,\n")
-        fh.write("// k128 = 1000; 0 = not measured) per kernel variant, storage and width.\n")
-        fh.write("// {two_row, local, half, in_nx, k32, k64, k128, k256}\n")
-        fh.write("static const BpTune kBpTable[] = {\n")
-        for r in rows:
-            fh.write("    {%d, %d, %d, %d, %d, %d, %d, %d},\n" % r)
+        fh.write("// Backprojection: per-padded-slice cost k of each launch shape (tile depth,\n")
+        fh.write("// views per LDS stage, x positions per lane) per kernel variant, storage,\n")
+        fh.write("// width and thickness, relative to shape (128, 1, 1) = 1000.\n")
+        fh.write("// {two_row, local, half, in_nx, thickness, tile_z, views_per_stage, x_per_lane, k}\n")
+        fh.write("static const BpCost kBpCostTable[] = {\n")
+        for r in cost_rows:
+            fh.write("    {%d, %d, %d, %d, %d, %d, %d, %d, %d},\n" % r)
+        if not cost_rows:
+            fh.write("    {-1, 0, 0, 0, 0, 0, 0, 0, 0},\n")
         fh.write("};\n// Forward projection: fastest bins per lane per width.  {in_nx, bins_per_lane}\n")
         fh.write("static const FpTune kFpTable[] = {\n")
         for r in fp_rows:
@@ -175,10 +204,12 @@ def main():
                 P = np.ones((61, w, w), np.float32)
                 for th in thick:
                     ms = {str(tz): bp_time(P, th, two, loc, storage, tz) for tz in TILES}
+                    shapes = {shape_key(tz, v, x): bp_time(P, th, two, loc, storage, tz, vps=v, xpl=x)
+                              for tz, v, x in SHAPES}
                     meas["bp"].append({"variant": name, "two_row": two, "local": loc,
                                        "half": int(storage != "f32"), "width": w, "thickness": th,
-                                       "ms": ms})
-                    print(f"# bp {name} {storage} w={w} z={th}: {ms}", flush=True)
+                                       "ms": ms, "shapes": shapes})
+                    print(f"# bp {name} {storage} w={w} z={th}: {ms} {shapes}", flush=True)
     for w in FP_WIDTHS if not a.quick else FP_WIDTHS[::3]:
         ms = {str(b): fp_time(w, b) for b in FP_BINS}
         meas["fp"].append({"width": w, "ms": ms})

@@ -135,18 +135,23 @@ __device__ __forceinline__ void grid_shift(const BpParams& P, int view, float x,
     ody = ay + fy * (by - ay);
 }
 
-constexpr int kSlots = 2;  // staged entries prefetched per thread
+constexpr int kSlots = 2;  // staged entries prefetched per thread (one view per stage)
 
-// Per-view state of one lane and of its wave.  a, ldx, fy and row are per lane;
-// the staging geometry (seg_lo .. total, inv_L) is wave-uniform and equal in all
-// four waves (they hold the same 64 ox positions), so it lives in SGPRs.
-struct Vs {
-    float a;      // (dx*cos_t)*cos_xt
-    float ldx;    // x shift (0 when none)
-    float fy;     // weight of row r+1 (0: one row)
-    int row;      // row pair start r, or -1 (no contribution)
+// Staging geometry of one view for the workgroup's tile: wave-uniform and equal
+// in all four waves (they hold the same x positions), so it lives in SGPRs.
+struct Stg {
     int seg_lo, L, r_lo, NR, total, shift;
     float inv_L;  // 1/L (row of a linear entry index, two-row geometry)
+};
+
+// Per-view state of one lane: for each of its XPL x positions (ox + 64*j)
+// a, ldx, fy and row, plus the view's staging geometry.
+template <int XPL>
+struct Vs : Stg {
+    float a[XPL];    // (dx*cos_t)*cos_xt
+    float ldx[XPL];  // x shift (0 when none)
+    float fy[XPL];   // weight of row r+1 (0: one row)
+    int row[XPL];    // row pair start r, or -1 (no contribution)
 };
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -168,75 +173,98 @@ __device__ __forceinline__ int b128_lane_x(int lane) {
     return (lane & 32) | ((__builtin_popcount(b) & 1) << 4) | ((b >> 1) << 2) | (lane & 3);
 }
 
-template <bool TWO, bool LOCAL>
-__device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int ox, int iy,
-                                         int zt0, int TZ) {
-    Vs s;
+template <bool TWO, bool LOCAL, int XPL>
+__device__ __forceinline__ Vs<XPL> view_setup(const BpParams& P, int p, const float (&dx)[XPL],
+                                              const int (&ox)[XPL], int iy, int zt0, int TZ) {
+    Vs<XPL> s;
     const TILTGPU_CONSTANT BpView* V = cptr(P.views) + p;
     const float cos_t = V->cos_t, cos_xt = V->cos_xt, sin_xt = V->sin_xt;
     const int mode = LOCAL ? V->shift_mode : 0;
     s.shift = mode;
-    s.a = (dx * cos_t) * cos_xt;
-    float ldx = 0.f, ldy = 0.f;
-    if (LOCAL) {
-        if (mode == 1) {
-            ldx = V->sdx;
-            ldy = V->sdy;
-        } else if (mode == 2) {
-            grid_shift(P, p, static_cast<float>(ox), static_cast<float>(iy), ldx, ldy);
-        }
-    }
-    s.ldx = ldx;
     const TILTGPU_CONSTANT float* zc = cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad;
     const float z_first = zc[zt0], z_last = zc[zt0 + TZ - 1];
-    // with local shifts the shift is always added: ldx = +0 without one, and
-    // x + 0 == x for every x but -0 (same floor and fraction)
-    float pa = (s.a + z_first) + P.cx;
-    float pb = (s.a + z_last) + P.cx;
-    if (LOCAL) {
-        pa = pa + ldx;
-        pb = pb + ldx;
+    float ldy[XPL];
+    int flo = INT_MAX, fhi = INT_MIN;
+#pragma unroll
+    for (int j = 0; j < XPL; ++j) {
+        s.a[j] = (dx[j] * cos_t) * cos_xt;
+        float lx = 0.f, ly = 0.f;
+        if (LOCAL) {
+            if (mode == 1) {
+                lx = V->sdx;
+                ly = V->sdy;
+            } else if (mode == 2) {
+                grid_shift(P, p, static_cast<float>(ox[j]), static_cast<float>(iy), lx, ly);
+            }
+        }
+        s.ldx[j] = lx;
+        ldy[j] = ly;
+        // with local shifts the shift is always added: ldx = +0 without one, and
+        // x + 0 == x for every x but -0 (same floor and fraction)
+        float pa = (s.a[j] + z_first) + P.cx;
+        float pb = (s.a[j] + z_last) + P.cx;
+        if (LOCAL) {
+            pa = pa + lx;
+            pb = pb + lx;
+        }
+        // floor is monotonic, so the floors of the extremes bound the segment
+        flo = min(flo, cvt_floor(fminf(pa, pb)));
+        fhi = max(fhi, cvt_floor(fmaxf(pa, pb)));
     }
-    // floor is monotonic, so the floors of the extremes bound the segment
-    const int flo = cvt_floor(fminf(pa, pb)), fhi = cvt_floor(fmaxf(pa, pb));
     int seg_lo, seg_hi;
     if (mode == 2) {
         seg_lo = uni(wave_min_i(flo));
         seg_hi = uni(wave_max_i(fhi));
     } else {
-        // proj_x is monotonic in ox (a constant shift keeps it so): extremes at lanes 0, 63
+        // proj_x is monotonic in ox (a constant shift keeps it so): the extremes
+        // are at x offset 0 (lane 0, j = 0) and 64*XPL-1 (lane 63, j = XPL-1)
         seg_lo = min(__builtin_amdgcn_readlane(flo, 0), __builtin_amdgcn_readlane(flo, 63));
         seg_hi = max(__builtin_amdgcn_readlane(fhi, 0), __builtin_amdgcn_readlane(fhi, 63));
     }
     s.seg_lo = seg_lo;
     s.L = seg_hi - seg_lo + 1;
-    s.row = iy;
-    s.fy = 0.f;
     s.r_lo = iy;
+#pragma unroll
+    for (int j = 0; j < XPL; ++j) {
+        s.row[j] = iy;
+        s.fy[j] = 0.f;
+    }
     int NR = 1;
     if constexpr (TWO) {
-        const float yo = dx * sin_xt;
-        float sy = static_cast<float>(iy) + yo;
-        if (LOCAL) sy = sy + ldy;
-        const float fl = floorf(sy);
-        const int ifl = static_cast<int>(fl);
         const float fny = static_cast<float>(P.in_ny);
-        const bool two = LOCAL || fabsf(yo) > 1e-6f;
-        s.row = -1;
-        if (two && fl >= 0.f && fl + 1.f < fny) {
-            s.row = ifl;
-            s.fy = sy - fl;
-        } else if (fl >= 0.f && fl < fny) {
-            s.row = ifl;
+        int ilo = INT_MAX, ihi = INT_MIN;  // floor(sy) over the lane's x positions
+        int rlo = INT_MAX, rhi = -1;       // rows the lane uses
+#pragma unroll
+        for (int j = 0; j < XPL; ++j) {
+            const float yo = dx[j] * sin_xt;
+            float sy = static_cast<float>(iy) + yo;
+            if (LOCAL) sy = sy + ldy[j];
+            const float fl = floorf(sy);
+            const int ifl = static_cast<int>(fl);
+            const bool two = LOCAL || fabsf(yo) > 1e-6f;
+            s.row[j] = -1;
+            if (two && fl >= 0.f && fl + 1.f < fny) {
+                s.row[j] = ifl;
+                s.fy[j] = sy - fl;
+            } else if (fl >= 0.f && fl < fny) {
+                s.row[j] = ifl;
+            }
+            ilo = min(ilo, ifl);
+            ihi = max(ihi, ifl);
+            if (s.row[j] >= 0) {
+                rlo = min(rlo, s.row[j]);
+                rhi = max(rhi, s.row[j]);
+            }
         }
         int rmin, rmax;
         if (mode == 2) {
-            rmin = uni(wave_min_i(s.row >= 0 ? s.row : INT_MAX));
-            rmax = uni(wave_max_i(s.row));
+            rmin = uni(wave_min_i(rlo));
+            rmax = uni(wave_max_i(rhi));
         } else {
             // floor(sy) is monotonic in ox; clamp the extremes into the valid rows
-            const int f0 = __builtin_amdgcn_readlane(ifl, 0), f1 = __builtin_amdgcn_readlane(ifl, 63);
-            const int lo_r = max(min(f0, f1), 0), hi_r = min(max(f0, f1), P.in_ny - 1);
+            const int f0 = min(__builtin_amdgcn_readlane(ilo, 0), __builtin_amdgcn_readlane(ilo, 63));
+            const int f1 = max(__builtin_amdgcn_readlane(ihi, 0), __builtin_amdgcn_readlane(ihi, 63));
+            const int lo_r = max(f0, 0), hi_r = min(f1, P.in_ny - 1);
             rmin = lo_r <= hi_r ? lo_r : INT_MAX;
             rmax = lo_r <= hi_r ? hi_r : -1;
         }
@@ -259,7 +287,7 @@ __device__ __forceinline__ Vs view_setup(const BpParams& P, int p, float dx, int
 // product: with e < 2^16 the error of (e + 0.5) * rcp(L) is below 1e-3 / L,
 // far inside the 0.5 / L margin to the nearest integer.
 template <bool TWO>
-__device__ __forceinline__ void entry_pos(const Vs& s, int e, int& ri, int& x) {
+__device__ __forceinline__ void entry_pos(const Stg& s, int e, int& ri, int& x) {
     if constexpr (TWO) {
         ri = static_cast<int>(fmaf(static_cast<float>(e), s.inv_L, 0.5f * s.inv_L));
         x = s.seg_lo + (e - __mul24(ri, s.L));
@@ -307,7 +335,7 @@ __device__ __forceinline__ unsigned load_at(__amdgpu_buffer_rsrc_t r, unsigned v
 // offset is wave-uniform (scalar); per lane only a 32-bit byte offset is
 // formed.  A row r+1 past the last row (weight 0) is read as row r.
 template <bool TWO, int ST>
-__device__ __forceinline__ void load_entry(const BpParams& P, const Vs& s, int p, int e,
+__device__ __forceinline__ void load_entry(const BpParams& P, const Stg& s, int p, int e,
                                            unsigned (&r)[4], int& x) {
     constexpr unsigned esz = ST == 0 ? 4u : 2u;
     const int n = P.in_nx;
@@ -362,30 +390,30 @@ __device__ __forceinline__ void put_entry(void* lds, int idx, float4 q) {
         static_cast<float2*>(lds)[idx] = make_float2(q.x, q.y);
 }
 
-// Slot k of a thread is entry e = tid + 256*k; entries past the slots (more
-// than 512 per view) are loaded and stored directly in stage_store.
-template <bool TWO, int ST>
-__device__ __forceinline__ void stage_load(const BpParams& P, const Vs& s, int p,
-                                           unsigned (&raw)[kSlots][4], int (&xs)[kSlots]) {
+// Slot k of a thread is entry e = tid + 256*k; entries past the KS slots (more
+// than 256*KS per view) are loaded and stored directly in stage_store.
+template <bool TWO, int ST, int KS>
+__device__ __forceinline__ void stage_load(const BpParams& P, const Stg& s, int p,
+                                           unsigned (&raw)[KS][4], int (&xs)[KS]) {
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) {
+    for (int k = 0; k < KS; ++k) {
         const int e = threadIdx.x + k * kBpThreads;
         if (e < s.total) load_entry<TWO, ST>(P, s, p, e, raw[k], xs[k]);
     }
 }
 
-template <bool TWO, int ST>
-__device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int p,
-                                            const unsigned (&raw)[kSlots][4],
-                                            const int (&xs)[kSlots], void* lds, int buf) {
+template <bool TWO, int ST, int KS>
+__device__ __forceinline__ void stage_store(const BpParams& P, const Stg& s, int p,
+                                            const unsigned (&raw)[KS][4],
+                                            const int (&xs)[KS], void* lds, int buf) {
     const float scale = (ST == 0 || P.view_scale == nullptr) ? 1.f : cptr(P.view_scale)[p];
     const float cfill = P.fill_mode == 0 ? cptr(P.means)[p] : 0.f;
 #pragma unroll
-    for (int k = 0; k < kSlots; ++k) {
+    for (int k = 0; k < KS; ++k) {
         const int e = threadIdx.x + k * kBpThreads;
         if (e < s.total) put_entry<TWO, ST>(lds, buf + e, make_entry<TWO, ST>(P, xs[k], raw[k], scale, cfill));
     }
-    for (int e = threadIdx.x + kSlots * kBpThreads; e < s.total; e += kBpThreads) {
+    for (int e = threadIdx.x + KS * kBpThreads; e < s.total; e += kBpThreads) {
         unsigned one[4];
         int x;
         load_entry<TWO, ST>(P, s, p, e, one, x);
@@ -395,22 +423,28 @@ __device__ __forceinline__ void stage_store(const BpParams& P, const Vs& s, int 
 
 // The kernel: one tile of one series per workgroup (BATCH = false, the
 // workgroup id picks the tile), or a persistent loop over the tiles of several
-// series (BATCH = true, continuous batching).  A tile is Y row iy, 64 x positions
-// and 4*ZPT z positions; its result depends only on (series, tile), so the same
-// tile computed alone or inside a batched launch is bit-identical.
+// series (BATCH = true, continuous batching).  A tile is Y row iy, 64*XPL x
+// positions and 4*ZPT z positions; its result depends only on (series, tile
+// position of each voxel), so the same voxel computed alone, inside a batched
+// launch or under another launch shape is bit-identical.
+//
+// Launch shape: XPL x positions per lane (tile x 64 or 128) and VPS views
+// staged per barrier (1 or 2; with 2, the LDS holds two views per buffer and
+// each thread prefetches one slot of entries per view).
 //
 // Batched tile claiming: the tiles are split into 8 contiguous chunks, one per
 // XCD; a workgroup drains its own XCD's chunk (L2 locality: consecutive tiles
 // share projection rows), then steals from the others.  Every workgroup visits
 // every chunk once and exits when all are drained.
-// Register budget: 4 waves per SIMD (<= 128 VGPRs) up to 32 z per lane; the
-// 64-z tile keeps 3 (<= 168).
-template <int ZPT, bool TWO, bool LOCAL, int ST, bool BATCH>
+// Register budget: 4 waves per SIMD (<= 128 VGPRs) up to 32 accumulators per
+// lane; 64 keep 3 (<= 168).
+template <int ZPT, bool TWO, bool LOCAL, int ST, bool BATCH, int VPS, int XPL>
 __global__ void __launch_bounds__(kBpThreads)
-    __attribute__((amdgpu_waves_per_eu(ZPT > 32 ? 3 : 4, 8))) bp_kernel(BpParams P1, BpBatch B) {
+    __attribute__((amdgpu_waves_per_eu(ZPT * XPL > 32 ? 3 : 4, 8))) bp_kernel(BpParams P1, BpBatch B) {
     extern __shared__ float4 lds4[];
     __shared__ int s_tile;
     constexpr int TZ = 4 * ZPT;
+    constexpr int KS = VPS == 1 ? kSlots : 1;
     int chunk_i = 0, lo = 0, hi = 0;
     if constexpr (BATCH) {
         lo = (blockIdx.x % 8) * B.chunk;
@@ -457,122 +491,172 @@ __global__ void __launch_bounds__(kBpThreads)
         const int rest = logical / P.ntx;
         const int tz = rest % P.ntz;
         const int iy = P.y0 + rest / P.ntz;
-        const int ox = tx * kBpTileX + (TWO ? b128_lane_x(lane) : lane);
+        const int lx = TWO ? b128_lane_x(lane) : lane;
+        int ox[XPL];
+        float dx[XPL];
+#pragma unroll
+        for (int j = 0; j < XPL; ++j) {
+            ox[j] = (tx * XPL + j) * kBpTileX + lx;
+            dx[j] = static_cast<float>(ox[j]) - P.ocx;
+        }
         const int zt0 = tz * TZ;
         const int zw0 = zt0 + wave * ZPT;
-        const float dx = static_cast<float>(ox) - P.ocx;
 
-        float acc[ZPT];
-    #pragma unroll
-        for (int k = 0; k < ZPT; ++k) acc[k] = 0.f;
+        float acc[XPL][ZPT];
+#pragma unroll
+        for (int j = 0; j < XPL; ++j)
+#pragma unroll
+            for (int k = 0; k < ZPT; ++k) acc[j][k] = 0.f;
 
         if (P.nviews > 0) {
-            unsigned raw[kSlots][4];
-            int xs[kSlots];
-            Vs cur = view_setup<TWO, LOCAL>(P, 0, dx, ox, iy, zt0, TZ);
-            stage_load<TWO, ST>(P, cur, 0, raw, xs);
-            for (int p = 0; p < P.nviews; ++p) {
-                if (cur.total > P.lds_cap) {
-                    // host bound violated: report it (the host fails the call) and skip
-                    // the view rather than overrun LDS
-                    if (tid == 0) atomicOr(P.err, 1);
-                    cur.total = 0;
-                    cur.NR = 0;
-                    cur.L = 0;
-                    cur.row = -1;
+            unsigned raw[VPS][KS][4];
+            int xs[VPS][KS];
+            Vs<XPL> cur[VPS];
+#pragma unroll
+            for (int v = 0; v < VPS; ++v) {
+                if (v < P.nviews) {
+                    cur[v] = view_setup<TWO, LOCAL, XPL>(P, v, dx, ox, iy, zt0, TZ);
+                    stage_load<TWO, ST, KS>(P, cur[v], v, raw[v], xs[v]);
                 }
-                // buffer (p & 1) was last read by the compute of view p-2; every thread
-                // has passed the barrier of view p-1 since, so it is free.
-                const int buf = (p & 1) * P.lds_cap;
-                stage_store<TWO, ST>(P, cur, p, raw, xs, lds4, buf);
-                // the next view's loads go out before the barrier: they touch only
+            }
+            const int ngroups = (P.nviews + VPS - 1) / VPS;
+            for (int g = 0; g < ngroups; ++g) {
+                // buffer set (g & 1) was last read by the compute of group g-2; every
+                // thread has passed the barrier of group g-1 since, so it is free.
+                const int set = (g & 1) * VPS * P.lds_cap;
+#pragma unroll
+                for (int v = 0; v < VPS; ++v) {
+                    const int p = g * VPS + v;
+                    if (p >= P.nviews) break;
+                    if (cur[v].total > P.lds_cap) {
+                        // host bound violated: report it (the host fails the call) and
+                        // skip the view rather than overrun LDS
+                        if (tid == 0) atomicOr(P.err, 1);
+                        cur[v].total = 0;
+                        cur[v].NR = 0;
+                        cur[v].L = 0;
+#pragma unroll
+                        for (int j = 0; j < XPL; ++j) cur[v].row[j] = -1;
+                    }
+                    stage_store<TWO, ST, KS>(P, cur[v], p, raw[v], xs[v], lds4, set + v * P.lds_cap);
+                }
+                // the next group's loads go out before the barrier: they touch only
                 // registers and global memory, and their latency then overlaps the
                 // wait for the other waves' stores
-                Vs nxt = cur;
-                if (p + 1 < P.nviews) {
-                    nxt = view_setup<TWO, LOCAL>(P, p + 1, dx, ox, iy, zt0, TZ);
-                    stage_load<TWO, ST>(P, nxt, p + 1, raw, xs);
+                Vs<XPL> nxt[VPS];
+#pragma unroll
+                for (int v = 0; v < VPS; ++v) {
+                    nxt[v] = cur[v];
+                    const int p = (g + 1) * VPS + v;
+                    if (p < P.nviews) {
+                        nxt[v] = view_setup<TWO, LOCAL, XPL>(P, p, dx, ox, iy, zt0, TZ);
+                        stage_load<TWO, ST, KS>(P, nxt[v], p, raw[v], xs[v]);
+                    }
                 }
                 __syncthreads();
-                const TILTGPU_CONSTANT float* zc =
-                    cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
-                if constexpr (TWO) {
-                    if (cur.row >= 0) {
-                        const float4* E = lds4 + buf + (cur.row - cur.r_lo) * cur.L - cur.seg_lo;
-                        const float a = cur.a, fy = cur.fy, ldx = cur.ldx;
-                        // the fences keep the scheduler from hoisting every voxel's
-                        // 16-byte LDS read at once (registers, not issue, limit it).
-                        // With local shifts every view takes the shifted chain: a view
-                        // without one has ldx = +0, and px + 0 == px for every px but
-                        // -0 (which becomes +0: same floor, same fraction).
-    #pragma unroll
-                        for (int k = 0; k < ZPT; ++k) {
-                            float px = (a + zc[k]) + P.cx;
-                            if constexpr (LOCAL) px = px + ldx;
-                            const float4 q = E[cvt_floor(px)];
-                            const float f = __builtin_amdgcn_fractf(px);
-                            acc[k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
-                            if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int v = 0; v < VPS; ++v) {
+                    const int p = g * VPS + v;
+                    if (p >= P.nviews) break;
+                    const Vs<XPL>& c = cur[v];
+                    const int buf = set + v * P.lds_cap;
+                    const TILTGPU_CONSTANT float* zc =
+                        cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad + zw0;
+                    if constexpr (TWO) {
+#pragma unroll
+                        for (int j = 0; j < XPL; ++j) {
+                            if (c.row[j] >= 0) {
+                                const float4* E = lds4 + buf + (c.row[j] - c.r_lo) * c.L - c.seg_lo;
+                                const float a = c.a[j], fy = c.fy[j], ldx = c.ldx[j];
+                                // the fences keep the scheduler from hoisting every voxel's
+                                // 16-byte LDS read at once (registers, not issue, limit it).
+                                // With local shifts every view takes the shifted chain: a view
+                                // without one has ldx = +0, and px + 0 == px for every px but
+                                // -0 (which becomes +0: same floor, same fraction).
+#pragma unroll
+                                for (int k = 0; k < ZPT; ++k) {
+                                    float px = (a + zc[k]) + P.cx;
+                                    if constexpr (LOCAL) px = px + ldx;
+                                    const float4 q = E[cvt_floor(px)];
+                                    const float f = __builtin_amdgcn_fractf(px);
+                                    acc[j][k] += fmaf(fy, fmaf(f, q.w, q.z), fmaf(f, q.y, q.x));
+                                    if ((k & 15) == 15) __builtin_amdgcn_sched_barrier(0);
+                                }
+                            }
+                        }
+                    } else if (c.total > 0) {
+                        const float2* E = reinterpret_cast<const float2*>(lds4) + buf - c.seg_lo;
+#pragma unroll
+                        for (int j = 0; j < XPL; ++j) {
+                            const float a = c.a[j];
+#pragma unroll
+                            for (int k = 0; k < ZPT; ++k) {
+                                const float px = (a + zc[k]) + P.cx;
+                                const float2 q = E[cvt_floor(px)];
+                                const float f = __builtin_amdgcn_fractf(px);
+                                acc[j][k] += fmaf(f, q.y, q.x);
+                            }
                         }
                     }
-                } else if (cur.total > 0) {
-                    const float2* E = reinterpret_cast<const float2*>(lds4) + buf - cur.seg_lo;
-                    const float a = cur.a;
-    #pragma unroll
-                    for (int k = 0; k < ZPT; ++k) {
-                        const float px = (a + zc[k]) + P.cx;
-                        const float2 q = E[cvt_floor(px)];
-                        const float f = __builtin_amdgcn_fractf(px);
-                        acc[k] += fmaf(f, q.y, q.x);
-                    }
                 }
-                cur = nxt;
+#pragma unroll
+                for (int v = 0; v < VPS; ++v) cur[v] = nxt[v];
             }
         }
 
         // no early return: in the persistent batched kernel the workgroup goes on
         // to further tiles and its barriers
         const long slice = static_cast<long>(P.out_nz) * P.out_nx;
-        float* dst = P.out + static_cast<long>(iy - P.y0) * slice + ox;
-    #pragma unroll
-        for (int k = 0; k < ZPT; ++k) {
-            const int oz = zw0 + k;
-            if (oz < P.out_nz && ox < P.out_nx) {
-                const float v = acc[k] * P.inv_n;
-                float* o = dst + static_cast<long>(oz) * P.out_nx;
-                if (P.out_mode == 0)
-                    *o = v;
-                else if (P.out_mode == 1)
-                    *o = (v * P.dw) * P.os + P.oa;
-                else
-                    *o = *o + P.relax * v;
+#pragma unroll
+        for (int j = 0; j < XPL; ++j) {
+            float* dst = P.out + static_cast<long>(iy - P.y0) * slice + ox[j];
+#pragma unroll
+            for (int k = 0; k < ZPT; ++k) {
+                const int oz = zw0 + k;
+                if (oz < P.out_nz && ox[j] < P.out_nx) {
+                    const float v = acc[j][k] * P.inv_n;
+                    float* o = dst + static_cast<long>(oz) * P.out_nx;
+                    if (P.out_mode == 0)
+                        *o = v;
+                    else if (P.out_mode == 1)
+                        *o = (v * P.dw) * P.os + P.oa;
+                    else
+                        *o = *o + P.relax * v;
+                }
             }
         }
         if constexpr (!BATCH) break;
     }
 }
 
-size_t bp_lds_bytes(int lds_cap, int two_row) {
-    return static_cast<size_t>(2) * lds_cap * (two_row ? 16 : 8);
+size_t bp_lds_bytes(int lds_cap, int two_row, int vps) {
+    return static_cast<size_t>(2) * vps * lds_cap * (two_row ? 16 : 8);
 }
 
-template <int ZPT, bool TWO, bool LOCAL, int ST>
+bool bp_shape_supported(int tile_z, int vps, int xpl) {
+    if (vps == 1 && xpl == 1) return tile_z == 32 || tile_z == 64 || tile_z == 128 || tile_z == 256;
+    if (vps == 2 && xpl == 1) return tile_z == 32 || tile_z == 64 || tile_z == 128;
+    if (vps == 1 && xpl == 2) return tile_z == 32 || tile_z == 64;
+    return false;
+}
+
+template <int ZPT, bool TWO, bool LOCAL, int ST, int VPS, int XPL>
 static hipError_t launch_bp_t(const BpParams& p, hipStream_t st) {
     const long nblk = static_cast<long>(p.ntx) * p.ntz * (p.y1 - p.y0);
     if (nblk <= 0) return hipSuccess;
-    const size_t lds = bp_lds_bytes(p.lds_cap, TWO);
-    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, false>), dim3(static_cast<unsigned>(nblk)),
-                       dim3(kBpThreads), lds, st, p, BpBatch{});
+    const size_t lds = bp_lds_bytes(p.lds_cap, TWO, VPS);
+    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, false, VPS, XPL>),
+                       dim3(static_cast<unsigned>(nblk)), dim3(kBpThreads), lds, st, p, BpBatch{});
     return hipGetLastError();
 }
 
-template <int ZPT>
+template <int ZPT, int VPS, int XPL>
 static hipError_t launch_bp_z(const BpParams& p, int storage, int two_row, hipStream_t st) {
     const bool local = p.has_local != 0;
     if (!two_row && local) return hipErrorInvalidValue;
 #define TILTGPU_BP_CASE(TW, LC, S)                         \
     if (two_row == TW && local == LC && storage == S)      \
-        return launch_bp_t<ZPT, TW, LC, S>(p, st);
+        return launch_bp_t<ZPT, TW, LC, S, VPS, XPL>(p, st);
     TILTGPU_BP_CASE(false, false, 0)
     TILTGPU_BP_CASE(false, false, 1)
     TILTGPU_BP_CASE(false, false, 2)
@@ -588,8 +672,8 @@ static hipError_t launch_bp_z(const BpParams& p, int storage, int two_row, hipSt
 
 template <int ZPT, bool TWO, bool LOCAL, int ST>
 static hipError_t launch_bp_batch_t(const BpBatch& b, int grid, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, true>), dim3(grid), dim3(kBpThreads), lds,
-                       st, BpParams{}, b);
+    hipLaunchKernelGGL((bp_kernel<ZPT, TWO, LOCAL, ST, true, 1, 1>), dim3(grid), dim3(kBpThreads),
+                       lds, st, BpParams{}, b);
     return hipGetLastError();
 }
 
@@ -625,14 +709,25 @@ hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, 
     }
 }
 
-hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z,
-                              hipStream_t st) {
+hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z, int vps,
+                              int xpl, hipStream_t st) {
+    if (!bp_shape_supported(tile_z, vps, xpl)) return hipErrorInvalidValue;
+    if (vps == 2) {
+        switch (tile_z) {
+            case 32: return launch_bp_z<8, 2, 1>(p, storage, two_row, st);
+            case 64: return launch_bp_z<16, 2, 1>(p, storage, two_row, st);
+            default: return launch_bp_z<32, 2, 1>(p, storage, two_row, st);
+        }
+    }
+    if (xpl == 2) {
+        if (tile_z == 32) return launch_bp_z<8, 1, 2>(p, storage, two_row, st);
+        return launch_bp_z<16, 1, 2>(p, storage, two_row, st);
+    }
     switch (tile_z) {
-        case 32: return launch_bp_z<8>(p, storage, two_row, st);
-        case 64: return launch_bp_z<16>(p, storage, two_row, st);
-        case 128: return launch_bp_z<32>(p, storage, two_row, st);
-        case 256: return launch_bp_z<64>(p, storage, two_row, st);
-        default: return hipErrorInvalidValue;
+        case 32: return launch_bp_z<8, 1, 1>(p, storage, two_row, st);
+        case 64: return launch_bp_z<16, 1, 1>(p, storage, two_row, st);
+        case 128: return launch_bp_z<32, 1, 1>(p, storage, two_row, st);
+        default: return launch_bp_z<64, 1, 1>(p, storage, two_row, st);
     }
 }
 

@@ -97,7 +97,8 @@ extern "C" int tiltgpu_reconstruct_rows_noalloc(
     const int nv = n_views;
     const bool two_row = needs_two_rows(params, nv, false);
     BpTileChoice tc{};
-    if ((rc = choose_bp_tile(nullptr, params, nv, two_row, false, false, in_nx, out_nz, 0, &tc)))
+    if ((rc = choose_bp_tile(nullptr, params, nv, two_row, false, false, in_nx, out_nz, 0, &tc, 1, 1,
+                             false)))
         return rc;
     const int nz_pad = (out_nz + tc.tile_z - 1) / tc.tile_z * tc.tile_z;
     const size_t off_params = 256;
@@ -160,7 +161,7 @@ extern "C" int tiltgpu_reconstruct_rows_noalloc(
     p.has_local = 0;
     p.lds_cap = tc.lds_cap;
     p.err = err;
-    HIP_TRY(launch_backproject(p, TILTGPU_STORE_F32, two_row ? 1 : 0, tc.tile_z, st));
+    HIP_TRY(launch_backproject(p, TILTGPU_STORE_F32, two_row ? 1 : 0, tc.tile_z, 1, 1, st));
     int e = 0;
     HIP_TRY(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));

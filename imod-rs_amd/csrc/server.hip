@@ -329,6 +329,7 @@ int setup(tiltgpu_server* S, Req* r) {
         (rc = tiltgpu_set_local_alignment(s, r->lnx.data(), r->lny.data(), r->lxs.data(),
                                           r->lys.data(), r->ldx.data(), r->ldy.data())))
         return rc;
+    s->fixed_shape = true;  // the batched backprojection runs launch shape (1, 1)
     if ((rc = prepare_pipeline_geometry(s))) return rc;
     if (r->kind == kSeries) dev_free(s, s->vol);  // slab requests bring their own outputs
     HIP_TRY(hipMemcpyAsync(s->raw, r->proj, r->proj_n * 4, hipMemcpyHostToDevice, S->up_stream));

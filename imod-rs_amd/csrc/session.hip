@@ -75,12 +75,12 @@ int pick_pairs_per_wg(int n) { return std::max(1, 2048 / std::max(1, n)); }
 
 // Upper bound on the LDS entries one backprojection tile needs for a view whose
 // local shifts span ldx_span x ldy_span pixels.
-void tile_bounds(const tiltgpu_view_params& g, int tile_z, double ldx_span, double ldy_span,
-                 int* seg, int* rows) {
-    const double sx = std::fabs(static_cast<double>(g.cos_t) * g.cos_xt) * (kBpTileX - 1) +
+void tile_bounds(const tiltgpu_view_params& g, int tile_z, int tile_x, double ldx_span,
+                 double ldy_span, int* seg, int* rows) {
+    const double sx = std::fabs(static_cast<double>(g.cos_t) * g.cos_xt) * (tile_x - 1) +
                       std::fabs(static_cast<double>(g.sin_t) * g.zfactor) * (tile_z - 1) +
                       ldx_span + 4.0;
-    const double sy = std::fabs(static_cast<double>(g.sin_xt)) * (kBpTileX - 1) + ldy_span + 3.0;
+    const double sy = std::fabs(static_cast<double>(g.sin_xt)) * (tile_x - 1) + ldy_span + 3.0;
     *seg = static_cast<int>(std::min(sx, 1e9));
     *rows = static_cast<int>(std::min(sy, 1e9));
 }
@@ -114,55 +114,70 @@ bool needs_two_rows(const tiltgpu_view_params* vp, int nv, bool has_local) {
 
 int choose_bp_tile(const tiltgpu_session* local_src, const tiltgpu_view_params* vp, int nv,
                    bool two_row, bool has_local, bool half, int in_nx, int out_nz, int forced,
-                   BpTileChoice* out) {
-    // tile depth: minimise padded depth x (1 + per-view overhead / voxels per lane);
-    // LDS capacity from the per-view bounds
+                   BpTileChoice* out, int forced_vps, int forced_xpl, bool allow_shapes) {
+    // Candidates in order: the measured shape (tile depth, views per stage, x per
+    // lane), then the depths by padded depth x (1 + per-view overhead / voxels per
+    // lane) at shape (1, 1); the first whose per-view LDS bound fits is taken.
     const int max_lds_bytes = 64 * 1024;
-    int order[4] = {256, 128, 64, 32};
-    int norder = 4;
+    struct Cand {
+        int tz, vps, xpl;
+    };
+    Cand order[5];
+    int norder = 0;
+    int depths[4] = {256, 128, 64, 32};
     auto cost = [&](int tz) {
         const double pad = static_cast<double>((out_nz + tz - 1) / tz * tz);
         return pad * (1.0 + 32.0 / tz);
     };
     for (int i = 1; i < 4; ++i)  // stable insertion sort by cost (no temporary buffer)
-        for (int j = i; j > 0 && cost(order[j]) < cost(order[j - 1]); --j)
-            std::swap(order[j], order[j - 1]);
+        for (int j = i; j > 0 && cost(depths[j]) < cost(depths[j - 1]); --j)
+            std::swap(depths[j], depths[j - 1]);
     out->source = kTuneHeuristic;
-    const int tuned = tuned_tile_z(two_row ? 1 : 0, has_local ? 1 : 0, half ? 1 : 0, in_nx, out_nz);
-    if (tuned > 0) {
-        // the measured depth first; the others remain as LDS fallbacks
-        int k = 0;
-        int rest[4];
-        for (int i = 0; i < 4; ++i)
-            if (order[i] != tuned) rest[k++] = order[i];
-        order[0] = tuned;
-        for (int i = 0; i < k; ++i) order[1 + i] = rest[i];
-        norder = 1 + k;
-        out->source = kTuneTable;
-    }
+    if (!allow_shapes) forced_vps = forced_xpl = 1;
+    int t_vps = 1, t_xpl = 1;
+    const int tuned = allow_shapes
+                          ? tuned_bp_shape(two_row ? 1 : 0, has_local ? 1 : 0, half ? 1 : 0, in_nx,
+                                           out_nz, &t_vps, &t_xpl)
+                          : tuned_tile_z(two_row ? 1 : 0, has_local ? 1 : 0, half ? 1 : 0, in_nx,
+                                         out_nz);
+    const int fv = forced_vps > 0 ? forced_vps : 0, fx = forced_xpl > 0 ? forced_xpl : 0;
     if (forced > 0) {
-        order[0] = forced;
-        norder = 1;
+        order[norder++] = {forced, fv ? fv : 1, fx ? fx : 1};
         out->source = kTuneForced;
+    } else {
+        if (tuned > 0) {
+            order[norder++] = {tuned, fv ? fv : t_vps, fx ? fx : t_xpl};
+            out->source = kTuneTable;
+        }
+        for (int i = 0; i < 4; ++i) order[norder++] = {depths[i], fv ? fv : 1, fx ? fx : 1};
+        if (fv || fx) out->source = kTuneForced;
     }
     for (int i = 0; i < norder; ++i) {
-        const int tz = order[i];
+        const Cand c = order[i];
+        if (!bp_shape_supported(c.tz, c.vps, c.xpl)) continue;
         int cap = 1;
         for (int v = 0; v < nv; ++v) {
             int seg, rows;
             double lx, ly;
             local_spans(has_local ? local_src : nullptr, v, &lx, &ly);
-            tile_bounds(vp[v], tz, lx, ly, &seg, &rows);
+            tile_bounds(vp[v], c.tz, kBpTileX * c.xpl, lx, ly, &seg, &rows);
             const int need = two_row ? seg * rows : seg;
             cap = std::max(cap, need);
         }
         cap = (cap + 63) / 64 * 64;
-        if (bp_lds_bytes(cap, two_row) <= static_cast<size_t>(max_lds_bytes)) {
-            out->tile_z = tz;
+        if (bp_lds_bytes(cap, two_row, c.vps) <= static_cast<size_t>(max_lds_bytes)) {
+            out->tile_z = c.tz;
             out->lds_cap = cap;
+            out->vps = c.vps;
+            out->xpl = c.xpl;
             return TILTGPU_OK;
         }
     }
+    if (forced > 0 || fv || fx)
+        return fail(TILTGPU_ERR_ARG,
+                    "launch shape tile depth %d, %d views per stage, %d x per lane is not "
+                    "built or does not fit the LDS for this geometry",
+                    forced > 0 ? forced : 0, fv ? fv : 1, fx ? fx : 1);
     return fail(TILTGPU_ERR_ARG,
                 "geometry needs more LDS per backprojection tile than supported "
                 "(local-alignment shifts vary too much within 64 pixels)");
@@ -176,9 +191,11 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
     BpTileChoice tc{};
     int rc0 = choose_bp_tile(s, vp.data(), nv, s->two_row, s->has_local,
                              s->o.storage != TILTGPU_STORE_F32, s->d.in_nx, s->d.out_nz, forced,
-                             &tc);
+                             &tc, s->tune_vps, s->tune_xpl, !s->fixed_shape);
     if (rc0) return rc0;
     s->tile_z = tc.tile_z;
+    s->bp_vps = tc.vps;
+    s->bp_xpl = tc.xpl;
     s->lds_cap = tc.lds_cap;
     s->tile_source = tc.source;
     const int out_nz = s->d.out_nz;
@@ -409,7 +426,8 @@ BpParams make_bp(tiltgpu_session* s, const void* proj, const float* scale, int y
     p.nz_pad = s->nz_pad;
     p.y0 = y0;
     p.y1 = y1;
-    p.ntx = (s->d.out_nx + kBpTileX - 1) / kBpTileX;
+    const int tile_x = kBpTileX * s->bp_xpl;
+    p.ntx = (s->d.out_nx + tile_x - 1) / tile_x;
     p.ntz = s->nz_pad / s->tile_z;
     p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
     p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
@@ -440,7 +458,7 @@ int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int
     BpParams p = make_bp(s, proj, scale, y0, y1, out_mode, out);
     Scope sc(s, kClsBp);
     HIP_TRY(launch_backproject(p, s->seam ? TILTGPU_STORE_F32 : s->o.storage, s->two_row ? 1 : 0,
-                               s->tile_z, s->stream));
+                               s->tile_z, s->bp_vps, s->bp_xpl, s->stream));
     s->n_bp++;
     return TILTGPU_OK;
 }
@@ -1142,6 +1160,27 @@ int tiltgpu_get_tuning(tiltgpu_session* s, int32_t* out4) {
     out4[1] = s->tile_source;
     out4[2] = bins;
     out4[3] = fp_src;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_set_bp_shape(tiltgpu_session* s, int32_t views_per_stage, int32_t x_per_lane) {
+    if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
+    if (views_per_stage < 0 || views_per_stage > 2 || x_per_lane < 0 || x_per_lane > 2)
+        return fail(TILTGPU_ERR_ARG, "views per stage %d / x per lane %d not in 0..2",
+                    views_per_stage, x_per_lane);
+    s->tune_vps = views_per_stage;
+    s->tune_xpl = x_per_lane;
+    s->geometry_ready = false;
+    return TILTGPU_OK;
+}
+
+int tiltgpu_get_bp_shape(tiltgpu_session* s, int32_t* out3) {
+    if (!s || s->seam || !out3) return fail(TILTGPU_ERR_ARG, "bad arguments");
+    int rc;
+    if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
+    out3[0] = s->tile_z;
+    out3[1] = s->bp_vps;
+    out3[2] = s->bp_xpl;
     return TILTGPU_OK;
 }
 

@@ -121,6 +121,10 @@ struct tiltgpu_session {
     int tile_z = 64;
     int tile_z_override = 0;  // TILTGPU_TILE_Z (tuning): force the backprojection tile depth
     int tune_tile_z = 0;      // tiltgpu_set_tuning: forced tile depth (0: table / heuristic)
+    int tune_vps = 0;         // tiltgpu_set_bp_shape: forced views per stage (0: table)
+    int tune_xpl = 0;         // tiltgpu_set_bp_shape: forced x per lane (0: table)
+    int bp_vps = 1, bp_xpl = 1;  // backprojection launch shape in use
+    bool fixed_shape = false;    // shape (1, 1) only (the server's batched kernel)
     int tune_fp_bins = 0;     // tiltgpu_set_tuning: forced FP bins per lane (0: table / rule)
     int tile_source = 0;      // kTuneHeuristic / kTuneTable / kTuneForced
     int lds_cap = 0;
@@ -237,10 +241,13 @@ int setup_geometry(tiltgpu_session* s, const std::vector<tiltgpu_view_params>& v
 // has_local.
 struct BpTileChoice {
     int tile_z, lds_cap, source;
+    int vps = 1, xpl = 1;  // views per LDS stage, x positions per lane
 };
+// forced_vps / forced_xpl: 0 = measured table; allow_shapes false keeps (1, 1).
 int choose_bp_tile(const tiltgpu_session* local_src, const tiltgpu_view_params* vp, int nv,
                    bool two_row, bool has_local, bool half, int in_nx, int out_nz, int forced,
-                   BpTileChoice* out);
+                   BpTileChoice* out, int forced_vps = 0, int forced_xpl = 0,
+                   bool allow_shapes = true);
 // Whether a view table needs the two-row backprojection (x-axis tilt or local shifts).
 bool needs_two_rows(const tiltgpu_view_params* vp, int nv, bool has_local);
 int upload_local(tiltgpu_session* s);

@@ -56,7 +56,7 @@ struct BpParams {
     float* out;
     int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
     int y0, y1;
-    int ntx, ntz;             // tiles along x (64 wide) and z (tile_z deep)
+    int ntx, ntz;             // tiles along x (64 * xpl wide) and z (tile_z deep)
     float cx, ocx, inv_n;
     int fill_mode;
     int out_mode;             // 0: acc/nz, 1: output transform, 2: out += relax*acc/nz
@@ -187,9 +187,13 @@ struct FpBatch {
 };
 
 // ---- launchers (kernels_*.hip); all asynchronous on `st` ----
-hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z,
-                              hipStream_t st);
-size_t bp_lds_bytes(int lds_cap, int two_row);
+// Launch shape: tile depth tile_z, vps views staged per barrier (1, 2) and xpl
+// x positions per lane (tile x 64 * xpl); bp_shape_supported lists the built
+// combinations.  The batched kernel runs shape (1, 1).
+hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z, int vps,
+                              int xpl, hipStream_t st);
+bool bp_shape_supported(int tile_z, int vps, int xpl);
+size_t bp_lds_bytes(int lds_cap, int two_row, int vps = 1);
 hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, int local,
                                     int tile_z, int grid, size_t lds, hipStream_t st);
 

@@ -146,6 +146,8 @@ SIGNATURES = [
     ("tiltgpu_reproject", C.c_int, [_vp, _fp, _fp]),
     ("tiltgpu_set_tuning", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_get_tuning", C.c_int, [_vp, _ip]),
+    ("tiltgpu_set_bp_shape", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("tiltgpu_get_bp_shape", C.c_int, [_vp, _ip]),
     ("tiltgpu_set_sirt_trace", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("tiltgpu_read_sirt_trace", C.c_int, [_vp, C.c_int32, _fp]),
     ("tiltgpu_read_filtered", C.c_int, [_vp, _fp]),
@@ -433,6 +435,16 @@ class Session:
     def set_tuning(self, bp_tile_z=0, fp_bins_per_lane=0):
         """Force launch shapes (0 = tuned table / analytic rule)."""
         _check(lib().tiltgpu_set_tuning(self.h, bp_tile_z, fp_bins_per_lane))
+
+    def set_bp_shape(self, views_per_stage=0, x_per_lane=0):
+        """Force the backprojection launch shape beyond its depth (0 = tuned table)."""
+        _check(lib().tiltgpu_set_bp_shape(self.h, views_per_stage, x_per_lane))
+
+    def bp_shape(self):
+        """{'tile_z', 'views_per_stage', 'x_per_lane'} of the next run."""
+        out = np.zeros(3, np.int32)
+        _check(lib().tiltgpu_get_bp_shape(self.h, out.ctypes.data_as(_ip)))
+        return dict(zip(("tile_z", "views_per_stage", "x_per_lane"), map(int, out)))
 
     def tuning(self):
         """{'tile_z', 'tile_source', 'fp_bins', 'fp_source'}; source 0 rule, 1 table, 2 forced."""

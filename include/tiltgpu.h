@@ -208,6 +208,17 @@ int tiltgpu_set_tuning(tiltgpu_session* s, int32_t bp_tile_z, int32_t fp_bins_pe
 /* out4 = {tile depth, its source, FP bins per lane, its source}; sources:
  * 0 analytic rule, 1 tuned table, 2 forced by tiltgpu_set_tuning. */
 int tiltgpu_get_tuning(tiltgpu_session* s, int32_t* out4);
+/* Diagnostic and tuning (no counterpart in the reference; for
+ * imod-rs_amd/autotune.py and callers pinning a shape): the backprojection's
+ * launch shape beyond its tile depth.  views_per_stage: views staged in LDS per
+ * barrier (1 or 2); x_per_lane: output x positions per lane (1 or 2, tile x 64
+ * or 128); 0 = the tuned table's choice.  Built shapes: (1, 1) at every depth,
+ * (2, 1) at depths 32-128, (1, 2) at depths 32 and 64; a forced shape that is
+ * not built or does not fit the geometry's LDS need fails the next run with
+ * TILTGPU_ERR_ARG.  Results do not depend on the shape. */
+int tiltgpu_set_bp_shape(tiltgpu_session* s, int32_t views_per_stage, int32_t x_per_lane);
+/* out3 = {tile depth, views per stage, x per lane} the session runs with. */
+int tiltgpu_get_bp_shape(tiltgpu_session* s, int32_t* out3);
 
 /* Keep output rows [y0, y1) of the volume after the initial WBP and after every
  * SIRT iteration of the next tiltgpu_run (before the output transform), for

@@ -296,30 +296,58 @@ def test_run_rows_equals_full_run(gpu):
             s.run_rows(0, 4)
 
 
-@pytest.mark.parametrize("xtilt", [None, 0.5])
-def test_launch_shapes_do_not_change_results(gpu, xtilt):
-    """Every tile depth and FP bins-per-lane choice gives the same bits (WBP and
-    SIRT); the default comes from the tuned table."""
-    xt = None if xtilt is None else np.full(21, xtilt, np.float32)
+@pytest.mark.parametrize("geometry", ["plain", "xtilt", "local"])
+def test_launch_shapes_do_not_change_results(gpu, geometry):
+    """Every backprojection launch shape (tile depth, views staged per barrier,
+    x positions per lane) and FP bins-per-lane choice gives the same bits (WBP
+    and SIRT); the default comes from the tuned table."""
+    xt = None if geometry == "plain" else np.full(21, 0.5, np.float32)
     P, ang = case(gpu, nx=160, ny=10, nviews=21, thick=72, seed=41, xtilt=xt)
+    grids = None
+    if geometry == "local":
+        rng = np.random.default_rng(43)
+        grids = O.build_local_grids([(v, x, y, float(rng.normal(0, 1.5)), float(rng.normal(0, 0.7)))
+                                     for v in range(21) for x in (20.0, 140.0) for y in (1.0, 8.0)],
+                                    21)
     outs = []
-    for tz, bins in [(0, 0), (32, 4), (64, 8), (128, 16), (256, 8)]:
+    for tz, bins, vps, xpl in [(0, 0, 0, 0), (32, 4, 1, 1), (64, 8, 1, 1), (128, 16, 1, 1),
+                               (256, 8, 1, 1), (32, 8, 2, 1), (64, 4, 2, 1), (128, 8, 2, 1),
+                               (32, 16, 1, 2), (64, 8, 1, 2)]:
         with gpu.Session(160, 10, 21, gpu.TiltOptions(thickness=72, sirt=2)) as s:
             s.set_angles(ang)
             if xt is not None:
                 s.set_xtilt(xt)
+            if grids is not None:
+                s.set_local(*grids.flat())
             s.set_tuning(tz, bins)
+            s.set_bp_shape(vps, xpl)
             s.upload(P)
             s.run()
             outs.append(s.read_rows()[0])
             t = s.tuning()
             if tz:
                 assert t["tile_z"] == tz and t["tile_source"] == 2 and t["fp_bins"] == bins
+                assert s.bp_shape() == {"tile_z": tz, "views_per_stage": vps, "x_per_lane": xpl}
             else:
                 assert t["tile_source"] == 1 and t["fp_source"] == 1  # the tuned table
     for o in outs[1:]:
         # the two-row reprojection sums in fixed point: order-independent too
         assert np.array_equal(o, outs[0])
+
+
+def test_unbuilt_launch_shape_is_refused(gpu):
+    P, ang = case(gpu, nx=96, ny=4, nviews=9, thick=40, seed=44)
+    with gpu.Session(96, 4, 9, gpu.TiltOptions(thickness=40)) as s:
+        s.set_angles(ang)
+        s.set_tuning(256, 0)
+        s.set_bp_shape(2, 1)  # two views per stage is not built at depth 256
+        s.upload(P)
+        with pytest.raises(gpu.TiltGpuError) as e:
+            s.run()
+        assert e.value.code == gpu.ERR_ARG
+    with pytest.raises(gpu.TiltGpuError):
+        with gpu.Session(96, 4, 9, gpu.TiltOptions(thickness=40)) as s:
+            s.set_bp_shape(3, 1)
 
 
 def test_sirt_xtilt_scatter(gpu, rel):
