@@ -75,19 +75,26 @@ def bp_time(P, th, two_row, local, storage, tz, reps=3, vps=1, xpl=1):
         s.close()
 
 
-def fp_time(w, bins, th=128, nviews=61, reps=3):
+def fp_time(w, bins, th=256, nviews=61, reps=2):
+    """Median ms per forward-projection launch inside SIRT-4 runs of a w x w
+    series of seeded normal projections, thickness th: the launches then run as
+    they do in a reconstruction, between backprojections and filters, on a
+    volume of realistic values (a constant volume draws less power and ran some
+    shapes faster than real volumes do)."""
     ang = np.linspace(-60.0, 60.0, nviews).astype(np.float32)
-    V = np.ones((w, th, w), np.float32)
-    s = tg.Session(w, w, nviews, tg.TiltOptions(thickness=th, no_cosine_stretch=True))
+    P = np.random.default_rng(w).standard_normal((nviews, w, w)).astype(np.float32)
+    s = tg.Session(w, w, nviews, tg.TiltOptions(thickness=th, sirt=4))
     try:
         s.set_angles(ang)
         s.set_tuning(0, bins)
-        s.reproject(V)
+        s.upload(P)
+        s.run()
         ts = []
         for _ in range(reps):
             s.reset_timing()
-            s.reproject(V)
-            ts.append(s.timing()["fp_ms"])
+            s.run()
+            t = s.timing()
+            ts.append(t["fp_ms"] / max(1, t["fp_launches"]))
         return float(np.median(ts))
     finally:
         s.close()
@@ -113,8 +120,8 @@ def emit(meas, out):
           "Written by `python imod-rs_amd/autotune.py --out DIR`.  Backprojection: median",
           "ms per launch of a w x w series at 61 views (w output rows) for every launch",
           "shape -- tile depth tz, views staged per barrier vps, x positions per lane xpl",
-          "(tile x 64 * xpl); forward projection: median ms of one reprojection of a",
-          "w x w x 128 volume at 61 views.", "",
+          "(tile x 64 * xpl); forward projection: median ms per launch, reprojecting a",
+          "w x w x 256 volume inside SIRT-4 runs of seeded normal projections at 61 views.", "",
           "## Backprojection tile depth (1 view per stage, 1 x per lane)", "",
           "| variant | storage | width | thickness | tz=32 | tz=64 | tz=128 | tz=256 | fastest |",
           "|---|---|---|---|---|---|---|---|---|"]
@@ -188,16 +195,22 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--quick", action="store_true", help="two widths and thicknesses only")
     ap.add_argument("--from-json", help="re-emit the table from a saved autotune.json")
+    ap.add_argument("--fp-only", action="store_true",
+                    help="with --from-json: keep its backprojection measurements, measure FP again")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
-    if a.from_json:
+    if a.from_json and not a.fp_only:
         with open(a.from_json) as fh:
             emit(json.load(fh), a.out)
         return
     widths, thick = (WIDTHS[1::2], THICK[1::2]) if a.quick else (WIDTHS, THICK)
     t0 = time.time()
     meas = {"bp": [], "fp": []}
-    for name, two, loc in VARIANTS:
+    if a.fp_only:
+        with open(a.from_json) as fh:
+            meas = json.load(fh)
+        meas["fp"] = []
+    for name, two, loc in ([] if a.fp_only else VARIANTS):
         for storage in ("f32", "f16"):
             for w in widths:
                 # the backprojection's time does not depend on the values
