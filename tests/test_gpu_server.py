@@ -153,3 +153,40 @@ def test_server_requests_with_geometry(gpu, rel):
     o_s, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=48, sirt=2), xtilt=xt, zfactor=zf,
                         local=grids)
     assert rel(got[4], o_s) <= 1e-4
+
+
+def test_server_batched_sizes_against_oracle(gpu, rel):
+    """Requests of the batched configuration's own sizes through one server
+    launch set: a 768-wide SIRT-10 (97 views, 211 thick), a 1024-wide WBP (121
+    views, 300 thick), a 512-wide SIRT-10 with FP16 storage and a 256-wide WBP.
+    FP32 results match the FP32 oracle on row bands at the 1e-4 bar (SIRT: rows
+    evolve independently in the single-row geometry, so the oracle's band is the
+    whole run's); every result equals a plain session bit for bit."""
+    def make(w, nv, th, seed):
+        ang = np.linspace(-60, 60, nv).astype(np.float32)
+        P = gpu.synth_series(w, w, ang, vol_nz=th, n_ellipsoids=int(min(400, 200 * (w / 512) ** 2)),
+                             n_beads=30, seed=seed, noise_rel=0.1)
+        return P, ang
+    reqs = [(make(768, 97, 211, 31), gpu.TiltOptions(thickness=211, sirt=10)),
+            (make(1024, 121, 300, 32), gpu.TiltOptions(thickness=300)),
+            (make(512, 61, 150, 33), gpu.TiltOptions(thickness=150, sirt=10, storage="f16")),
+            (make(256, 41, 64, 34), gpu.TiltOptions(thickness=64))]
+    print("case: served 768 SIRT-10, 1024 WBP, 512 SIRT-10 f16, 256 WBP", flush=True)
+    with gpu.Server(32 << 30) as srv:
+        ids = [srv.submit(P, ang, o) for (P, ang), o in reqs]
+        got = [srv.wait(i)[0] for i in ids]
+        st = srv.stats()
+    assert st["completed"] == 4 and st["failed"] == 0
+    for ((P, ang), o), g in zip(reqs, got):
+        assert np.array_equal(g, gpu.reconstruct(P, ang, o)[0])
+    (P, ang), _ = reqs[0]
+    tr, _ = O.sirt_trace(P, ang, O.TiltOptions(thickness=211, sirt=10), (380, 388), prec="f32",
+                         nthreads=16)
+    assert rel(got[0][380:388], tr[10]) <= 1e-4
+    (P, ang), _ = reqs[1]
+    ref, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=300), prec="f32", nthreads=16,
+                        rows=(508, 516))
+    assert rel(got[1][508:516], ref) <= 1e-4
+    (P, ang), _ = reqs[3]
+    ref, _ = O.tilt_run(P, ang, O.TiltOptions(thickness=64), prec="f32", nthreads=16)
+    assert rel(got[3], ref) <= 1e-4
