@@ -691,3 +691,20 @@ def test_degenerate_shapes(gpu, rel, nv, ny, nx, thick, kw):
     g, sg, o, so = both(gpu, P, ang, gpu.TiltOptions(thickness=thick, **kw), xtilt=xt)
     assert g.shape == o.shape
     assert rel(g, o) <= TOL
+
+
+def test_section_transforms_odd_sizes_bitwise(gpu):
+    """newstack -x on sections whose size is not a multiple of the kernel's
+    64 x 32 pixel blocks (333 x 77, and 65 x 33), bit for bit the oracle's."""
+    rng = np.random.default_rng(26)
+    for nx, ny, nz in ((333, 77, 4), (65, 33, 3)):
+        P = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+        xf = _small_xf(rng, nz)
+        print(f"case: section transforms {nx}x{ny}x{nz}", flush=True)
+        with gpu.Session(nx, ny, nz, gpu.TiltOptions(thickness=16)) as s:
+            s.set_angles(np.linspace(-30, 30, nz).astype(np.float32))
+            s.set_section_transforms(xf, fill=-1.25)
+            s.upload(P)
+            got = s.read_projections()
+        exp = O.apply_xf_stack(P, xf, fill=-1.25)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
