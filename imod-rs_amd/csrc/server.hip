@@ -59,6 +59,10 @@ constexpr int64_t kAdmitBytes = 2ll << 30;   // projection bytes uploaded per ro
 // 4 Gvoxel-view/ms, 5e8 voxel-views is where the two meet
 // (TILTGPU_SERVER_MERGE_WORK overrides it).
 constexpr double kMergeWork = 5e8;
+// A round's SIRT iterations carry at most this much work (about 20 ms of the
+// device at 2000 Gvoxel-view/s), so that a round, which every newly arrived
+// request waits for, stays short when large SIRT requests are in flight.
+constexpr double kRoundWork = 4e10;
 
 struct Req {
     int64_t id = 0;
@@ -78,6 +82,7 @@ struct Req {
     float* slab = nullptr;  // device slab output
     size_t slab_bytes = 0;
     int phase = 0;          // 0: first round; k: SIRT iteration k
+    int64_t last_round = -1;  // the last round that ran one of its SIRT iterations
     int state = kQueued;
     int rc = 0;
     std::string err;
@@ -250,6 +255,11 @@ struct tiltgpu_server {
     // depth).  TILTGPU_SERVER_MERGE_TILES=0 keeps one launch per depth.
     bool merge_tiles = true;
     double merge_work = kMergeWork;
+    // SIRT iterations per round: at most this many voxel-views of forward and
+    // back projection (at least one iteration); the iterations of other requests
+    // wait for later rounds, the longest-waiting first.  0: every request in
+    // flight iterates every round.  TILTGPU_SERVER_ROUND_WORK overrides.
+    double round_work = kRoundWork;
     int n_cu = 256;
     int64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;     // rounds
@@ -441,6 +451,21 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
         else
             iter.push_back(r);
     }
+    if (S->round_work > 0 && iter.size() > 1) {
+        std::stable_sort(iter.begin(), iter.end(),
+                         [](const Req* a, const Req* b) { return a->last_round < b->last_round; });
+        std::vector<Req*> run;
+        double work = 0;
+        for (Req* r : iter) {
+            const tiltgpu_session* s = r->s;
+            const double w = 2.0 * s->d.in_ny * s->d.out_nz * s->d.out_nx * s->nviews;
+            if (!run.empty() && work + w > S->round_work) continue;  // waits for a later round
+            run.push_back(r);
+            work += w;
+        }
+        iter.swap(run);
+    }
+    for (Req* r : iter) r->last_round = S->rounds;
     int rc;
     // 1. log + means of new series
     for (Req* r : prep)
@@ -926,6 +951,7 @@ int tiltgpu_server_create(int64_t budget_bytes, tiltgpu_server** out) {
     }
     if (const char* e = std::getenv("TILTGPU_SERVER_MERGE_TILES")) S->merge_tiles = std::atoi(e) != 0;
     if (const char* e = std::getenv("TILTGPU_SERVER_MERGE_WORK")) S->merge_work = std::atof(e);
+    if (const char* e = std::getenv("TILTGPU_SERVER_ROUND_WORK")) S->round_work = std::atof(e);
     bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&S->up_stream, hipStreamNonBlocking) == hipSuccess;
     for (hipStream_t& st : S->down_stream)
