@@ -1,0 +1,51 @@
+"""The autotuner's table writer (imod-rs_amd/autotune.py emit) on synthetic
+measurements: per-thickness costs relative to shape (128, 1, 1), every measured
+shape present, unmeasured ones absent, and the FP rows.  CPU only."""
+import os
+import re
+import sys
+
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "imod-rs_amd"))
+import autotune as at  # noqa: E402
+
+
+def _meas():
+    bp = []
+    for th, ms128 in ((64, 2.0), (512, 10.0)):
+        ms = {"32": 3.0 * th / 64, "64": 2.5 * th / 64, "128": ms128, "256": None}
+        shapes = {at.shape_key(*c): None for c in at.SHAPES}
+        shapes[at.shape_key(64, 1, 2)] = 1.5 * th / 64
+        bp.append({"variant": "single-row", "two_row": 0, "local": 0, "half": 0, "width": 1024,
+                   "thickness": th, "ms": ms, "shapes": shapes})
+    fp = [{"width": 1024, "ms": {"4": 3.0, "8": 2.0, "16": 2.5}}]
+    return {"bp": bp, "fp": fp}
+
+
+def test_emit_writes_per_thickness_costs(tmp_path):
+    at.emit(_meas(), str(tmp_path))
+    inc = (tmp_path / "tile_table.inc").read_text()
+    rows = [tuple(int(v) for v in m.groups()) for m in re.finditer(
+        r"\{(-?\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)\}", inc)]
+    # two thicknesses x (3 measured depths + 1 measured shape); depth 256 unmeasured
+    assert len(rows) == 8
+    for r in rows:
+        assert r[:4] == (0, 0, 0, 1024) and r[4] in (64, 512)
+    by = {(r[4], r[5], r[6], r[7]): r[8] for r in rows}
+    assert by[(64, 128, 1, 1)] == 1000 and by[(512, 128, 1, 1)] == 1000
+    # thickness 64: t / pad relative to 2.0 / 128; depth 64 pads 64 -> 2.5/64 vs 2/128
+    assert by[(64, 64, 1, 1)] == round(1000 * (2.5 / 64) / (2.0 / 128))
+    assert by[(64, 64, 1, 2)] == round(1000 * (1.5 / 64) / (2.0 / 128))
+    assert (64, 256, 1, 1) not in by and (64, 128, 2, 1) not in by
+    assert "static const FpTune kFpTable[] = {\n    {1024, 8},\n};" in inc
+    md = (tmp_path / "autotune.md").read_text()
+    # the choice at the measured thickness is the fastest measured launch
+    assert "| single-row | f32 | 1024 | 64 | (64, 1, 2) | 1.500 |" in md
+
+
+def test_emit_without_measurements_writes_a_sentinel(tmp_path):
+    at.emit({"bp": [], "fp": []}, str(tmp_path))
+    inc = (tmp_path / "tile_table.inc").read_text()
+    assert "{-1, 0, 0, 0, 0, 0, 0, 0, 0}" in inc
