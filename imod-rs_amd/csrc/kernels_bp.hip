@@ -523,6 +523,11 @@ __global__ void __launch_bounds__(kBpThreads)
             for (int g = 0; g < ngroups; ++g) {
                 // buffer set (g & 1) was last read by the compute of group g-2; every
                 // thread has passed the barrier of group g-1 since, so it is free.
+                // Staging (the stores of this group, the next group's setup and loads)
+                // runs at raised wave priority, so the four waves reach the barrier
+                // sooner while other workgroups' waves accumulate; the accumulate
+                // runs at normal priority.  C1 launch: -1.2 to -1.7%.
+                __builtin_amdgcn_s_setprio(2);
                 const int set = (g & 1) * VPS * P.lds_cap;
 #pragma unroll
                 for (int v = 0; v < VPS; ++v) {
@@ -554,6 +559,7 @@ __global__ void __launch_bounds__(kBpThreads)
                     }
                 }
                 __syncthreads();
+                __builtin_amdgcn_s_setprio(0);
 #pragma unroll
                 for (int v = 0; v < VPS; ++v) {
                     const int p = g * VPS + v;

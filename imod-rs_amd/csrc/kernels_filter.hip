@@ -41,6 +41,22 @@ __device__ __forceinline__ double wg_sum(double v) {
     return t;
 }
 
+__shared__ double2 s_wave_sum2[kFilterThreads / 64];
+
+// Two such sums at once (one pair of barriers).
+__device__ __forceinline__ double2 wg_sum2(double a, double b) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) s_wave_sum2[threadIdx.x >> 6] = make_double2(a, b);
+    __syncthreads();
+    const double2 w0 = s_wave_sum2[0], w1 = s_wave_sum2[1], w2 = s_wave_sum2[2], w3 = s_wave_sum2[3];
+    __syncthreads();
+    return make_double2((w0.x + w1.x) + (w2.x + w3.x), (w0.y + w1.y) + (w2.y + w3.y));
+}
+
 // sample_row_cubic / cubic_interp_1d (main.rs:408-441) over an LDS row held in
 // component `c` of float2 entries.
 __device__ __forceinline__ float lds_at(const float2* row, int i, int c) {
@@ -148,9 +164,16 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
     float2 twr[P2<N>::tw_total() > 0 ? P2<N>::tw_total() : 1];
     if (P.weights) p2_load_tw<N, 0>(twr, j, P.tw8);
 
-    // ---- load two rows per pair (row a -> re, row b -> im), zero past the end
-    double sq = 0.0;
-    for (int idx = tid; idx < PPW * N; idx += kFilterThreads) {
+    // ---- load two rows per pair (row a -> re, row b -> im), zero past the end.
+    // PPW * N = 2048: eight steps of 256 elements.  For N >= 256 every step
+    // lies in one pair (IT = N / 256 consecutive steps per pair), so the f64
+    // row sums (the per-view means' input, main.rs:615-623) are reduced over
+    // the workgroup once per pair.
+    constexpr int IT = N >= 256 ? N / 256 : 0;
+    double sq = 0.0, sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int k = 0; k < PPW * N / kFilterThreads; ++k) {
+        const int idx = tid + k * kFilterThreads;
         const int qq = idx >> LOGN;
         const int i = idx & (N - 1);
         float va = 0.f, vb = 0.f;
@@ -165,6 +188,23 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
         }
         sq += static_cast<double>(va) * va + static_cast<double>(vb) * vb;
         sm[qq * RS + p2_slot(i)] = make_float2(va, vb);
+        if constexpr (IT > 0) {
+            if (P.row_sum) {  // uniform
+                sa += static_cast<double>(va);
+                sb += static_cast<double>(vb);
+                if ((k + 1) % IT == 0) {
+                    const double2 t = wg_sum2(sa, sb);
+                    const int pq = k / IT;
+                    const int ra = 2 * (pair0 + pq);
+                    if (tid == 0 && pq < pairs) {
+                        P.row_sum[ra] = t.x;
+                        if (ra + 1 < P.nrows) P.row_sum[ra + 1] = t.y;
+                    }
+                    sa = 0.0;
+                    sb = 0.0;
+                }
+            }
+        }
     }
     if (P.sq_partial) {
         const double t = wg_sum(sq);

@@ -60,6 +60,28 @@ __global__ void means_finalize(const double* partial, int nblk, int64_t nxy, flo
     means[view] = nxy > 0 ? static_cast<float>(s / static_cast<double>(nxy)) : 0.f;
 }
 
+// One workgroup per view: thread t sums rows t, t+256, ... in order, then the
+// 256 partial sums in block_sum's fixed tree.
+__global__ void __launch_bounds__(256) means_from_rows_kernel(const double* row_sum, int rows,
+                                                              int64_t nxy, float* means) {
+    __shared__ double red[256];
+    const int view = blockIdx.x;
+    const double* rs = row_sum + static_cast<int64_t>(view) * rows;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) s += rs[r];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0)
+        means[view] = nxy > 0 ? static_cast<float>(s / static_cast<double>(nxy)) : 0.f;
+}
+
+hipError_t launch_means_from_rows(const double* row_sum, int rows, int64_t nxy, int nz,
+                                  float* means, hipStream_t st) {
+    if (nz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(means_from_rows_kernel, dim3(nz), dim3(256), 0, st, row_sum, rows, nxy,
+                       means);
+    return hipGetLastError();
+}
+
 hipError_t launch_log_means(const float* proj, float* logd, int64_t nxy, int nz, int do_log,
                             double* partial, int nblk, float* means, hipStream_t st) {
     if (nz <= 0) return hipSuccess;

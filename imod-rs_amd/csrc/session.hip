@@ -507,8 +507,9 @@ static int ensure_partial(tiltgpu_session* s, int nblk) {
 }
 
 int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
-               double* sq_out) {
+               double* sq_out, double* row_sum) {
     FilterParams f = make_filter(s, src, sub, stretch);
+    f.row_sum = row_sum;
     const int nblk = filter_blocks(f);
     if (sq_out) {
         int rc = ensure_partial(s, nblk);
@@ -1046,10 +1047,25 @@ int tiltgpu_run(tiltgpu_session* s) {
     if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
     if ((rc = ensure_aligned(s))) return rc;
     const float* O = s->o.log ? s->logd : stack_src(s);
-    // log + means
-    if ((rc = run_means(s))) return rc;
+    // Per-view means (main.rs:615-623).  Without --log the power-of-two filter
+    // path (rows 256-2048 wide) sums each row as it loads it, in f64, and the
+    // means follow from those row sums: the stack is read once for both.
+    // Otherwise (and with --log, whose kernel writes the logged stack) the
+    // log/means kernel reads it first.
+    const bool fused_means = !s->o.log && s->tw8 && s->d.in_nx >= 256;
+    if (fused_means && !s->row_sum &&
+        (rc = dev_alloc(s, &s->row_sum, static_cast<size_t>(s->d.nz) * s->d.in_ny)))
+        return rc;
+    if (!fused_means && (rc = run_means(s))) return rc;
     // R-weighting + cosine stretch (+ storage conversion)
-    if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
+    if ((rc = run_filter(s, O, nullptr, true, nullptr, fused_means ? s->row_sum : nullptr)))
+        return rc;
+    if (fused_means) {
+        Scope sc(s, kClsPrep);
+        HIP_TRY(launch_means_from_rows(s->row_sum, s->d.in_ny,
+                                       static_cast<int64_t>(s->d.in_nx) * s->d.in_ny, s->d.nz,
+                                       s->means, s->stream));
+    }
     s->filtered = s->o.sirt == 0;  // SIRT filters its residuals into the same buffers
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
                                                          : s->store;

@@ -91,6 +91,7 @@ struct tiltgpu_session {
     float* means = nullptr;
     double* partial = nullptr;
     int partial_n = 0;
+    double* row_sum = nullptr;  // per-row f64 sums of the stack (means fused into the filter)
     float* stretch = nullptr;
     float* weights = nullptr;
     float2* tw = nullptr;
@@ -261,7 +262,7 @@ FpParams make_fp(tiltgpu_session* s);
 int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
            int out_mode, float* out);
 int run_filter(tiltgpu_session* s, const float* src, const float* sub, bool stretch,
-               double* sq_out);
+               double* sq_out, double* row_sum = nullptr);
 int run_forward(tiltgpu_session* s);
 int run_means(tiltgpu_session* s);
 const float* stack_src(const tiltgpu_session* s);  // the transformed stack, or raw

@@ -115,6 +115,7 @@ struct FilterParams {
     const float2* tw;       // [n] exp(-2*pi*i*t/n)
     const float* stretch;   // [nviews] 1/cos(theta) or 0 (no stretch); null = none
     double* sq_partial;     // optional: per-workgroup sum of squares of the input rows
+    double* row_sum;        // optional: f64 sum of each input row (power-of-two path, n >= 256)
     int n, nrows, rows_per_view;
     int pairs_per_wg;
     int cubic;
@@ -209,6 +210,10 @@ int filter_plan(int n, int* radix, int* nradix);  // 0 ok, -1 unsupported n
 
 hipError_t launch_log_means(const float* proj, float* logd, int64_t nxy, int nz, int do_log,
                             double* partial, int nblk, float* means, hipStream_t st);
+// Per-view means from per-row f64 sums (row_sum [nz][rows]): each view's rows
+// summed in a fixed order, mean = sum / (rows * n).
+hipError_t launch_means_from_rows(const double* row_sum, int rows, int64_t nxy, int nz,
+                                  float* means, hipStream_t st);
 hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
                                   unsigned* absmax_bits, int64_t nxy, int nz, int storage,
                                   hipStream_t st);
