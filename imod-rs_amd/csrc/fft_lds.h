@@ -364,11 +364,23 @@ __device__ __forceinline__ void p2_load_tw(float2* twr, int j, const float2* __r
 // thread j does the butterflies b = j + u*T (T = N/8, u < N/(R*T)); inputs
 // b + r*N/R, outputs (b - k)*R + k + r*NS with k = b mod NS.  All reads finish
 // (barrier) before any write.
-template <int N, int S, int FUSE>
+// TWG: twr is the per-pass twiddle table itself (tw8), and the pass reads the
+// thread's twiddles from it at its start instead of holding every pass's
+// twiddles in registers across the transforms.
+template <int N, int S, int FUSE, bool TWG = false>
 __device__ __forceinline__ void p2_pass(float2* row, int j, const float2* twr,
                                         const float* __restrict__ weights, float inv_n) {
     constexpr int R = P2<N>::radix(S), NS = P2<N>::span(S), PER = P2<N>::per(S);
     constexpr int T = P2<N>::T, BF = N / R;
+    float2 wl[PER][R - 1 > 0 ? R - 1 : 1];
+    if constexpr (TWG && NS > 1) {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int k = (j + u * T) & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) wl[u][r - 1] = twr[P2<N>::tab_off(S) + k * R + r];
+        }
+    }
     float2 x[PER][R];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -382,7 +394,7 @@ __device__ __forceinline__ void p2_pass(float2* row, int j, const float2* twr,
         const int b = j + u * T;
         const int k = b & (NS - 1);
         if constexpr (NS > 1) {
-            const float2* w = twr + P2<N>::tw_before(S) + u * (R - 1);
+            const float2* w = TWG ? wl[u] : twr + P2<N>::tw_before(S) + u * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) x[u][r] = cmul(x[u][r], w[r - 1]);
         }
@@ -405,13 +417,13 @@ __device__ __forceinline__ void p2_pass(float2* row, int j, const float2* twr,
     __syncthreads();
 }
 
-template <int N, int S, int FUSE>
+template <int N, int S, int FUSE, bool TWG = false>
 __device__ __forceinline__ void p2_fft(float2* row, int j, const float2* twr,
                                        const float* __restrict__ weights, float inv_n) {
     if constexpr (S < P2<N>::passes()) {
         constexpr bool LAST = S + 1 == P2<N>::passes();
-        p2_pass<N, S, LAST ? FUSE : kP2None>(row, j, twr, weights, inv_n);
-        p2_fft<N, S + 1, FUSE>(row, j, twr, weights, inv_n);
+        p2_pass<N, S, LAST ? FUSE : kP2None, TWG>(row, j, twr, weights, inv_n);
+        p2_fft<N, S + 1, FUSE, TWG>(row, j, twr, weights, inv_n);
     }
 }
 

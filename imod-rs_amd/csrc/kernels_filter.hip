@@ -161,8 +161,6 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
     const int pair0 = blk * PPW;
     const int total_pairs = (P.nrows + 1) / 2;
     const int pairs = min(PPW, total_pairs - pair0);
-    float2 twr[P2<N>::tw_total() > 0 ? P2<N>::tw_total() : 1];
-    if (P.weights) p2_load_tw<N, 0>(twr, j, P.tw8);
 
     // ---- load two rows per pair (row a -> re, row b -> im), zero past the end.
     // PPW * N = 2048: eight steps of 256 elements.  For N >= 256 every step
@@ -186,7 +184,7 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
                 if (P.src_sub) vb = vb - P.src_sub[static_cast<long>(rb) * N + i];
             }
         }
-        sq += static_cast<double>(va) * va + static_cast<double>(vb) * vb;
+        if (P.sq_partial) sq += static_cast<double>(va) * va + static_cast<double>(vb) * vb;
         sm[qq * RS + p2_slot(i)] = make_float2(va, vb);
         if constexpr (IT > 0) {
             if (P.row_sum) {  // uniform
@@ -216,8 +214,8 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
         float2* row = sm + q * RS;
         // forward transform with the weight multiply fused into its last pass,
         // then the inverse transform of the conjugate with the 1/n scale fused
-        p2_fft<N, 0, kP2Weight>(row, j, twr, P.weights, 0.f);
-        p2_fft<N, 0, kP2Scale>(row, j, twr, nullptr, 1.0f / static_cast<float>(N));
+        p2_fft<N, 0, kP2Weight, true>(row, j, P.tw8, P.weights, 0.f);
+        p2_fft<N, 0, kP2Scale, true>(row, j, P.tw8, nullptr, 1.0f / static_cast<float>(N));
     }
     // ---- cosine stretch (per row: its own view) and coalesced write
     for (int idx = tid; idx < pairs * N; idx += kFilterThreads) {
@@ -337,6 +335,14 @@ __device__ __forceinline__ void filter_body(const FilterParams& P, int blk) {
 
 __global__ void __launch_bounds__(256) filter_kernel(FilterParams P) { filter_body(P, blockIdx.x); }
 
+// The power-of-two path as kernels of their own: their register allocation
+// (63 VGPRs at n = 2048, 8 waves per SIMD) is not raised to the mixed-radix
+// path's.
+template <int N>
+__global__ void __launch_bounds__(256) filter_p2_kernel(FilterParams P) {
+    filter_body_p2<N>(P, blockIdx.x);
+}
+
 // Several series in one launch: workgroup b belongs to the series r with
 // block_start[r] <= b < block_start[r+1]; each series keeps its own length,
 // plan and options, so its rows are filtered exactly as in a launch of its own.
@@ -432,6 +438,22 @@ hipError_t launch_filter(const FilterParams& p, hipStream_t st) {
     const int nblk = (total_pairs + p.pairs_per_wg - 1) / p.pairs_per_wg;
     if (nblk <= 0) return hipSuccess;
     const size_t lds = filter_lds_bytes(p.n, p.pairs_per_wg);
+    if (p.tw8) {
+        void (*k)(FilterParams) = nullptr;
+        switch (p.log2n) {
+            case 6: k = filter_p2_kernel<64>; break;
+            case 7: k = filter_p2_kernel<128>; break;
+            case 8: k = filter_p2_kernel<256>; break;
+            case 9: k = filter_p2_kernel<512>; break;
+            case 10: k = filter_p2_kernel<1024>; break;
+            case 11: k = filter_p2_kernel<2048>; break;
+            default: break;
+        }
+        if (k) {
+            hipLaunchKernelGGL(k, dim3(nblk), dim3(256), lds, st, p);
+            return hipGetLastError();
+        }
+    }
     const hipError_t e = allow_large_lds(reinterpret_cast<const void*>(filter_kernel), lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(filter_kernel, dim3(nblk), dim3(256), lds, st, p);

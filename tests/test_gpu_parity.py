@@ -164,7 +164,9 @@ def test_filter_power_of_two_widths(gpu, rel, nx):
     """The power-of-two filter (register radix-8 passes, 64 <= n <= 2048; 4096
     takes the mixed-radix passes) against the oracle's R-weighting + stretch,
     with an odd row count so a row pair straddles two views and the last
-    workgroup holds fewer pairs than it can."""
+    workgroup holds fewer pairs than it can.  Widths 256-2048 also take the
+    per-view means from the row sums of the filter's load; every width's means
+    equal the oracle's (min_max_mean) bit for bit."""
     ny, nv = 5, 7
     ang = np.linspace(-60.0, 60.0, nv).astype(np.float32)
     rng = np.random.default_rng(nx)
@@ -172,7 +174,10 @@ def test_filter_power_of_two_widths(gpu, rel, nx):
     print(f"case: filter width {nx}", flush=True)
     vol, hs, s = gpu.reconstruct(P, ang, gpu.TiltOptions(thickness=8), return_session=True)
     filt = s.read_filtered()
+    means = s.read_means()
     s.close()
+    ref_means = np.array([O.min_max_mean(P[v])[2] for v in range(nv)], np.float32)
+    assert np.array_equal(means, ref_means)
     rw = O.rweight(P.reshape(-1, nx), prec="f32").reshape(P.shape)
     st = O.cosine_stretch(rw, ang, prec="f32")
     print(f"width {nx}: vs FP32 oracle {rel(filt, st):.2e}")
