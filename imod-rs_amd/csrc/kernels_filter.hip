@@ -217,7 +217,73 @@ __device__ void filter_body_p2(const FilterParams& P, int blk) {
         p2_fft<N, 0, kP2Weight, true>(row, j, P.tw8, P.weights, 0.f);
         p2_fft<N, 0, kP2Scale, true>(row, j, P.tw8, nullptr, 1.0f / static_cast<float>(N));
     }
-    // ---- cosine stretch (per row: its own view) and coalesced write
+    // ---- cosine stretch (per row: its own view) and coalesced write.
+    // For N >= 256 with the linear stretch, a step of 256 outputs lies in one
+    // pair; when both rows of the pair belong to one view (always, for an even
+    // row count per view) they share the source coordinate and the two
+    // float2 reads, each component interpolated with the arithmetic of
+    // p2_stretch_sample.
+    if constexpr (IT > 0) {
+        if (!P.cubic) {
+            const float center = static_cast<float>(N) / 2.0f;
+            const float fn = static_cast<float>(N);
+            // not unrolled: the registers of eight steps in flight would cost
+            // occupancy (89 instead of 63 VGPRs at n = 2048)
+#pragma unroll 1
+            for (int k = 0; k < PPW * N / kFilterThreads; ++k) {
+                const int pq = k / IT;
+                if (pq >= pairs) break;  // uniform
+                const int ra = 2 * (pair0 + pq);
+                const int rb = ra + 1;
+                const int va_view = __builtin_amdgcn_readfirstlane(ra / P.rows_per_view);
+                const bool hasb = rb < P.nrows;
+                const bool same = !hasb || rb / P.rows_per_view == va_view;
+                if (!same) {
+                    // rows of two views: per row, as the general loop below
+                    const int ox = (tid + k * kFilterThreads) & (N - 1);
+                    const float2* row = sm + pq * RS;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const int r = ra + c;
+                        const float sv = P.stretch ? P.stretch[r / P.rows_per_view] : 0.f;
+                        P.dst[static_cast<long>(r) * N + ox] =
+                            sv != 0.f ? p2_stretch_sample(row, c, N, sv, 0, ox) : p2_at(row, ox, c);
+                    }
+                    continue;
+                }
+                const float sv = P.stretch ? P.stretch[va_view] : 0.f;
+                const int ox = (tid + k * kFilterThreads) & (N - 1);
+                const float2* row = sm + pq * RS;
+                float oa, ob;
+                if (sv != 0.f) {
+                    const float dxs = static_cast<float>(ox) - center;
+                    const float src_x = dxs * sv + center;
+                    const float fl = floorf(src_x);
+                    if (fl >= 0.f && fl + 1.f < fn) {
+                        const int i = static_cast<int>(fl);
+                        const float frac = src_x - fl;
+                        const float2 v0 = row[p2_slot(i)], v1 = row[p2_slot(i + 1)];
+                        oa = v0.x * (1.0f - frac) + v1.x * frac;
+                        ob = v0.y * (1.0f - frac) + v1.y * frac;
+                    } else if (fl >= 0.f && fl < fn) {
+                        const float2 v = row[p2_slot(static_cast<int>(fl))];
+                        oa = v.x;
+                        ob = v.y;
+                    } else {
+                        oa = 0.f;
+                        ob = 0.f;
+                    }
+                } else {
+                    const float2 v = row[p2_slot(ox)];
+                    oa = v.x;
+                    ob = v.y;
+                }
+                P.dst[static_cast<long>(ra) * N + ox] = oa;
+                if (hasb) P.dst[static_cast<long>(rb) * N + ox] = ob;
+            }
+            return;
+        }
+    }
     for (int idx = tid; idx < pairs * N; idx += kFilterThreads) {
         const int qq = idx >> LOGN;
         const int ox = idx & (N - 1);
