@@ -770,9 +770,9 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
         slab_alone = float(np.median(ts))
         out["slab_alone_ms"] = round(slab_alone, 2)
 
-    def slowdowns(lat, kinds):
-        # latency / the job's own time alone: stream request k repeats request
-        # k % n_req of the batched mix (same series, same SIRT count: n_req is even)
+    def own_times(kinds):
+        # each job's own time alone: stream request k repeats request k % n_req
+        # of the batched mix (same series, same SIRT count: n_req is even)
         own, k = [], 0
         for kind in kinds:
             if kind == "recon":
@@ -780,7 +780,10 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
                 k += 1
             else:
                 own.append(slab_alone)
-        return lat / np.array(own, np.float64)
+        return np.array(own, np.float64)
+
+    def slowdowns(lat, kinds):
+        return lat / own_times(kinds)
 
     def summary(lat, kinds, st=None):
         e = {"p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
@@ -790,6 +793,12 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
             sd = slowdowns(lat, kinds)
             e["slowdown_p50"] = round(pct(sd, 50), 2)
             e["slowdown_p99"] = round(pct(sd, 99), 2)
+            # the same jobs' own times alone: the spread the mix itself has,
+            # which no scheduler removes from the latency percentiles
+            own = own_times(kinds)
+            e["alone_p50_ms"] = round(pct(own, 50), 2)
+            e["alone_p99_ms"] = round(pct(own, 99), 2)
+            e["alone_p99_over_p50"] = round(pct(own, 99) / pct(own, 50), 2)
         rec = kinds == "recon"
         e["reconstruction_p50_ms"] = round(pct(lat[rec], 50), 2)
         e["reconstruction_p99_ms"] = round(pct(lat[rec], 99), 2)
