@@ -1014,6 +1014,29 @@ int tiltgpu_copy_to_host(void* dst, const void* src, int64_t bytes) {
     return TILTGPU_OK;
 }
 
+// Per-view means (main.rs:615-623), then the R-weighting + cosine stretch of
+// O.  Without --log the power-of-two filter path (rows 256-2048 wide) sums each
+// row as it loads it, in f64, and the means follow from those row sums: the
+// stack is read once for both.  Otherwise (and with --log, whose kernel writes
+// the logged stack) the log/means kernel reads it first.
+static int run_means_and_filter(tiltgpu_session* s, const float* O) {
+    int rc;
+    const bool fused_means = !s->o.log && s->tw8 && s->d.in_nx >= 256;
+    if (fused_means && !s->row_sum &&
+        (rc = dev_alloc(s, &s->row_sum, static_cast<size_t>(s->d.nz) * s->d.in_ny)))
+        return rc;
+    if (!fused_means && (rc = run_means(s))) return rc;
+    if ((rc = run_filter(s, O, nullptr, true, nullptr, fused_means ? s->row_sum : nullptr)))
+        return rc;
+    if (fused_means) {
+        Scope sc(s, kClsPrep);
+        HIP_TRY(launch_means_from_rows(s->row_sum, s->d.in_ny,
+                                       static_cast<int64_t>(s->d.in_nx) * s->d.in_ny, s->d.nz,
+                                       s->means, s->stream));
+    }
+    return TILTGPU_OK;
+}
+
 int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {
     if (!s || s->seam) return fail(TILTGPU_ERR_STATE, "not a pipeline session");
     if (!s->uploaded) return fail(TILTGPU_ERR_STATE, "projections not uploaded");
@@ -1026,8 +1049,7 @@ int tiltgpu_run_rows(tiltgpu_session* s, int32_t y0, int32_t y1) {
     if (!s->filtered) {
         if ((rc = ensure_aligned(s))) return rc;
         const float* O = s->o.log ? s->logd : stack_src(s);
-        if ((rc = run_means(s))) return rc;
-        if ((rc = run_filter(s, O, nullptr, true, nullptr))) return rc;
+        if ((rc = run_means_and_filter(s, O))) return rc;
         s->filtered = true;
     }
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
@@ -1047,25 +1069,8 @@ int tiltgpu_run(tiltgpu_session* s) {
     if (!s->geometry_ready && (rc = prepare_pipeline_geometry(s))) return rc;
     if ((rc = ensure_aligned(s))) return rc;
     const float* O = s->o.log ? s->logd : stack_src(s);
-    // Per-view means (main.rs:615-623).  Without --log the power-of-two filter
-    // path (rows 256-2048 wide) sums each row as it loads it, in f64, and the
-    // means follow from those row sums: the stack is read once for both.
-    // Otherwise (and with --log, whose kernel writes the logged stack) the
-    // log/means kernel reads it first.
-    const bool fused_means = !s->o.log && s->tw8 && s->d.in_nx >= 256;
-    if (fused_means && !s->row_sum &&
-        (rc = dev_alloc(s, &s->row_sum, static_cast<size_t>(s->d.nz) * s->d.in_ny)))
-        return rc;
-    if (!fused_means && (rc = run_means(s))) return rc;
-    // R-weighting + cosine stretch (+ storage conversion)
-    if ((rc = run_filter(s, O, nullptr, true, nullptr, fused_means ? s->row_sum : nullptr)))
-        return rc;
-    if (fused_means) {
-        Scope sc(s, kClsPrep);
-        HIP_TRY(launch_means_from_rows(s->row_sum, s->d.in_ny,
-                                       static_cast<int64_t>(s->d.in_nx) * s->d.in_ny, s->d.nz,
-                                       s->means, s->stream));
-    }
+    // log + means, R-weighting + cosine stretch (+ storage conversion)
+    if ((rc = run_means_and_filter(s, O))) return rc;
     s->filtered = s->o.sirt == 0;  // SIRT filters its residuals into the same buffers
     const void* proj = s->o.storage == TILTGPU_STORE_F32 ? static_cast<const void*>(s->filt)
                                                          : s->store;
