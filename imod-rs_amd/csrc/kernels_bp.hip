@@ -173,16 +173,34 @@ __device__ __forceinline__ int b128_lane_x(int lane) {
     return (lane & 32) | ((__builtin_popcount(b) & 1) << 4) | ((b >> 1) << 2) | (lane & 3);
 }
 
+// floor(proj_x) at the extremes of z of one x position: the bounds of its
+// staged entries (floor is monotonic).  With local shifts the shift is always
+// added: ldx = +0 without one, and x + 0 == x for every x but -0 (same floor
+// and fraction).
+template <bool LOCAL>
+__device__ __forceinline__ void seg_bounds(float a, float z_first, float z_last, float cx,
+                                           float lx, int& flo, int& fhi) {
+    float pa = (a + z_first) + cx;
+    float pb = (a + z_last) + cx;
+    if (LOCAL) {
+        pa = pa + lx;
+        pb = pb + lx;
+    }
+    flo = min(flo, cvt_floor(fminf(pa, pb)));
+    fhi = max(fhi, cvt_floor(fmaxf(pa, pb)));
+}
+
 template <bool TWO, bool LOCAL, int XPL>
 __device__ __forceinline__ Vs<XPL> view_setup(const BpParams& P, int p, const float (&dx)[XPL],
-                                              const int (&ox)[XPL], int iy, int zt0, int TZ) {
+                                              const int (&ox)[XPL], int iy, int zt0, int TZ,
+                                              int tile) {
     Vs<XPL> s;
     const TILTGPU_CONSTANT BpView* V = cptr(P.views) + p;
     const float cos_t = V->cos_t, cos_xt = V->cos_xt, sin_xt = V->sin_xt;
     const int mode = LOCAL ? V->shift_mode : 0;
     s.shift = mode;
-    const TILTGPU_CONSTANT float* zc = cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad;
-    const float z_first = zc[zt0], z_last = zc[zt0 + TZ - 1];
+    // the segment from the launch's table (views without a patch grid)
+    const bool tab = P.seg_tab != nullptr && mode != 2;  // uniform
     float ldy[XPL];
     int flo = INT_MAX, fhi = INT_MIN;
 #pragma unroll
@@ -199,20 +217,18 @@ __device__ __forceinline__ Vs<XPL> view_setup(const BpParams& P, int p, const fl
         }
         s.ldx[j] = lx;
         ldy[j] = ly;
-        // with local shifts the shift is always added: ldx = +0 without one, and
-        // x + 0 == x for every x but -0 (same floor and fraction)
-        float pa = (s.a[j] + z_first) + P.cx;
-        float pb = (s.a[j] + z_last) + P.cx;
-        if (LOCAL) {
-            pa = pa + lx;
-            pb = pb + lx;
+        if (!tab) {
+            const TILTGPU_CONSTANT float* zc = cptr(P.zc_tab) + static_cast<long>(p) * P.nz_pad;
+            seg_bounds<LOCAL>(s.a[j], zc[zt0], zc[zt0 + TZ - 1], P.cx, lx, flo, fhi);
         }
-        // floor is monotonic, so the floors of the extremes bound the segment
-        flo = min(flo, cvt_floor(fminf(pa, pb)));
-        fhi = max(fhi, cvt_floor(fmaxf(pa, pb)));
     }
     int seg_lo, seg_hi;
-    if (mode == 2) {
+    if (tab) {
+        const TILTGPU_CONSTANT int* sg = reinterpret_cast<const TILTGPU_CONSTANT int*>(
+            cptr(P.seg_tab) + static_cast<long>(p) * (P.ntz * P.ntx) + tile);
+        seg_lo = sg[0];
+        seg_hi = sg[1];
+    } else if (mode == 2) {
         seg_lo = uni(wave_min_i(flo));
         seg_hi = uni(wave_max_i(fhi));
     } else {
@@ -501,6 +517,7 @@ __global__ void __launch_bounds__(kBpThreads)
         }
         const int zt0 = tz * TZ;
         const int zw0 = zt0 + wave * ZPT;
+        const int seg_tile = tz * P.ntx + tx;
 
         float acc[XPL][ZPT];
 #pragma unroll
@@ -515,7 +532,7 @@ __global__ void __launch_bounds__(kBpThreads)
 #pragma unroll
             for (int v = 0; v < VPS; ++v) {
                 if (v < P.nviews) {
-                    cur[v] = view_setup<TWO, LOCAL, XPL>(P, v, dx, ox, iy, zt0, TZ);
+                    cur[v] = view_setup<TWO, LOCAL, XPL>(P, v, dx, ox, iy, zt0, TZ, seg_tile);
                     stage_load<TWO, ST, KS>(P, cur[v], v, raw[v], xs[v]);
                 }
             }
@@ -554,7 +571,7 @@ __global__ void __launch_bounds__(kBpThreads)
                     nxt[v] = cur[v];
                     const int p = (g + 1) * VPS + v;
                     if (p < P.nviews) {
-                        nxt[v] = view_setup<TWO, LOCAL, XPL>(P, p, dx, ox, iy, zt0, TZ);
+                        nxt[v] = view_setup<TWO, LOCAL, XPL>(P, p, dx, ox, iy, zt0, TZ, seg_tile);
                         stage_load<TWO, ST, KS>(P, nxt[v], p, raw[v], xs[v]);
                     }
                 }
@@ -713,6 +730,51 @@ hipError_t launch_backproject_batch(const BpBatch& b, int storage, int two_row, 
         case 256: return launch_bp_batch_z<64>(b, storage, two_row, local, grid, lds, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+// The staged segment of every (view, tile z, tile x) for views without a patch
+// grid: the floors of proj_x at the x positions of lanes 0 and 63 (each of
+// their xpl positions) and the tile's first and last z, exactly as view_setup
+// evaluates them (proj_x is monotonic in x and in z there, so these bound every
+// voxel of the tile).  One thread per entry; patch-grid views keep computing
+// theirs in the workgroup.
+__global__ void __launch_bounds__(256) bp_segment_kernel(BpParams P, int TZ, int xpl, int local,
+                                                         int2* seg) {
+    const long n = static_cast<long>(P.nviews) * P.ntz * P.ntx;
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int tx = static_cast<int>(i % P.ntx);
+    const long rest = i / P.ntx;
+    const int tz = static_cast<int>(rest % P.ntz);
+    const int p = static_cast<int>(rest / P.ntz);
+    const BpView V = P.views[p];
+    const int mode = local ? V.shift_mode : 0;
+    int flo = INT_MAX, fhi = INT_MIN;
+    if (mode != 2) {
+        const float* zc = P.zc_tab + static_cast<long>(p) * P.nz_pad;
+        const float z_first = zc[tz * TZ], z_last = zc[tz * TZ + TZ - 1];
+        const float lx = mode == 1 ? V.sdx : 0.f;
+        for (int j = 0; j < xpl; ++j) {
+            for (int e = 0; e < 2; ++e) {
+                const int ox = (tx * xpl + j) * kBpTileX + (e ? kBpTileX - 1 : 0);
+                const float dx = static_cast<float>(ox) - P.ocx;
+                const float a = (dx * V.cos_t) * V.cos_xt;
+                if (local)
+                    seg_bounds<true>(a, z_first, z_last, P.cx, lx, flo, fhi);
+                else
+                    seg_bounds<false>(a, z_first, z_last, P.cx, lx, flo, fhi);
+            }
+        }
+    }
+    seg[i] = make_int2(flo, fhi);
+}
+
+hipError_t launch_bp_segments(const BpParams& p, int tile_z, int xpl, int2* seg, hipStream_t st) {
+    const long n = static_cast<long>(p.nviews) * p.ntz * p.ntx;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bp_segment_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256),
+                       0, st, p, tile_z, xpl, p.has_local, seg);
+    return hipGetLastError();
 }
 
 hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z, int vps,

@@ -456,7 +456,22 @@ BpParams make_bp(tiltgpu_session* s, const void* proj, const float* scale, int y
 int run_bp(tiltgpu_session* s, const void* proj, const float* scale, int y0, int y1,
            int out_mode, float* out) {
     BpParams p = make_bp(s, proj, scale, y0, y1, out_mode, out);
+    // the staged segment of every (view, tile) of this launch shape, computed
+    // once per launch instead of by the four waves of each workgroup per view
+    const size_t nseg = static_cast<size_t>(p.nviews) * p.ntz * p.ntx;
+    if (nseg > s->seg_cap) {
+        dev_free(s, s->seg_tab);
+        s->seg_tab = nullptr;
+        s->seg_cap = 0;
+        int rc = dev_alloc(s, &s->seg_tab, nseg);
+        if (rc) return rc;
+        s->seg_cap = nseg;
+    }
     Scope sc(s, kClsBp);
+    if (nseg > 0) {
+        HIP_TRY(launch_bp_segments(p, s->tile_z, s->bp_xpl, s->seg_tab, s->stream));
+        p.seg_tab = s->seg_tab;
+    }
     HIP_TRY(launch_backproject(p, s->seam ? TILTGPU_STORE_F32 : s->o.storage, s->two_row ? 1 : 0,
                                s->tile_z, s->bp_vps, s->bp_xpl, s->stream));
     s->n_bp++;

@@ -92,6 +92,8 @@ struct tiltgpu_session {
     double* partial = nullptr;
     int partial_n = 0;
     double* row_sum = nullptr;  // per-row f64 sums of the stack (means fused into the filter)
+    int2* seg_tab = nullptr;    // staged segments of the backprojection tiles (launch_bp_segments)
+    size_t seg_cap = 0;
     float* stretch = nullptr;
     float* weights = nullptr;
     float2* tw = nullptr;

@@ -74,6 +74,10 @@ struct BpParams {
     const float* ldy;
     int lds_cap;              // capacity of one LDS buffer in entries
     int* err;                 // set to 1 when a tile needs more than lds_cap entries
+    // optional: staged segment [lo, hi] of every (view, tile z, tile x) of this
+    // launch's shape, for views without a patch grid ((p * ntz + tz) * ntx + tx);
+    // null: each workgroup computes its segments itself
+    const int2* seg_tab;
 };
 
 // Persistent batched backprojection: the tiles of nreq series (descs[r] covers
@@ -191,6 +195,9 @@ struct FpBatch {
 // Launch shape: tile depth tile_z, vps views staged per barrier (1, 2) and xpl
 // x positions per lane (tile x 64 * xpl); bp_shape_supported lists the built
 // combinations.  The batched kernel runs shape (1, 1).
+// The staged segments of launch_backproject's tiles (BpParams::seg_tab) for
+// tile depth tile_z and xpl x positions per lane: nviews * ntz * ntx entries.
+hipError_t launch_bp_segments(const BpParams& p, int tile_z, int xpl, int2* seg, hipStream_t st);
 hipError_t launch_backproject(const BpParams& p, int storage, int two_row, int tile_z, int vps,
                               int xpl, hipStream_t st);
 bool bp_shape_supported(int tile_z, int vps, int xpl);
