@@ -40,6 +40,7 @@ TILES = [32, 64, 128, 256]
 SHAPES = [(32, 2, 1), (64, 2, 1), (128, 2, 1), (32, 1, 2), (64, 1, 2)]
 FP_WIDTHS = [256, 384, 512, 768, 1024, 1536, 2048]
 FP_BINS = [4, 8, 16]
+FP_ROWS2_MAX = 2048  # widest volume row the two-rows-per-workgroup walk takes
 
 
 def bp_time(P, th, two_row, local, storage, tz, reps=3, vps=1, xpl=1):
@@ -75,14 +76,16 @@ def bp_time(P, th, two_row, local, storage, tz, reps=3, vps=1, xpl=1):
         s.close()
 
 
-def fp_time(w, bins, th=256, nviews=61, reps=2):
+def fp_time(w, bins, th=256, nviews=61, reps=2, rows=1):
     """Median ms per forward-projection launch inside SIRT-4 runs of a w x w
     series of seeded normal projections, thickness th: the launches then run as
     they do in a reconstruction, between backprojections and filters, on a
     volume of realistic values (a constant volume draws less power and ran some
-    shapes faster than real volumes do)."""
+    shapes faster than real volumes do).  rows: volume rows per workgroup (the
+    library reads TILTGPU_FP_ROWS when it builds each launch)."""
     ang = np.linspace(-60.0, 60.0, nviews).astype(np.float32)
     P = np.random.default_rng(w).standard_normal((nviews, w, w)).astype(np.float32)
+    os.environ["TILTGPU_FP_ROWS"] = str(rows)
     s = tg.Session(w, w, nviews, tg.TiltOptions(thickness=th, sirt=4))
     try:
         s.set_angles(ang)
@@ -98,6 +101,7 @@ def fp_time(w, bins, th=256, nviews=61, reps=2):
         return float(np.median(ts))
     finally:
         s.close()
+        del os.environ["TILTGPU_FP_ROWS"]
 
 
 def pad(th, tz):
@@ -162,14 +166,18 @@ def emit(meas, out):
         b11 = min(t for c, t in cand if c[1:] == (1, 1))
         md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | {th} | "
                   f"({ch[0][0]}, {ch[0][1]}, {ch[0][2]}) | {ch[1]:.3f} | {b11:.3f} |")
-    md += ["", "## Forward-projection bins per lane", "",
-           "| width | 4 | 8 | 16 | chosen |", "|---|---|---|---|---|"]
+    md += ["", "## Forward-projection bins per lane and volume rows per workgroup", "",
+           "| width | 4 | 8 | 16 | 4, two rows | 8, two rows | 16, two rows | chosen (bins, rows) |",
+           "|---|---|---|---|---|---|---|---|"]
     fp_rows = []
     for m in meas["fp"]:
-        ms = {int(b): v for b, v in m["ms"].items()}
-        best = min(ms, key=ms.get)
-        fp_rows.append((m["width"], best))
-        md.append(f"| {m['width']} | " + " | ".join(f"{ms[b]:.3f}" for b in FP_BINS) + f" | {best} |")
+        cand = {(int(b), 1): v for b, v in m["ms"].items()}
+        cand.update({(int(b), 2): v for b, v in m.get("ms_rows2", {}).items()})
+        best = min(cand, key=cand.get)
+        fp_rows.append((m["width"], best[0], best[1]))
+        two = [f"{cand[(b, 2)]:.3f}" if (b, 2) in cand else "-" for b in FP_BINS]
+        md.append(f"| {m['width']} | " + " | ".join(f"{cand[(b, 1)]:.3f}" for b in FP_BINS) + " | "
+                  + " | ".join(two) + f" | ({best[0]}, {best[1]}) |")
     with open(os.path.join(out, "tile_table.inc"), "w") as fh:
         fh.write("// Measured launch shapes for gfx950, written by imod-rs_amd/autotune.py.\n")
         fh.write("// Backprojection: per-padded-slice cost k of each launch shape (tile depth,\n")
@@ -181,10 +189,11 @@ def emit(meas, out):
             fh.write("    {%d, %d, %d, %d, %d, %d, %d, %d, %d},\n" % r)
         if not cost_rows:
             fh.write("    {-1, 0, 0, 0, 0, 0, 0, 0, 0},\n")
-        fh.write("};\n// Forward projection: fastest bins per lane per width.  {in_nx, bins_per_lane}\n")
+        fh.write("};\n// Forward projection: fastest bins per lane and volume rows per workgroup per width.\n")
+        fh.write("// {in_nx, bins_per_lane, rows_per_workgroup}\n")
         fh.write("static const FpTune kFpTable[] = {\n")
         for r in fp_rows:
-            fh.write("    {%d, %d},\n" % r)
+            fh.write("    {%d, %d, %d},\n" % r)
         fh.write("};\n")
     with open(os.path.join(out, "autotune.md"), "w") as fh:
         fh.write("\n".join(md) + "\n")
@@ -225,8 +234,11 @@ def main():
                     print(f"# bp {name} {storage} w={w} z={th}: {ms} {shapes}", flush=True)
     for w in FP_WIDTHS if not a.quick else FP_WIDTHS[::3]:
         ms = {str(b): fp_time(w, b) for b in FP_BINS}
-        meas["fp"].append({"width": w, "ms": ms})
-        print(f"# fp w={w}: {ms}", flush=True)
+        entry = {"width": w, "ms": ms}
+        if w <= FP_ROWS2_MAX:
+            entry["ms_rows2"] = {str(b): fp_time(w, b, rows=2) for b in FP_BINS}
+        meas["fp"].append(entry)
+        print(f"# fp w={w}: {entry}", flush=True)
     meas["seconds"] = round(time.time() - t0, 1)
     with open(os.path.join(a.out, "autotune.json"), "w") as fh:
         json.dump(meas, fh, indent=1)

@@ -432,7 +432,220 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
     if (live) walk_store<B>(P, a, x0, iy, p, 1);
 }
 
+// ------------------------------------------------ two volume rows per workgroup
+//
+// In the single-row geometry proj_x does not depend on the volume row, so a
+// lane's window of a z row, and each voxel's px, floor, fraction and bin
+// advance, serve two rows at once; only the products and the register-carried
+// sums are per row.  Row 1's slots sit R1 = (B+3)*256 floats after row 0's (an
+// immediate offset of the same address), and the two rows' staged z rows are
+// interleaved as float2, so one paired read feeds two voxels of both rows.
+// Each row's bins receive exactly the terms, in exactly the order, of the
+// one-row walk: the sinogram is the same bits.
+
+struct RegWalk2 {
+    float r0[2], r1[2], la[2];
+    float* q;
+    int fi;
+};
+
+template <int B>
+__device__ __forceinline__ void reg_begin2(const WalkLane& L, char* qb, float dx, float zcv,
+                                           float2 v, RegWalk2& W) {
+    constexpr int R1 = (B + 3) * kFpWG;
+    const float px = walk_px(L, dx, zcv);
+    W.fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    const float omf = 1.0f - frac;
+    W.q = reinterpret_cast<float*>(qb + W.fi * (4 * kFpWG));
+    W.r0[0] = W.q[0] + v.x * omf;
+    W.r1[0] = W.q[kFpWG] + v.x * frac;
+    W.la[0] = W.q[2 * kFpWG];
+    W.r0[1] = W.q[R1] + v.y * omf;
+    W.r1[1] = W.q[R1 + kFpWG] + v.y * frac;
+    W.la[1] = W.q[R1 + 2 * kFpWG];
+}
+
+template <int B>
+__device__ __forceinline__ void reg_step2(const WalkLane& L, char* qb, float dx, float zcv,
+                                          float2 v, RegWalk2& W) {
+    constexpr int R1 = (B + 3) * kFpWG;
+    const float px = walk_px(L, dx, zcv);
+    const int fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    const float omf = 1.0f - frac;
+    const float a0 = v.x * omf, b0 = v.x * frac;
+    const float a1 = v.y * omf, b1 = v.y * frac;
+    W.q[0] = W.r0[0];
+    W.q[R1] = W.r0[1];
+    float* q = reinterpret_cast<float*>(qb + fi * (4 * kFpWG));
+    const float la0 = q[2 * kFpWG];
+    const float la1 = q[R1 + 2 * kFpWG];
+    const bool adv = fi != W.fi;  // fi == W.fi + 1
+    const float s00 = adv ? W.r1[0] : W.r0[0];
+    const float s01 = adv ? W.la[0] : W.r1[0];
+    const float s10 = adv ? W.r1[1] : W.r0[1];
+    const float s11 = adv ? W.la[1] : W.r1[1];
+    W.r0[0] = s00 + a0;
+    W.r1[0] = s01 + b0;
+    W.la[0] = la0;
+    W.r0[1] = s10 + a1;
+    W.r1[1] = s11 + b1;
+    W.la[1] = la1;
+    W.q = q;
+    W.fi = fi;
+}
+
+template <int B>
+__device__ __forceinline__ void walk_voxel_exact2(const WalkLane& L, char* lds, float dx,
+                                                  float zcv, float2 v) {
+    constexpr int R1 = (B + 3) * kFpWG;
+    const float px = walk_px(L, dx, zcv);
+    const int fi = cvt_floor(px);
+    const float frac = __builtin_amdgcn_fractf(px);
+    float* q = reinterpret_cast<float*>(lds + (static_cast<unsigned>(fi) * (4u * kFpWG) + L.qoff));
+    q[0] += v.x * (1.0f - frac);
+    q[kFpWG] += v.x * frac;
+    q[R1] += v.y * (1.0f - frac);
+    q[R1 + kFpWG] += v.y * frac;
+}
+
+// walk_row's staged form for two rows (vz: the z row of both, interleaved).
+template <int B>
+__device__ __forceinline__ void walk_row2(const WalkLane& L, char* lds,
+                                          const float2* __restrict__ vz, float zcv) {
+    constexpr int R1 = (B + 3) * kFpWG;
+    int e0, e1;
+    walk_window_exact(L, zcv, e0, e1);
+    float dx = static_cast<float>(e0) - L.ocx;
+    const float2* vp = vz + e0;
+    const float2* vend = vz + e1;
+    const float2* vlast = vend - 1;
+    if (reg_walk_ok(L, zcv)) {
+        if (vp < vend) {
+            char* const qb = lds + L.qoff;
+            RegWalk2 W;
+            reg_begin2<B>(L, qb, dx, zcv, vp[0], W);
+            ++vp;
+            dx += 1.0f;
+            for (; vp < vlast; vp += 2) {
+                const float2 v0 = vp[0], v1 = vp[1];
+                reg_step2<B>(L, qb, dx, zcv, v0, W);
+                reg_step2<B>(L, qb, dx + 1.0f, zcv, v1, W);
+                dx += 2.0f;
+            }
+            if (vp < vend) reg_step2<B>(L, qb, dx, zcv, vp[0], W);
+            W.q[0] = W.r0[0];
+            W.q[kFpWG] = W.r1[0];
+            W.q[R1] = W.r0[1];
+            W.q[R1 + kFpWG] = W.r1[1];
+        }
+    } else {
+        for (; vp < vlast; vp += 2) {
+            const float2 v0 = vp[0], v1 = vp[1];
+            walk_voxel_exact2<B>(L, lds, dx, zcv, v0);
+            walk_voxel_exact2<B>(L, lds, dx + 1.0f, zcv, v1);
+            dx += 2.0f;
+        }
+        if (vp < vend) walk_voxel_exact2<B>(L, lds, dx, zcv, vp[0]);
+    }
+}
+
+constexpr int kStageRegs2 = kFpRows2Nx / kFpWG;
+
+template <int B>
+__device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, int blk) {
+    constexpr int R1 = (B + 3) * kFpWG;
+    const int nbx = (P.in_nx + 64 * B - 1) / (64 * B);
+    const int nvg = (P.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
+    const int bx = blk % nbx;
+    const int rest = blk / nbx;
+    const int vg = rest % nvg;
+    const int iy = (rest / nvg) * 2;
+    const bool two = iy + 1 < P.in_ny;  // uniform: the last row of an odd count walks alone
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int p = __builtin_amdgcn_readfirstlane(vg * kFpViewsPerWG + wave);
+    const int x0 = (bx * 64 + lane) * B;
+    bool staged = P.out_nx <= kFpRows2Nx;
+    for (int w = 0; w < kFpViewsPerWG; ++w) {
+        const int pw = vg * kFpViewsPerWG + w;
+        if (pw < P.nviews) {
+            const float4 gw = P.vgeom[pw];
+            if (!(gw.x * gw.z >= kWalkMinC)) staged = false;
+        }
+    }
+    if (!staged) {
+        if (p >= P.nviews || x0 >= P.in_nx) return;
+        fp_walk_global<B>(P, lds, x0, iy, p);
+        if (two) fp_walk_global<B>(P, lds, x0, iy + 1, p);
+        return;
+    }
+    const bool wave_live = p < P.nviews;
+    const bool live = wave_live && x0 < P.in_nx;
+    const int pv = wave_live ? p : 0;
+    const float4 g = P.vgeom[pv];
+    const WalkLane L = walk_lane(P, g, x0, B, 1);
+    float* a = lds + threadIdx.x;
+#pragma unroll
+    for (int s = 1; s < B + 1; ++s) {
+        a[s * kFpWG] = 0.f;
+        a[R1 + s * kFpWG] = 0.f;
+    }
+    const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    const int nx = P.out_nx;
+    const int nz = P.out_nz;
+    const int t = static_cast<int>(threadIdx.x);
+    float2* stg = reinterpret_cast<float2*>(lds + 2 * R1);
+    const int ns = fp_stage_stride(nx);
+    const float* vrow0 = P.vol + static_cast<int64_t>(iy) * nz * nx;
+    const float* vrow1 = two ? vrow0 + static_cast<int64_t>(nz) * nx : vrow0;
+    float r0[kStageRegs2], r1[kStageRegs2];
+    auto fetch = [&](int oz) {
+        const float* s0 = vrow0 + static_cast<int64_t>(oz) * nx;
+        const float* s1 = vrow1 + static_cast<int64_t>(oz) * nx;
+#pragma unroll
+        for (int k = 0; k < kStageRegs2; ++k)
+            if (t + k * kFpWG < nx) {
+                r0[k] = s0[t + k * kFpWG];
+                r1[k] = two ? s1[t + k * kFpWG] : 0.f;
+            }
+    };
+    auto put = [&](int buf) {
+        float2* dst = stg + buf * ns;
+#pragma unroll
+        for (int k = 0; k < kStageRegs2; ++k)
+            if (t + k * kFpWG < nx) dst[t + k * kFpWG] = make_float2(r0[k], r1[k]);
+    };
+    fetch(0);
+    put(0);
+    if (nz > 1) fetch(1);
+    __syncthreads();
+    float zcv = zc[0];
+    for (int oz = 0; oz < nz; ++oz) {
+        const float zc_next = zc[min(oz + 1, nz - 1)];
+        if (oz + 1 < nz) put((oz + 1) & 1);
+        if (oz + 2 < nz) fetch(oz + 2);
+        if (live) walk_row2<B>(L, reinterpret_cast<char*>(lds), stg + (oz & 1) * ns, zcv);
+        zcv = zc_next;
+        __syncthreads();
+    }
+    if (live) {
+        walk_store<B>(P, a, x0, iy, p, 1);
+        if (two) walk_store<B>(P, a + R1, x0, iy + 1, p, 1);
+    }
+}
+
+template <int RPW>
 __device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk) {
+    if constexpr (RPW == 2) {
+        switch (P.bins_per_lane) {
+            case 4: fp_walk_block2<4>(P, lds, blk); break;
+            case 8: fp_walk_block2<8>(P, lds, blk); break;
+            default: fp_walk_block2<16>(P, lds, blk); break;
+        }
+        return;
+    }
     switch (P.bins_per_lane) {
         case 4: fp_walk_block<4>(P, lds, blk); break;
         case 8: fp_walk_block<8>(P, lds, blk); break;
@@ -440,16 +653,20 @@ __device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk)
     }
 }
 
+// RPW: volume rows per workgroup (FpParams::rows_per_wg), a kernel each so that
+// the one-row walk keeps its own register allocation.
+template <int RPW>
 __global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_kernel(FpParams P, int nblk) {
     extern __shared__ float fp_lds[];
-    fp_block(P, fp_lds, xcd_remap(blockIdx.x, nblk));
+    fp_block<RPW>(P, fp_lds, xcd_remap(blockIdx.x, nblk));
 }
 
 __host__ __device__ inline int fp_blocks(const FpParams& p) {
     const int B = p.bins_per_lane;
     const int nbx = (p.in_nx + 64 * B - 1) / (64 * B);
     const int nvg = (p.nviews + kFpViewsPerWG - 1) / kFpViewsPerWG;
-    return nbx * nvg * p.in_ny;
+    const int rows = p.rows_per_wg == 2 ? (p.in_ny + 1) / 2 : p.in_ny;
+    return nbx * nvg * rows;
 }
 
 // Several series in one launch; a workgroup's work and order are those of a
@@ -457,6 +674,7 @@ __host__ __device__ inline int fp_blocks(const FpParams& p) {
 // a range that starts at a multiple of 8 and is padded to one, so that its
 // workgroups spread evenly over the 8 XCDs (a contiguous run of each series per
 // XCD); padding workgroups exit at once.
+template <int RPW>
 __global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
     extern __shared__ float fp_lds[];
     const int b = blockIdx.x;
@@ -474,7 +692,7 @@ __global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7)))
     const int per = (Bt.block_start[lo + 1] - Bt.block_start[lo]) / 8;
     const int logical = (local % 8) * per + local / 8;
     if (logical >= fp_blocks(P)) return;
-    fp_block(P, fp_lds, logical);
+    fp_block<RPW>(P, fp_lds, logical);
 }
 
 // ---------------------------------------------------------------- two-row
@@ -640,8 +858,15 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
 
 size_t fp_lds_bytes(const FpParams& p) {
     // unstaged workgroups: B + 4 slots per lane; staged: B + 3 slots and the
-    // two staged volume rows
+    // two staged volume rows (double buffer); two volume rows per workgroup:
+    // two slot sets and the rows' z rows interleaved
     size_t b = static_cast<size_t>(p.bins_per_lane + 4) * kFpWG * sizeof(float);
+    if (p.rows_per_wg == 2) {
+        if (p.out_nx <= kFpRows2Nx)
+            b = std::max(b, 2 * static_cast<size_t>(p.bins_per_lane + 3) * kFpWG * sizeof(float) +
+                                2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float2));
+        return b;
+    }
     if (p.out_nx <= kStageNx)
         b = std::max(b, static_cast<size_t>(p.bins_per_lane + 3) * kFpWG * sizeof(float) +
                             2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float));
@@ -654,16 +879,24 @@ hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {
     if (p.nviews <= 0 || p.in_ny <= 0) return hipSuccess;
     if (p.bins_per_lane != 4 && p.bins_per_lane != 8 && p.bins_per_lane != 16)
         return hipErrorInvalidValue;
+    if (p.rows_per_wg != 1 && p.rows_per_wg != 2) return hipErrorInvalidValue;
     const int nblk = fp_gather_blocks(p);
-    hipLaunchKernelGGL(fp_walk_kernel, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
+    if (p.rows_per_wg == 2)
+        hipLaunchKernelGGL(fp_walk_kernel<2>, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
+    else
+        hipLaunchKernelGGL(fp_walk_kernel<1>, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
     return hipGetLastError();
 }
 
 hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_t lds,
-                                       hipStream_t st) {
+                                       int rows_per_wg, hipStream_t st) {
     if (total_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fp_walk_batch_kernel, dim3(total_blocks), dim3(kFpWG), lds, st, b,
-                       total_blocks);
+    if (rows_per_wg == 2)
+        hipLaunchKernelGGL(fp_walk_batch_kernel<2>, dim3(total_blocks), dim3(kFpWG), lds, st, b,
+                           total_blocks);
+    else
+        hipLaunchKernelGGL(fp_walk_batch_kernel<1>, dim3(total_blocks), dim3(kFpWG), lds, st, b,
+                           total_blocks);
     return hipGetLastError();
 }
 

@@ -470,39 +470,57 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
     // 1. log + means of new series
     for (Req* r : prep)
         if ((rc = run_means(r->s))) return rc;
-    // 2. forward projection of SIRT iterations (one-row: one batched launch)
+    // 2. forward projection of SIRT iterations (one-row: one batched launch per
+    // walk form, one or two volume rows per workgroup, as each series' tuned
+    // launch would take; every form gives the same bits)
     {
-        std::vector<FpParams> g;
+        std::vector<FpParams> grp[2];
         for (Req* r : iter) {
             if (r->s->two_row) {
                 if ((rc = run_forward(r->s))) return rc;
             } else {
-                g.push_back(make_fp(r->s));
+                const FpParams q = make_fp(r->s);
+                grp[q.rows_per_wg == 2 ? 1 : 0].push_back(q);
             }
         }
-        if (!g.empty()) {
-            const int n = static_cast<int>(g.size());
-            const size_t off_b = n * sizeof(FpParams);
-            const size_t bytes = off_b + (n + 1) * sizeof(int);
+        size_t off_d[2] = {0, 0}, off_b[2] = {0, 0}, bytes = 0;
+        for (int k = 0; k < 2; ++k) {
+            if (grp[k].empty()) continue;
+            off_d[k] = bytes;
+            bytes += grp[k].size() * sizeof(FpParams);
+            off_b[k] = bytes;
+            bytes += (grp[k].size() + 1) * sizeof(int);
+            bytes = (bytes + 255) / 256 * 256;
+        }
+        if (bytes > 0) {
             if ((rc = ensure_batch(sl.fp, bytes))) return rc;
             auto* h = static_cast<char*>(sl.fp.host);
-            auto* hp = reinterpret_cast<FpParams*>(h);
-            auto* hb = reinterpret_cast<int*>(h + off_b);
-            // each series' workgroup range starts at a multiple of 8 and is
-            // padded to one: its workgroups then spread evenly over the 8 XCDs
-            int total = 0;
-            size_t lds = 0;
-            for (int i = 0; i < n; ++i) {
-                hp[i] = g[i];
-                hb[i] = total;
-                total += (fp_gather_blocks(g[i]) + 7) / 8 * 8;
-                lds = std::max(lds, fp_lds_bytes(g[i]));
+            int total[2] = {0, 0};
+            size_t lds[2] = {0, 0};
+            for (int k = 0; k < 2; ++k) {
+                const int n = static_cast<int>(grp[k].size());
+                if (n == 0) continue;
+                auto* hp = reinterpret_cast<FpParams*>(h + off_d[k]);
+                auto* hb = reinterpret_cast<int*>(h + off_b[k]);
+                // each series' workgroup range starts at a multiple of 8 and is
+                // padded to one: its workgroups then spread evenly over the 8 XCDs
+                for (int i = 0; i < n; ++i) {
+                    hp[i] = grp[k][i];
+                    hb[i] = total[k];
+                    total[k] += (fp_gather_blocks(grp[k][i]) + 7) / 8 * 8;
+                    lds[k] = std::max(lds[k], fp_lds_bytes(grp[k][i]));
+                }
+                hb[n] = total[k];
             }
-            hb[n] = total;
             HIP_TRY(hipMemcpyAsync(sl.fp.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
-            FpBatch b{reinterpret_cast<const FpParams*>(sl.fp.dev),
-                      reinterpret_cast<const int*>(static_cast<char*>(sl.fp.dev) + off_b), n};
-            HIP_TRY(launch_forward_gather_batch(b, total, lds, S->stream));
+            auto* d = static_cast<char*>(sl.fp.dev);
+            for (int k = 0; k < 2; ++k) {
+                if (grp[k].empty()) continue;
+                FpBatch b{reinterpret_cast<const FpParams*>(d + off_d[k]),
+                          reinterpret_cast<const int*>(d + off_b[k]),
+                          static_cast<int>(grp[k].size())};
+                HIP_TRY(launch_forward_gather_batch(b, total[k], lds[k], k + 1, S->stream));
+            }
         }
     }
     // 3. filter: new series (R-weighting + stretch) and SIRT residuals O - R

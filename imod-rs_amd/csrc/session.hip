@@ -561,7 +561,16 @@ FpParams make_fp(tiltgpu_session* s) {
     p.nz_pad = s->nz_pad;
     p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
     p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
-    p.bins_per_lane = s->tune_fp_bins > 0 ? s->tune_fp_bins : tuned_fp_bins(s->d.in_nx, nullptr);
+    int rows = 1;
+    const int bins = tuned_fp_bins(s->d.in_nx, nullptr, &rows);
+    p.bins_per_lane = s->tune_fp_bins > 0 ? s->tune_fp_bins : bins;
+    // volume rows per workgroup: the table's choice for the width, or
+    // TILTGPU_FP_ROWS (1 or 2); two rows only for rows up to kFpRows2Nx wide
+    if (const char* e = std::getenv("TILTGPU_FP_ROWS")) {
+        const int r = std::atoi(e);
+        if (r == 1 || r == 2) rows = r;
+    }
+    p.rows_per_wg = rows == 2 && s->d.out_nx <= kFpRows2Nx ? 2 : 1;
     return p;
 }
 

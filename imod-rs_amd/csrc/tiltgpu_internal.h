@@ -151,7 +151,10 @@ struct FpParams {
     int in_nx, in_ny, nviews, out_nx, out_nz, nz_pad;
     float cx, ocx;
     int bins_per_lane;    // 4, 8 or 16 (tuned_fp_bins)
+    int rows_per_wg;      // volume rows walked per workgroup: 1, or 2 (rows up to kFpRows2Nx wide)
 };
+// Widest volume row the two-rows-per-workgroup walk stages (wider rows walk one row).
+constexpr int kFpRows2Nx = 2048;
 
 // Scatter-form forward projection (x-axis tilt and/or local alignment).
 struct FpScatterParams {
@@ -225,8 +228,9 @@ hipError_t launch_convert_storage(const float* src, void* dst, float* scale,
                                   unsigned* absmax_bits, int64_t nxy, int nz, int storage,
                                   hipStream_t st);
 hipError_t launch_forward_gather(const FpParams& p, hipStream_t st);
+// rows_per_wg: the walk form of every series of the batch (FpParams::rows_per_wg)
 hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_t lds,
-                                       hipStream_t st);
+                                       int rows_per_wg, hipStream_t st);
 size_t fp_lds_bytes(const FpParams& p);  // dynamic LDS of one series' workgroups
 int fp_gather_blocks(const FpParams& p);
 // Two-row forward projection, deterministic: the terms of main.rs:866-889 as

@@ -18,6 +18,8 @@ int tuned_bp_shape(int two_row, int local, int half, int in_nx, int out_nz, int*
 
 // Forward-projection bins per lane (4, 8 or 16) for a width: the table entry
 // nearest in log width, else the width-only rule (at least 64 lanes per row).
-int tuned_fp_bins(int in_nx, int* source);
+// rows (optional): volume rows per workgroup of that entry (1 or 2; 1 without
+// a table entry).
+int tuned_fp_bins(int in_nx, int* source, int* rows = nullptr);
 
 }  // namespace tiltgpu

@@ -20,7 +20,9 @@ def _meas():
         shapes[at.shape_key(64, 1, 2)] = 1.5 * th / 64
         bp.append({"variant": "single-row", "two_row": 0, "local": 0, "half": 0, "width": 1024,
                    "thickness": th, "ms": ms, "shapes": shapes})
-    fp = [{"width": 1024, "ms": {"4": 3.0, "8": 2.0, "16": 2.5}}]
+    fp = [{"width": 1024, "ms": {"4": 3.0, "8": 2.0, "16": 2.5}},
+          {"width": 2048, "ms": {"4": 9.0, "8": 7.0, "16": 6.0},
+           "ms_rows2": {"4": 8.0, "8": 5.5, "16": 6.5}}]
     return {"bp": bp, "fp": fp}
 
 
@@ -39,7 +41,9 @@ def test_emit_writes_per_thickness_costs(tmp_path):
     assert by[(64, 64, 1, 1)] == round(1000 * (2.5 / 64) / (2.0 / 128))
     assert by[(64, 64, 1, 2)] == round(1000 * (1.5 / 64) / (2.0 / 128))
     assert (64, 256, 1, 1) not in by and (64, 128, 2, 1) not in by
-    assert "static const FpTune kFpTable[] = {\n    {1024, 8},\n};" in inc
+    # bins and rows per workgroup: one row where only one was measured, two where
+    # the two-row walk was fastest
+    assert "static const FpTune kFpTable[] = {\n    {1024, 8, 1},\n    {2048, 8, 2},\n};" in inc
     md = (tmp_path / "autotune.md").read_text()
     # the choice at the measured thickness is the fastest measured launch
     assert "| single-row | f32 | 1024 | 64 | (64, 1, 2) | 1.500 |" in md
