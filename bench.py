@@ -36,7 +36,7 @@ PEAK_FP32_TFLOPS = 157.3
 # VALU lane-instructions per voxel*view of the C1 backprojection launch
 # (SQ_INSTS_VALU x 64 / voxel*views, profiles/bp_c1_sq_counters.md) and the
 # SIMDs of an MI355X (256 CUs x 4)
-BP_VALU_PER_VV = 12.09
+BP_VALU_PER_VV = 11.66
 N_SIMDS = 1024
 PEAK_HBM_GBS = 8000.0
 
