@@ -597,7 +597,6 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
     const int nz = P.out_nz;
     const int t = static_cast<int>(threadIdx.x);
     float2* stg = reinterpret_cast<float2*>(lds + 2 * R1);
-    const int ns = fp_stage_stride(nx);
     const float* vrow0 = P.vol + static_cast<int64_t>(iy) * nz * nx;
     const float* vrow1 = two ? vrow0 + static_cast<int64_t>(nz) * nx : vrow0;
     float r0[kStageRegs2], r1[kStageRegs2];
@@ -611,24 +610,29 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
                 r1[k] = two ? s1[t + k * kFpWG] : 0.f;
             }
     };
-    auto put = [&](int buf) {
-        float2* dst = stg + buf * ns;
+    auto put = [&]() {
 #pragma unroll
         for (int k = 0; k < kStageRegs2; ++k)
-            if (t + k * kFpWG < nx) dst[t + k * kFpWG] = make_float2(r0[k], r1[k]);
+            if (t + k * kFpWG < nx) stg[t + k * kFpWG] = make_float2(r0[k], r1[k]);
     };
+    // One staging buffer (a second barrier per z row): the two slot sets and a
+    // double buffer would take 39 KB at B = 8, four workgroups per CU; 31 KB
+    // allows five, which outweighs the barrier (C2 launch -5%).
     fetch(0);
-    put(0);
+    put();
     if (nz > 1) fetch(1);
     __syncthreads();
     float zcv = zc[0];
     for (int oz = 0; oz < nz; ++oz) {
         const float zc_next = zc[min(oz + 1, nz - 1)];
-        if (oz + 1 < nz) put((oz + 1) & 1);
-        if (oz + 2 < nz) fetch(oz + 2);
-        if (live) walk_row2<B>(L, reinterpret_cast<char*>(lds), stg + (oz & 1) * ns, zcv);
+        if (live) walk_row2<B>(L, reinterpret_cast<char*>(lds), stg, zcv);
         zcv = zc_next;
         __syncthreads();
+        if (oz + 1 < nz) {
+            put();
+            if (oz + 2 < nz) fetch(oz + 2);
+            __syncthreads();
+        }
     }
     if (live) {
         walk_store<B>(P, a, x0, iy, p, 1);
@@ -859,12 +863,12 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FpScatterParams P) {
 size_t fp_lds_bytes(const FpParams& p) {
     // unstaged workgroups: B + 4 slots per lane; staged: B + 3 slots and the
     // two staged volume rows (double buffer); two volume rows per workgroup:
-    // two slot sets and the rows' z rows interleaved
+    // two slot sets and one buffer of the rows' z rows, interleaved
     size_t b = static_cast<size_t>(p.bins_per_lane + 4) * kFpWG * sizeof(float);
     if (p.rows_per_wg == 2) {
         if (p.out_nx <= kFpRows2Nx)
             b = std::max(b, 2 * static_cast<size_t>(p.bins_per_lane + 3) * kFpWG * sizeof(float) +
-                                2 * static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float2));
+                                static_cast<size_t>(fp_stage_stride(p.out_nx)) * sizeof(float2));
         return b;
     }
     if (p.out_nx <= kStageNx)
