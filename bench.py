@@ -33,7 +33,7 @@ METRIC = "Gvoxel·view/s WBP 2k²×512×121; served p50/p99 latency vs plain FP3
 # accumulate: 7 flop per voxel-view in all.
 BP_FLOP_PER_VV = 7.0
 PEAK_FP32_TFLOPS = 157.3
-# VALU lane-instructions per voxel*view of the C1 backprojection launch
+# VALU lane-instructions per voxel*view of the 2048^2 x 121 -> z512 backprojection launch
 # (SQ_INSTS_VALU x 64 / voxel*views, profiles/bp_c1_sq_counters.md) and the
 # SIMDs of an MI355X (256 CUs x 4)
 BP_VALU_PER_VV = 11.66
@@ -188,7 +188,7 @@ PAGE_CACHE_NOTE = ("input written, synced and evicted from the page cache before
 
 
 def end_to_end_section(tg, P, ang, xt, local):
-    """The `tilt` command line on the C1 workload from files: MRC read in chunks
+    """The `tilt` command line on the 2048^2 x 121 -> z512 workload from files: MRC read in chunks
     overlapped with upload, WBP, and read-back overlapped with the MRC write
     (page-locked buffers both ways).  Wall times as the command reports them;
     the files live in a temporary directory that is removed afterwards."""
@@ -320,7 +320,7 @@ def published_sizes_section(tg, cpu=True, reps=3):
 
 
 def section_transforms_section(tg, P, cpu=True, reps=3):
-    """newstack -x on the C1 stack (2048x2048x121): one seeded small rotation +
+    """newstack -x on the 2048x2048x121 stack: one seeded small rotation +
     shift per section applied on the device (xf_apply_kernel: 4 B read + 4 B
     written per pixel), timed with the session's HIP events; the oracle on two
     sections, single-threaded as the reference's loop, is the CPU figure and the
@@ -412,7 +412,7 @@ def ctf_end_to_end(P, tilt, pixel=4.0, axis=85.0):
 
 
 def ctf_section(tg, P, cpu=True, reps=3, pixel=4.0, e2e=True):
-    """ctfphaseflip on the C1 stack (2048x2048x121): 300 kV, Cs 2.7 mm, defocus
+    """ctfphaseflip on the 2048x2048x121 stack: 300 kV, Cs 2.7 mm, defocus
     -3000 nm, 4 A pixels (automatic strip width: 16 px), tilt axis at 0 deg (the
     reference's default: strips are column bands) and at 85 deg (bounding boxes
     the full width and ~195 rows).  Device time of tiltgpu_ctf_run (HIP events,
@@ -745,10 +745,10 @@ def serving_section(tg, n_req, n_stream, c1=None, budget=96 << 30, streams=True)
     if not streams:
         return out
     cap = out["batched_f32"]["requests_per_s"]
-    # served request streams at 30 / 60 / 90 % of the batched capacity, with slab
-    # requests of the resident C1 series mixed in when it is given; the plain
-    # baseline at the 30 % rate (below its own capacity) and at 90 % of its own
-    # capacity, where its queue is still stable
+    # served request streams at 20 / 60 / 90 % of the batched capacity, with slab
+    # requests of the resident 2048^2 x 121 series mixed in when it is given; the
+    # plain baseline at the 20 % rate (below its own capacity) and at 90 % of its
+    # own capacity, where its queue is still stable
     slab_session = None
     if c1 is not None:
         P1, ang1, xt1, loc1 = c1

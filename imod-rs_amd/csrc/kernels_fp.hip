@@ -617,7 +617,7 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
     };
     // One staging buffer (a second barrier per z row): the two slot sets and a
     // double buffer would take 39 KB at B = 8, four workgroups per CU; 31 KB
-    // allows five, which outweighs the barrier (C2 launch -5%).
+    // allows five, which outweighs the barrier (1024^2 x 61 -> z300 launch: -5%).
     fetch(0);
     put();
     if (nz > 1) fetch(1);

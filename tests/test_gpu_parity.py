@@ -628,11 +628,11 @@ def test_c0_config(gpu, rel):
     P = gpu.synth_series(512, 512, ang, vol_nz=128, n_ellipsoids=200, n_beads=30, seed=0,
                          noise_rel=0.1)
     opts = gpu.TiltOptions(thickness=128, radial=(0.35, 0.05))
-    print("case: C0 512x512x61 -> z128", flush=True)
+    print("case: WBP 512x512x61 -> z128", flush=True)
     g, sg = gpu.reconstruct(P, ang, opts)
     o, so = O.tilt_run(P, ang, O.TiltOptions(thickness=128), nthreads=16)
     err = rel(g, o)
-    print(f"C0 max|d|/rms = {err:.3e}")
+    print(f"512x512x61 -> z128 max|d|/rms = {err:.3e}")
     assert err <= TOL
 
 
