@@ -169,15 +169,8 @@ def emit(meas, out):
     md += ["", "## Forward-projection bins per lane and volume rows per workgroup", "",
            "| width | 4 | 8 | 16 | 4, two rows | 8, two rows | 16, two rows | chosen (bins, rows) |",
            "|---|---|---|---|---|---|---|---|"]
-    fp_rows = []
-    for m in meas["fp"]:
-        cand = {(int(b), 1): v for b, v in m["ms"].items()}
-        cand.update({(int(b), 2): v for b, v in m.get("ms_rows2", {}).items()})
-        best = min(cand, key=cand.get)
-        fp_rows.append((m["width"], best[0], best[1]))
-        two = [f"{cand[(b, 2)]:.3f}" if (b, 2) in cand else "-" for b in FP_BINS]
-        md.append(f"| {m['width']} | " + " | ".join(f"{cand[(b, 1)]:.3f}" for b in FP_BINS) + " | "
-                  + " | ".join(two) + f" | ({best[0]}, {best[1]}) |")
+    fp_rows, fp_md = fp_table_rows(meas["fp"])
+    md += fp_md
     with open(os.path.join(out, "tile_table.inc"), "w") as fh:
         fh.write("// Measured launch shapes for gfx950, written by imod-rs_amd/autotune.py.\n")
         fh.write("// Backprojection: per-padded-slice cost k of each launch shape (tile depth,\n")
@@ -199,15 +192,69 @@ def emit(meas, out):
         fh.write("\n".join(md) + "\n")
 
 
+def fp_table_rows(meas_fp):
+    """(width, bins, rows) of the fastest forward projection per measured width,
+    and the markdown rows of the measurements."""
+    rows, md = [], []
+    for m in meas_fp:
+        cand = {(int(b), 1): v for b, v in m["ms"].items()}
+        cand.update({(int(b), 2): v for b, v in m.get("ms_rows2", {}).items()})
+        best = min(cand, key=cand.get)
+        rows.append((m["width"], best[0], best[1]))
+        two = [f"{cand[(b, 2)]:.3f}" if (b, 2) in cand else "-" for b in FP_BINS]
+        md.append(f"| {m['width']} | " + " | ".join(f"{cand[(b, 1)]:.3f}" for b in FP_BINS) + " | "
+                  + " | ".join(two) + f" | ({best[0]}, {best[1]}) |")
+    return rows, md
+
+
+def replace_fp_table(inc_text, fp_rows):
+    """The table source with its kFpTable block replaced by fp_rows."""
+    head, sep, rest = inc_text.partition("static const FpTune kFpTable[] = {\n")
+    if not sep:
+        raise ValueError("no kFpTable block in the table source")
+    _, sep2, tail = rest.partition("};\n")
+    body = "".join("    {%d, %d, %d},\n" % r for r in fp_rows)
+    return head + sep + body + sep2 + tail
+
+
+def measure_fp_only(out, quick=False):
+    """Measure the forward projection only and write DIR/tile_table.inc (the
+    compiled table's backprojection rows, new FP rows) and DIR/autotune_fp.md."""
+    t0 = time.time()
+    meas = []
+    for w in FP_WIDTHS if not quick else FP_WIDTHS[::3]:
+        entry = {"width": w, "ms": {str(b): fp_time(w, b) for b in FP_BINS}}
+        if w <= FP_ROWS2_MAX:
+            entry["ms_rows2"] = {str(b): fp_time(w, b, rows=2) for b in FP_BINS}
+        meas.append(entry)
+        print(f"# fp w={w}: {entry}", flush=True)
+    rows, md = fp_table_rows(meas)
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "tile_table.inc")
+    with open(src) as fh:
+        text = replace_fp_table(fh.read(), rows)
+    with open(os.path.join(out, "tile_table.inc"), "w") as fh:
+        fh.write(text)
+    with open(os.path.join(out, "autotune_fp.json"), "w") as fh:
+        json.dump({"fp": meas, "seconds": round(time.time() - t0, 1)}, fh, indent=1)
+    with open(os.path.join(out, "autotune_fp.md"), "w") as fh:
+        fh.write("\n".join(["| width | 4 | 8 | 16 | 4, two rows | 8, two rows | 16, two rows | "
+                            "chosen (bins, rows) |", "|---|---|---|---|---|---|---|---|"] + md) + "\n")
+    print(f"# wrote {out}/tile_table.inc, autotune_fp.json and autotune_fp.md", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--quick", action="store_true", help="two widths and thicknesses only")
     ap.add_argument("--from-json", help="re-emit the table from a saved autotune.json")
     ap.add_argument("--fp-only", action="store_true",
-                    help="with --from-json: keep its backprojection measurements, measure FP again")
+                    help="measure FP again; with --from-json keep its backprojection "
+                         "measurements, without it keep the compiled table's backprojection rows")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
+    if a.fp_only and not a.from_json:
+        measure_fp_only(a.out, a.quick)
+        return
     if a.from_json and not a.fp_only:
         with open(a.from_json) as fh:
             emit(json.load(fh), a.out)

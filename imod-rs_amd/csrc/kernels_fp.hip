@@ -322,6 +322,141 @@ __device__ __forceinline__ void walk_store(const FpParams& P, const float* a, in
         if (x0 + j < P.in_nx) row[x0 + j] = a[(j + first) * kFpWG];
 }
 
+// ------------------------------------------------ sliding lane windows
+//
+// With fixed windows a lane owns bins [x0, x0+B) of its (view, row) for the
+// whole walk, and at tilt t only about a fraction cos t of a view's lanes find
+// bins of a z row in reach: the others idle while the wave walks.  A sliding
+// map gives the view's ncol lane columns (64 per workgroup in x) windows of
+//   bw = ceil((cos t * (out_nx-1) + 4) / (ncol-1)) <= B
+// bins, and for each z row makes windows w0 .. w0+ncol-1 live, with
+// w0 = floor(lo / bw) and lo = max(floor(px of voxel 0), 0).  px is
+// non-decreasing in ox, so every bin the row reaches lies in
+// [lo, lo + cos t*(out_nx-1) + 2]: inside the live windows, which reach
+// lo + (ncol-1)*bw.  Window w is always walked by column w mod ncol, whose
+// lane-private slots keep its sums, so each bin still receives its terms from
+// one lane in the reference's (oz, ox) order: the same bits as fixed windows.
+// px of voxel 0 is monotonic in z (zc is monotonic in oz and every rounding
+// step is), so w0 moves one way: a window that leaves the live range is final;
+// its lane writes it out, clears the slots and takes the window ncol further
+// on.  Windows of a column never live get no terms and are written as zeros.
+struct SlideMap {
+    int bw;          // bins per window of this view
+    int ncol;        // lane columns of the view
+    int col;         // this lane's column
+    int w;           // the lane's current window
+    int wmin, wmax;  // first and last window the lane has held
+    float inv_bw;
+    bool slide;      // uniform over the wave (one view)
+};
+
+template <int B>
+__device__ __forceinline__ SlideMap slide_map(const FpParams& P, float4 g, int nbx, int bx,
+                                              int lane) {
+    SlideMap M;
+    M.ncol = 64 * nbx;
+    M.col = bx * 64 + lane;
+    M.bw = B;
+    M.slide = false;
+    if (P.slide_windows) {
+        const float need = (g.x * g.z * static_cast<float>(P.out_nx - 1) + 4.0f) /
+                           static_cast<float>(M.ncol - 1);
+        const int b = static_cast<int>(ceilf(need));
+        if (b < B) {
+            M.bw = max(b, 1);
+            M.slide = true;
+        }
+    }
+    M.inv_bw = 1.0f / static_cast<float>(M.bw);
+    M.w = M.col;
+    M.wmin = M.wmax = M.col;
+    return M;
+}
+
+// First live window of z row zc (uniform over the wave).
+__device__ __forceinline__ int slide_w0(const WalkLane& L, const SlideMap& M, float zcv) {
+    const int lo = max(cvt_floor(walk_px(L, -L.ocx, zcv)), 0);
+    int w0 = static_cast<int>(static_cast<float>(lo) * M.inv_bw);
+    if (w0 * M.bw > lo) --w0;
+    if ((w0 + 1) * M.bw <= lo) ++w0;
+    return w0;
+}
+
+// Exact-window bounds and slot offset of the lane's window [x0, x0+bw).
+__device__ __forceinline__ void set_window(WalkLane& L, const FpParams& P, int x0, int bw) {
+    L.b_lo = static_cast<float>(max(x0 - 1, 0));
+    L.b_hi = static_cast<float>(min(x0 + bw, P.in_nx - 1));
+    L.qoff = 4u * threadIdx.x + static_cast<unsigned>(1 - x0) * (4u * kFpWG);
+}
+
+// Write the bins of window w (slots 1..bw of each row's slot set, R1 floats
+// apart) to the sinogram rows (row 0 at sino + rowoff, row 1 after it when
+// two), and clear the slots if CLEAR.
+template <int NR, bool CLEAR>
+__device__ __forceinline__ void window_flush(const FpParams& P, float* a, int R1, int x0, int bw,
+                                             int64_t rowoff, bool two) {
+    float* row0 = P.sino + rowoff;
+    float* row1 = two ? row0 + P.in_nx : nullptr;
+    for (int j = 0; j < bw; ++j) {
+        float* s = a + (j + 1) * kFpWG;
+        if (x0 + j < P.in_nx) {
+            row0[x0 + j] = s[0];
+            if (NR == 2 && row1) row1[x0 + j] = s[R1];
+        }
+        if constexpr (CLEAR) {
+            s[0] = 0.f;
+            if (NR == 2) s[R1] = 0.f;
+        }
+    }
+}
+
+// Before z row zc: move the lane to its live window, writing out the windows
+// that left the live range.
+template <int NR>
+__device__ __forceinline__ void slide_to(const FpParams& P, WalkLane& L, SlideMap& M, float* a,
+                                         int R1, float zcv, int64_t rowoff, bool two) {
+    const int w0 = slide_w0(L, M, zcv);
+    if (M.w < w0 || M.w >= w0 + M.ncol) {
+        do {
+            window_flush<NR, true>(P, a, R1, M.w * M.bw, M.bw, rowoff, two);
+            M.w += M.w < w0 ? M.ncol : -M.ncol;
+        } while (M.w < w0 || M.w >= w0 + M.ncol);
+        set_window(L, P, M.w * M.bw, M.bw);
+        M.wmin = min(M.wmin, M.w);
+        M.wmax = max(M.wmax, M.w);
+    }
+}
+
+// First window of the lane at z row zc.
+__device__ __forceinline__ void slide_start(const FpParams& P, WalkLane& L, SlideMap& M,
+                                            float zcv) {
+    const int w0 = slide_w0(L, M, zcv);
+    int r = (M.col - w0) % M.ncol;
+    if (r < 0) r += M.ncol;
+    M.w = w0 + r;
+    M.wmin = M.wmax = M.w;
+    set_window(L, P, M.w * M.bw, M.bw);
+}
+
+// After the walk: the current window, and zeros for the column's windows that
+// were never live.
+template <int NR>
+__device__ __forceinline__ void slide_finish(const FpParams& P, const SlideMap& M, float* a,
+                                             int R1, int64_t rowoff, bool two) {
+    window_flush<NR, false>(P, a, R1, M.w * M.bw, M.bw, rowoff, two);
+    if (!M.slide) return;
+    float* row0 = P.sino + rowoff;
+    float* row1 = two ? row0 + P.in_nx : nullptr;
+    for (int w = M.col; w * M.bw < P.in_nx; w += M.ncol) {
+        if (w >= M.wmin && w <= M.wmax) continue;
+        const int x0 = w * M.bw;
+        for (int j = 0; j < M.bw && x0 + j < P.in_nx; ++j) {
+            row0[x0 + j] = 0.f;
+            if (NR == 2 && row1) row1[x0 + j] = 0.f;
+        }
+    }
+}
+
 // Unstaged form: each wave walks its own (view, row) reading the volume from
 // HBM, or scans bin by bin for views with cos_t*cos_xt < kWalkMinC.
 template <int B>
@@ -389,14 +524,16 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
     }
     // Every thread takes part in staging and barriers; only live lanes walk.
     const bool wave_live = p < P.nviews;
-    const bool live = wave_live && x0 < P.in_nx;
     const int pv = wave_live ? p : 0;
     const float4 g = P.vgeom[pv];
     // exact windows: slot 0 and slot B+1 take the terms of bins x0-1 and x0+B
-    const WalkLane L = walk_lane(P, g, x0, B, 1);
+    WalkLane L = walk_lane(P, g, x0, B, 1);
+    SlideMap M = slide_map<B>(P, g, nbx, bx, lane);
+    const int64_t rowoff = (static_cast<int64_t>(pv) * P.in_ny + iy) * P.in_nx;
 #pragma unroll
     for (int s = 1; s < B + 1; ++s) a[s * kFpWG] = 0.f;
     const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    if (M.slide) slide_start(P, L, M, zc[0]);
     const int nx = P.out_nx;
     const int nz = P.out_nz;
     const int t = static_cast<int>(threadIdx.x);
@@ -425,11 +562,15 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
         const float zc_next = zc[min(oz + 1, nz - 1)];
         if (oz + 1 < nz) put((oz + 1) & 1);
         if (oz + 2 < nz) fetch(oz + 2);
-        if (live) walk_row<B, true>(L, reinterpret_cast<char*>(lds), stg + (oz & 1) * ns, zcv);
+        if (wave_live) {
+            if (M.slide && oz > 0) slide_to<1>(P, L, M, a, 0, zcv, rowoff, false);
+            if (M.w * M.bw < P.in_nx)
+                walk_row<B, true>(L, reinterpret_cast<char*>(lds), stg + (oz & 1) * ns, zcv);
+        }
         zcv = zc_next;
         __syncthreads();
     }
-    if (live) walk_store<B>(P, a, x0, iy, p, 1);
+    if (wave_live) slide_finish<1>(P, M, a, 0, rowoff, false);
 }
 
 // ------------------------------------------------ two volume rows per workgroup
@@ -582,10 +723,11 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
         return;
     }
     const bool wave_live = p < P.nviews;
-    const bool live = wave_live && x0 < P.in_nx;
     const int pv = wave_live ? p : 0;
     const float4 g = P.vgeom[pv];
-    const WalkLane L = walk_lane(P, g, x0, B, 1);
+    WalkLane L = walk_lane(P, g, x0, B, 1);
+    SlideMap M = slide_map<B>(P, g, nbx, bx, lane);
+    const int64_t rowoff = (static_cast<int64_t>(pv) * P.in_ny + iy) * P.in_nx;
     float* a = lds + threadIdx.x;
 #pragma unroll
     for (int s = 1; s < B + 1; ++s) {
@@ -593,6 +735,7 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
         a[R1 + s * kFpWG] = 0.f;
     }
     const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    if (M.slide) slide_start(P, L, M, zc[0]);
     const int nx = P.out_nx;
     const int nz = P.out_nz;
     const int t = static_cast<int>(threadIdx.x);
@@ -625,7 +768,10 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
     float zcv = zc[0];
     for (int oz = 0; oz < nz; ++oz) {
         const float zc_next = zc[min(oz + 1, nz - 1)];
-        if (live) walk_row2<B>(L, reinterpret_cast<char*>(lds), stg, zcv);
+        if (wave_live) {
+            if (M.slide && oz > 0) slide_to<2>(P, L, M, a, R1, zcv, rowoff, two);
+            if (M.w * M.bw < P.in_nx) walk_row2<B>(L, reinterpret_cast<char*>(lds), stg, zcv);
+        }
         zcv = zc_next;
         __syncthreads();
         if (oz + 1 < nz) {
@@ -634,10 +780,7 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
             __syncthreads();
         }
     }
-    if (live) {
-        walk_store<B>(P, a, x0, iy, p, 1);
-        if (two) walk_store<B>(P, a + R1, x0, iy + 1, p, 1);
-    }
+    if (wave_live) slide_finish<2>(P, M, a, R1, rowoff, two);
 }
 
 template <int RPW>

@@ -571,6 +571,9 @@ FpParams make_fp(tiltgpu_session* s) {
         if (r == 1 || r == 2) rows = r;
     }
     p.rows_per_wg = rows == 2 && s->d.out_nx <= kFpRows2Nx ? 2 : 1;
+    // sliding lane windows (same bits as fixed ones); TILTGPU_FP_SLIDE=0 keeps them fixed
+    p.slide_windows = 1;
+    if (const char* e = std::getenv("TILTGPU_FP_SLIDE")) p.slide_windows = std::atoi(e) != 0;
     return p;
 }
 

@@ -152,6 +152,8 @@ struct FpParams {
     float cx, ocx;
     int bins_per_lane;    // 4, 8 or 16 (tuned_fp_bins)
     int rows_per_wg;      // volume rows walked per workgroup: 1, or 2 (rows up to kFpRows2Nx wide)
+    int slide_windows;    // 1: each view's lane windows narrow to its cos and follow the
+                          // z row's bins (staged walks); 0: fixed windows of bins_per_lane
 };
 // Widest volume row the two-rows-per-workgroup walk stages (wider rows walk one row).
 constexpr int kFpRows2Nx = 2048;

@@ -53,3 +53,22 @@ def test_emit_without_measurements_writes_a_sentinel(tmp_path):
     at.emit({"bp": [], "fp": []}, str(tmp_path))
     inc = (tmp_path / "tile_table.inc").read_text()
     assert "{-1, 0, 0, 0, 0, 0, 0, 0, 0}" in inc
+
+
+def test_fp_only_rewrite_keeps_backprojection_rows():
+    """--fp-only without a saved measurement: the compiled table's backprojection
+    rows stay as they are and only the FP block is replaced."""
+    src = os.path.join(os.path.dirname(at.__file__), "csrc", "tile_table.inc")
+    with open(src) as fh:
+        text = fh.read()
+    rows, md = at.fp_table_rows(_meas()["fp"])
+    assert rows == [(1024, 8, 1), (2048, 8, 2)]
+    assert md[1].endswith("| (8, 2) |")
+    new = at.replace_fp_table(text, rows)
+    before, _, fp_old = text.partition("static const FpTune kFpTable[] = {")
+    after, _, fp_new = new.partition("static const FpTune kFpTable[] = {")
+    assert before == after
+    assert fp_new.startswith("\n    {1024, 8, 1},\n    {2048, 8, 2},\n};\n")
+    assert fp_new.split("};\n", 1)[1] == fp_old.split("};\n", 1)[1]
+    with pytest.raises(ValueError):
+        at.replace_fp_table("no table here", rows)

@@ -203,21 +203,26 @@ def test_sirt_matches_oracle(gpu, rel):
     (2300, 3, 2300, 12, 7),   # 16 bins per lane; wider than the two-row walk stages
 ])
 @pytest.mark.parametrize("rows", [1, 2])
-def test_reproject_bitwise(gpu, monkeypatch, in_nx, ny, out_nx, thick, nviews, rows):
+@pytest.mark.parametrize("slide", [0, 1])
+def test_reproject_bitwise(gpu, monkeypatch, in_nx, ny, out_nx, thick, nviews, rows, slide):
     """Forward projection (SIRT reprojection) equals the reference's serial
     scatter (main.rs:830-896) bit for bit in the single-row geometry, including
-    views at 0 and +-60 deg, z factors, and views steeper than 87 deg and past
-    90 deg (the per-bin scan path), with one or two volume rows per workgroup
-    (TILTGPU_FP_ROWS; an odd row count leaves the last row alone, and rows wider
-    than the two-row walk stages walk one row)."""
+    views at 0 and +-60 deg, z factors (one of 3.5, whose z rows move the
+    sliding windows by more than one window), and views steeper than 87 deg and
+    past 90 deg (the per-bin scan path), with one or two volume rows per
+    workgroup (TILTGPU_FP_ROWS; an odd row count leaves the last row alone, and
+    rows wider than the two-row walk stages walk one row), with fixed and with
+    sliding lane windows (TILTGPU_FP_SLIDE)."""
     monkeypatch.setenv("TILTGPU_FP_ROWS", str(rows))
+    monkeypatch.setenv("TILTGPU_FP_SLIDE", str(slide))
     rng = np.random.default_rng(in_nx + thick)
     ang = np.linspace(-62, 62, nviews).astype(np.float32)
     ang[1], ang[2], ang[-1] = 0.0, 87.5, 95.0
     zf = np.where(np.arange(nviews) % 3 == 0, 1.3, 1.0).astype(np.float32)
+    zf[4] = 3.5
     V = rng.standard_normal((ny, thick, out_nx)).astype(np.float32)
     print(f"case: reproject in_nx={in_nx} out_nx={out_nx} thick={thick} views={nviews} "
-          f"rows={rows}", flush=True)
+          f"rows={rows} slide={slide}", flush=True)
     with gpu.Session(in_nx, ny, nviews, gpu.TiltOptions(thickness=thick, width=out_nx,
                                                         no_cosine_stretch=True)) as s:
         s.set_angles(ang)
