@@ -76,16 +76,18 @@ def bp_time(P, th, two_row, local, storage, tz, reps=3, vps=1, xpl=1):
         s.close()
 
 
-def fp_time(w, bins, th=256, nviews=61, reps=2, rows=1):
+def fp_time(w, bins, th=256, nviews=61, reps=2, rows=1, slide=0):
     """Median ms per forward-projection launch inside SIRT-4 runs of a w x w
     series of seeded normal projections, thickness th: the launches then run as
     they do in a reconstruction, between backprojections and filters, on a
     volume of realistic values (a constant volume draws less power and ran some
-    shapes faster than real volumes do).  rows: volume rows per workgroup (the
-    library reads TILTGPU_FP_ROWS when it builds each launch)."""
+    shapes faster than real volumes do).  rows: volume rows per workgroup,
+    slide: lane window map, 0 fixed or 1 sliding (the library reads
+    TILTGPU_FP_ROWS and TILTGPU_FP_SLIDE when it builds each launch)."""
     ang = np.linspace(-60.0, 60.0, nviews).astype(np.float32)
     P = np.random.default_rng(w).standard_normal((nviews, w, w)).astype(np.float32)
     os.environ["TILTGPU_FP_ROWS"] = str(rows)
+    os.environ["TILTGPU_FP_SLIDE"] = str(slide)
     s = tg.Session(w, w, nviews, tg.TiltOptions(thickness=th, sirt=4))
     try:
         s.set_angles(ang)
@@ -102,6 +104,7 @@ def fp_time(w, bins, th=256, nviews=61, reps=2, rows=1):
     finally:
         s.close()
         del os.environ["TILTGPU_FP_ROWS"]
+        del os.environ["TILTGPU_FP_SLIDE"]
 
 
 def pad(th, tz):
@@ -166,9 +169,8 @@ def emit(meas, out):
         b11 = min(t for c, t in cand if c[1:] == (1, 1))
         md.append(f"| {m['variant']} | {'f16' if m['half'] else 'f32'} | {m['width']} | {th} | "
                   f"({ch[0][0]}, {ch[0][1]}, {ch[0][2]}) | {ch[1]:.3f} | {b11:.3f} |")
-    md += ["", "## Forward-projection bins per lane and volume rows per workgroup", "",
-           "| width | 4 | 8 | 16 | 4, two rows | 8, two rows | 16, two rows | chosen (bins, rows) |",
-           "|---|---|---|---|---|---|---|---|"]
+    md += ["", "## Forward-projection bins per lane, volume rows per workgroup, window map", "",
+           FP_MD_HEAD, "|---|" + "---|" * (len(FP_FORMS) * len(FP_BINS) + 1)]
     fp_rows, fp_md = fp_table_rows(meas["fp"])
     md += fp_md
     with open(os.path.join(out, "tile_table.inc"), "w") as fh:
@@ -182,28 +184,48 @@ def emit(meas, out):
             fh.write("    {%d, %d, %d, %d, %d, %d, %d, %d, %d},\n" % r)
         if not cost_rows:
             fh.write("    {-1, 0, 0, 0, 0, 0, 0, 0, 0},\n")
-        fh.write("};\n// Forward projection: fastest bins per lane and volume rows per workgroup per width.\n")
-        fh.write("// {in_nx, bins_per_lane, rows_per_workgroup}\n")
+        fh.write("};\n// Forward projection: fastest bins per lane, volume rows per workgroup and lane\n")
+        fh.write("// window map (0 fixed, 1 sliding) per width.\n")
+        fh.write("// {in_nx, bins_per_lane, rows_per_workgroup, sliding_windows}\n")
         fh.write("static const FpTune kFpTable[] = {\n")
         for r in fp_rows:
-            fh.write("    {%d, %d, %d},\n" % r)
+            fh.write("    {%d, %d, %d, %d},\n" % r)
         fh.write("};\n")
     with open(os.path.join(out, "autotune.md"), "w") as fh:
         fh.write("\n".join(md) + "\n")
 
 
+def fp_measure(w):
+    """Every forward-projection form at width w (two rows only up to FP_ROWS2_MAX)."""
+    entry = {"width": w}
+    for key, r, sl in FP_FORMS:
+        if r == 1 or w <= FP_ROWS2_MAX:
+            entry[key] = {str(b): fp_time(w, b, rows=r, slide=sl) for b in FP_BINS}
+    print(f"# fp w={w}: {entry}", flush=True)
+    return entry
+
+
+# measurement keys of the forward-projection forms: (rows per workgroup, window map)
+FP_FORMS = [("ms", 1, 0), ("ms_rows2", 2, 0), ("ms_slide", 1, 1), ("ms_rows2_slide", 2, 1)]
+FP_MD_HEAD = ("| width | " + " | ".join(
+    f"{b}{', two rows' if r == 2 else ''}{', sliding' if sl else ''}"
+    for _, r, sl in FP_FORMS for b in FP_BINS) + " | chosen (bins, rows, sliding) |")
+
+
 def fp_table_rows(meas_fp):
-    """(width, bins, rows) of the fastest forward projection per measured width,
-    and the markdown rows of the measurements."""
+    """(width, bins, rows, sliding) of the fastest forward projection per measured
+    width, and the markdown rows of the measurements."""
     rows, md = [], []
     for m in meas_fp:
-        cand = {(int(b), 1): v for b, v in m["ms"].items()}
-        cand.update({(int(b), 2): v for b, v in m.get("ms_rows2", {}).items()})
+        cand = {}
+        for key, r, sl in FP_FORMS:
+            cand.update({(int(b), r, sl): v for b, v in m.get(key, {}).items()})
         best = min(cand, key=cand.get)
-        rows.append((m["width"], best[0], best[1]))
-        two = [f"{cand[(b, 2)]:.3f}" if (b, 2) in cand else "-" for b in FP_BINS]
-        md.append(f"| {m['width']} | " + " | ".join(f"{cand[(b, 1)]:.3f}" for b in FP_BINS) + " | "
-                  + " | ".join(two) + f" | ({best[0]}, {best[1]}) |")
+        rows.append((m["width"], best[0], best[1], best[2]))
+        cells = [f"{cand[(b, r, sl)]:.3f}" if (b, r, sl) in cand else "-"
+                 for _, r, sl in FP_FORMS for b in FP_BINS]
+        md.append(f"| {m['width']} | " + " | ".join(cells)
+                  + f" | ({best[0]}, {best[1]}, {best[2]}) |")
     return rows, md
 
 
@@ -213,7 +235,7 @@ def replace_fp_table(inc_text, fp_rows):
     if not sep:
         raise ValueError("no kFpTable block in the table source")
     _, sep2, tail = rest.partition("};\n")
-    body = "".join("    {%d, %d, %d},\n" % r for r in fp_rows)
+    body = "".join("    {%d, %d, %d, %d},\n" % r for r in fp_rows)
     return head + sep + body + sep2 + tail
 
 
@@ -223,11 +245,7 @@ def measure_fp_only(out, quick=False):
     t0 = time.time()
     meas = []
     for w in FP_WIDTHS if not quick else FP_WIDTHS[::3]:
-        entry = {"width": w, "ms": {str(b): fp_time(w, b) for b in FP_BINS}}
-        if w <= FP_ROWS2_MAX:
-            entry["ms_rows2"] = {str(b): fp_time(w, b, rows=2) for b in FP_BINS}
-        meas.append(entry)
-        print(f"# fp w={w}: {entry}", flush=True)
+        meas.append(fp_measure(w))
     rows, md = fp_table_rows(meas)
     src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "tile_table.inc")
     with open(src) as fh:
@@ -237,8 +255,8 @@ def measure_fp_only(out, quick=False):
     with open(os.path.join(out, "autotune_fp.json"), "w") as fh:
         json.dump({"fp": meas, "seconds": round(time.time() - t0, 1)}, fh, indent=1)
     with open(os.path.join(out, "autotune_fp.md"), "w") as fh:
-        fh.write("\n".join(["| width | 4 | 8 | 16 | 4, two rows | 8, two rows | 16, two rows | "
-                            "chosen (bins, rows) |", "|---|---|---|---|---|---|---|---|"] + md) + "\n")
+        fh.write("\n".join([FP_MD_HEAD, "|---|" + "---|" * (len(FP_FORMS) * len(FP_BINS) + 1)]
+                            + md) + "\n")
     print(f"# wrote {out}/tile_table.inc, autotune_fp.json and autotune_fp.md", flush=True)
 
 
@@ -280,12 +298,7 @@ def main():
                                        "ms": ms, "shapes": shapes})
                     print(f"# bp {name} {storage} w={w} z={th}: {ms} {shapes}", flush=True)
     for w in FP_WIDTHS if not a.quick else FP_WIDTHS[::3]:
-        ms = {str(b): fp_time(w, b) for b in FP_BINS}
-        entry = {"width": w, "ms": ms}
-        if w <= FP_ROWS2_MAX:
-            entry["ms_rows2"] = {str(b): fp_time(w, b, rows=2) for b in FP_BINS}
-        meas["fp"].append(entry)
-        print(f"# fp w={w}: {entry}", flush=True)
+        meas["fp"].append(fp_measure(w))
     meas["seconds"] = round(time.time() - t0, 1)
     with open(os.path.join(a.out, "autotune.json"), "w") as fh:
         json.dump(meas, fh, indent=1)

@@ -92,6 +92,8 @@ struct WalkLane {
     int lo, hi;        // x0 - 2, x0 + B: clamp of floor(px)
     unsigned qoff;     // byte offset of the slot of floor(px) = 0 (mod 2^32)
     int last;
+    float margin;      // exact windows: bound of the inverted window ends (walk_view_bounds)
+    bool regwalk;      // exact windows: register-carried sums apply (walk_view_bounds)
 };
 
 // first: the slot of bin x0 (2 with the clamped walk's two dummies below the
@@ -150,7 +152,7 @@ __device__ __forceinline__ float walk_px(const WalkLane& L, float dx, float zcv)
 // rest.)
 __device__ __forceinline__ void walk_window_exact(const WalkLane& L, float zcv, int& e0,
                                                   int& e1) {
-    const float margin = (L.mbase + 1.8e-6f * fabsf(zcv)) * L.inv_c + 1e-3f;
+    const float margin = L.margin;
     const float tlo = ((L.b_lo - L.cx) - zcv) * L.inv_c + L.ocx;
     const float thi = ((L.b_hi - L.cx) - zcv) * L.inv_c + L.ocx;
     e0 = static_cast<int>(fminf(fmaxf(ceilf(tlo - margin), 0.f), L.fnx));
@@ -233,6 +235,16 @@ __device__ __forceinline__ bool reg_walk_ok(const WalkLane& L, float zcv) {
     return L.ct < 1.0f - (L.fnx + L.fin + fabsf(zcv)) * 4.76837158203125e-7f;
 }
 
+// The exact walk's per-view bounds, taken at the largest |zc| of the view's z
+// rows (zc = (dz*sin_t)*zfactor is monotonic in oz, so that is |zc| at the
+// first or the last row): the walk_window margin, and reg_walk_ok, which then
+// holds for every z row of the view.
+__device__ __forceinline__ void walk_view_bounds(WalkLane& L, float zc_first, float zc_last) {
+    const float mz = fmaxf(fabsf(zc_first), fabsf(zc_last));
+    L.margin = (L.mbase + 1.8e-6f * mz) * L.inv_c + 1e-3f;
+    L.regwalk = reg_walk_ok(L, mz);
+}
+
 // One voxel, main.rs:860-887 with px0 = floor(px): v*(1-frac) to bin px0 and
 // v*frac to bin px0+1.  A voxel outside 0 <= px0 < in_nx-1 adds nothing in the
 // reference; here it adds +0.0 (its value is replaced by 0), which leaves a sum
@@ -274,20 +286,25 @@ __device__ __forceinline__ void walk_row(const WalkLane& L, char* lds,
         // is exact, so every voxel sees the same dx as in a one-by-one walk
         const float* vend = vz + e1;
         const float* vlast = vend - 1;  // a pair starts before the last voxel
-        if (reg_walk_ok(L, zcv)) {
+        if (L.regwalk) {
             if (vp < vend) {
+                // An odd count starts with its first voxel; an even one with a
+                // voxel of value 0 at the first voxel's px, whose +0.0 terms
+                // leave the sums' bits unchanged.  The rest goes in pairs.
                 char* const qb = lds + L.qoff;
+                const bool odd = ((e1 - e0) & 1) != 0;
                 RegWalk W;
-                reg_begin(L, qb, dx, zcv, vp[0], W);
-                ++vp;
-                dx += 1.0f;
-                for (; vp < vlast; vp += 2) {
+                reg_begin(L, qb, dx, zcv, odd ? vp[0] : 0.0f, W);
+                if (odd) {
+                    ++vp;
+                    dx += 1.0f;
+                }
+                for (; vp < vend; vp += 2) {
                     const float v0 = vp[0], v1 = vp[1];
                     reg_step(L, qb, dx, zcv, v0, W);
                     reg_step(L, qb, dx + 1.0f, zcv, v1, W);
                     dx += 2.0f;
                 }
-                if (vp < vend) reg_step(L, qb, dx, zcv, vp[0], W);
                 W.q[0] = W.r0;
                 W.q[kFpWG] = W.r1;
             }
@@ -533,6 +550,7 @@ __device__ __forceinline__ void fp_walk_block(const FpParams& P, float* lds, int
 #pragma unroll
     for (int s = 1; s < B + 1; ++s) a[s * kFpWG] = 0.f;
     const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    walk_view_bounds(L, zc[0], zc[P.out_nz - 1]);
     if (M.slide) slide_start(P, L, M, zc[0]);
     const int nx = P.out_nx;
     const int nz = P.out_nz;
@@ -662,20 +680,23 @@ __device__ __forceinline__ void walk_row2(const WalkLane& L, char* lds,
     const float2* vp = vz + e0;
     const float2* vend = vz + e1;
     const float2* vlast = vend - 1;
-    if (reg_walk_ok(L, zcv)) {
+    if (L.regwalk) {
         if (vp < vend) {
+            // odd count: first voxel alone; even: a value-0 voxel first (walk_row)
             char* const qb = lds + L.qoff;
+            const bool odd = ((e1 - e0) & 1) != 0;
             RegWalk2 W;
-            reg_begin2<B>(L, qb, dx, zcv, vp[0], W);
-            ++vp;
-            dx += 1.0f;
-            for (; vp < vlast; vp += 2) {
+            reg_begin2<B>(L, qb, dx, zcv, odd ? vp[0] : make_float2(0.0f, 0.0f), W);
+            if (odd) {
+                ++vp;
+                dx += 1.0f;
+            }
+            for (; vp < vend; vp += 2) {
                 const float2 v0 = vp[0], v1 = vp[1];
                 reg_step2<B>(L, qb, dx, zcv, v0, W);
                 reg_step2<B>(L, qb, dx + 1.0f, zcv, v1, W);
                 dx += 2.0f;
             }
-            if (vp < vend) reg_step2<B>(L, qb, dx, zcv, vp[0], W);
             W.q[0] = W.r0[0];
             W.q[kFpWG] = W.r1[0];
             W.q[R1] = W.r0[1];
@@ -735,6 +756,7 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
         a[R1 + s * kFpWG] = 0.f;
     }
     const float* zc = P.zc_tab + static_cast<int64_t>(pv) * P.nz_pad;
+    walk_view_bounds(L, zc[0], zc[P.out_nz - 1]);
     if (M.slide) slide_start(P, L, M, zc[0]);
     const int nx = P.out_nx;
     const int nz = P.out_nz;

@@ -472,19 +472,22 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
         if ((rc = run_means(r->s))) return rc;
     // 2. forward projection of SIRT iterations (one-row: one batched launch per
     // walk form, one or two volume rows per workgroup, as each series' tuned
-    // launch would take; every form gives the same bits)
+    // launch would take, and per LDS class, so that series with 16 bins per
+    // lane do not lower the occupancy of the others; every form gives the same
+    // bits)
     {
-        std::vector<FpParams> grp[2];
+        constexpr int kGroups = 4;  // (rows per workgroup - 1) * 2 + (16 bins per lane)
+        std::vector<FpParams> grp[kGroups];
         for (Req* r : iter) {
             if (r->s->two_row) {
                 if ((rc = run_forward(r->s))) return rc;
             } else {
                 const FpParams q = make_fp(r->s);
-                grp[q.rows_per_wg == 2 ? 1 : 0].push_back(q);
+                grp[(q.rows_per_wg == 2 ? 2 : 0) + (q.bins_per_lane > 8 ? 1 : 0)].push_back(q);
             }
         }
-        size_t off_d[2] = {0, 0}, off_b[2] = {0, 0}, bytes = 0;
-        for (int k = 0; k < 2; ++k) {
+        size_t off_d[kGroups] = {}, off_b[kGroups] = {}, bytes = 0;
+        for (int k = 0; k < kGroups; ++k) {
             if (grp[k].empty()) continue;
             off_d[k] = bytes;
             bytes += grp[k].size() * sizeof(FpParams);
@@ -495,9 +498,9 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
         if (bytes > 0) {
             if ((rc = ensure_batch(sl.fp, bytes))) return rc;
             auto* h = static_cast<char*>(sl.fp.host);
-            int total[2] = {0, 0};
-            size_t lds[2] = {0, 0};
-            for (int k = 0; k < 2; ++k) {
+            int total[kGroups] = {};
+            size_t lds[kGroups] = {};
+            for (int k = 0; k < kGroups; ++k) {
                 const int n = static_cast<int>(grp[k].size());
                 if (n == 0) continue;
                 auto* hp = reinterpret_cast<FpParams*>(h + off_d[k]);
@@ -514,12 +517,12 @@ int run_round(tiltgpu_server* S, std::vector<Req*>& act, Slot& sl) {
             }
             HIP_TRY(hipMemcpyAsync(sl.fp.dev, h, bytes, hipMemcpyHostToDevice, S->stream));
             auto* d = static_cast<char*>(sl.fp.dev);
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < kGroups; ++k) {
                 if (grp[k].empty()) continue;
                 FpBatch b{reinterpret_cast<const FpParams*>(d + off_d[k]),
                           reinterpret_cast<const int*>(d + off_b[k]),
                           static_cast<int>(grp[k].size())};
-                HIP_TRY(launch_forward_gather_batch(b, total[k], lds[k], k + 1, S->stream));
+                HIP_TRY(launch_forward_gather_batch(b, total[k], lds[k], k / 2 + 1, S->stream));
             }
         }
     }

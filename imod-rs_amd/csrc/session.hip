@@ -561,8 +561,8 @@ FpParams make_fp(tiltgpu_session* s) {
     p.nz_pad = s->nz_pad;
     p.cx = static_cast<float>(s->d.in_nx) / 2.0f;
     p.ocx = static_cast<float>(s->d.out_nx) / 2.0f;
-    int rows = 1;
-    const int bins = tuned_fp_bins(s->d.in_nx, nullptr, &rows);
+    int rows = 1, slide = 1;
+    const int bins = tuned_fp_bins(s->d.in_nx, nullptr, &rows, &slide);
     p.bins_per_lane = s->tune_fp_bins > 0 ? s->tune_fp_bins : bins;
     // volume rows per workgroup: the table's choice for the width, or
     // TILTGPU_FP_ROWS (1 or 2); two rows only for rows up to kFpRows2Nx wide
@@ -571,8 +571,9 @@ FpParams make_fp(tiltgpu_session* s) {
         if (r == 1 || r == 2) rows = r;
     }
     p.rows_per_wg = rows == 2 && s->d.out_nx <= kFpRows2Nx ? 2 : 1;
-    // sliding lane windows (same bits as fixed ones); TILTGPU_FP_SLIDE=0 keeps them fixed
-    p.slide_windows = 1;
+    // lane window map (same bits either way): the table's choice for the width,
+    // or TILTGPU_FP_SLIDE (0 fixed, 1 sliding)
+    p.slide_windows = slide;
     if (const char* e = std::getenv("TILTGPU_FP_SLIDE")) p.slide_windows = std::atoi(e) != 0;
     return p;
 }

@@ -14,7 +14,7 @@ struct BpCost {
     int two_row, local, half, in_nx, thickness, tile_z, vps, xpl, k;
 };
 struct FpTune {
-    int in_nx, bins, rows;
+    int in_nx, bins, rows, slide;
 };
 
 #include "tile_table.inc"
@@ -77,8 +77,8 @@ int tuned_bp_shape(int two_row, int local, int half, int in_nx, int out_nz, int*
     return choose(two_row, local, half, in_nx, out_nz, true, vps, xpl);
 }
 
-int tuned_fp_bins(int in_nx, int* source, int* rows) {
-    int best = 0, best_rows = 1;
+int tuned_fp_bins(int in_nx, int* source, int* rows, int* slide) {
+    int best = 0, best_rows = 1, best_slide = 1;
     double bd = 1e30;
     for (const FpTune& e : kFpTable) {
         if (e.in_nx <= 0) continue;
@@ -87,12 +87,15 @@ int tuned_fp_bins(int in_nx, int* source, int* rows) {
             bd = d;
             best = e.bins;
             best_rows = e.rows == 2 ? 2 : 1;
+            best_slide = e.slide != 0;
         }
     }
     if (rows) *rows = 1;
+    if (slide) *slide = 1;
     if (best == 4 || best == 8 || best == 16) {
         if (source) *source = kTuneTable;
         if (rows) *rows = best_rows;
+        if (slide) *slide = best_slide;
         return best;
     }
     if (source) *source = kTuneHeuristic;

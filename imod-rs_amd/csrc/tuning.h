@@ -20,6 +20,6 @@ int tuned_bp_shape(int two_row, int local, int half, int in_nx, int out_nz, int*
 // nearest in log width, else the width-only rule (at least 64 lanes per row).
 // rows (optional): volume rows per workgroup of that entry (1 or 2; 1 without
 // a table entry).
-int tuned_fp_bins(int in_nx, int* source, int* rows = nullptr);
+int tuned_fp_bins(int in_nx, int* source, int* rows = nullptr, int* slide = nullptr);
 
 }  // namespace tiltgpu

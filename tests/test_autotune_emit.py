@@ -20,9 +20,11 @@ def _meas():
         shapes[at.shape_key(64, 1, 2)] = 1.5 * th / 64
         bp.append({"variant": "single-row", "two_row": 0, "local": 0, "half": 0, "width": 1024,
                    "thickness": th, "ms": ms, "shapes": shapes})
-    fp = [{"width": 1024, "ms": {"4": 3.0, "8": 2.0, "16": 2.5}},
+    fp = [{"width": 1024, "ms": {"4": 3.0, "8": 2.0, "16": 2.5},
+           "ms_slide": {"4": 3.1, "8": 2.1, "16": 2.4}},
           {"width": 2048, "ms": {"4": 9.0, "8": 7.0, "16": 6.0},
-           "ms_rows2": {"4": 8.0, "8": 5.5, "16": 6.5}}]
+           "ms_rows2": {"4": 8.0, "8": 5.5, "16": 6.5},
+           "ms_rows2_slide": {"4": 7.0, "8": 5.4, "16": 6.6}}]
     return {"bp": bp, "fp": fp}
 
 
@@ -43,7 +45,8 @@ def test_emit_writes_per_thickness_costs(tmp_path):
     assert (64, 256, 1, 1) not in by and (64, 128, 2, 1) not in by
     # bins and rows per workgroup: one row where only one was measured, two where
     # the two-row walk was fastest
-    assert "static const FpTune kFpTable[] = {\n    {1024, 8, 1},\n    {2048, 8, 2},\n};" in inc
+    assert ("static const FpTune kFpTable[] = {\n    {1024, 8, 1, 0},\n    {2048, 8, 2, 1},\n};"
+            in inc)
     md = (tmp_path / "autotune.md").read_text()
     # the choice at the measured thickness is the fastest measured launch
     assert "| single-row | f32 | 1024 | 64 | (64, 1, 2) | 1.500 |" in md
@@ -62,13 +65,15 @@ def test_fp_only_rewrite_keeps_backprojection_rows():
     with open(src) as fh:
         text = fh.read()
     rows, md = at.fp_table_rows(_meas()["fp"])
-    assert rows == [(1024, 8, 1), (2048, 8, 2)]
-    assert md[1].endswith("| (8, 2) |")
+    assert rows == [(1024, 8, 1, 0), (2048, 8, 2, 1)]
+    assert md[1].endswith("| (8, 2, 1) |")
+    # unmeasured forms are dashes: 1024 wide has no two-row columns
+    assert md[0].count(" - |") == 6
     new = at.replace_fp_table(text, rows)
     before, _, fp_old = text.partition("static const FpTune kFpTable[] = {")
     after, _, fp_new = new.partition("static const FpTune kFpTable[] = {")
     assert before == after
-    assert fp_new.startswith("\n    {1024, 8, 1},\n    {2048, 8, 2},\n};\n")
+    assert fp_new.startswith("\n    {1024, 8, 1, 0},\n    {2048, 8, 2, 1},\n};\n")
     assert fp_new.split("};\n", 1)[1] == fp_old.split("};\n", 1)[1]
     with pytest.raises(ValueError):
         at.replace_fp_table("no table here", rows)
