@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(kBpThreads)
 #pragma unroll
                         for (int j = 0; j < XPL; ++j) {
                             if (c.row[j] >= 0) {
-                                const float4* E = lds4 + buf + (c.row[j] - c.r_lo) * c.L - c.seg_lo;
+                                const float4* E = lds4 + buf + __mul24(c.row[j] - c.r_lo, c.L) - c.seg_lo;
                                 const float a = c.a[j], fy = c.fy[j], ldx = c.ldx[j];
                                 // the fences keep the scheduler from hoisting every voxel's
                                 // 16-byte LDS read at once (registers, not issue, limit it).
