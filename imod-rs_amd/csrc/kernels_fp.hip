@@ -825,7 +825,7 @@ __device__ __forceinline__ void fp_block(const FpParams& P, float* lds, int blk)
 // RPW: volume rows per workgroup (FpParams::rows_per_wg), a kernel each so that
 // the one-row walk keeps its own register allocation.
 template <int RPW>
-__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_kernel(FpParams P, int nblk) {
+__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(RPW == 2 ? 5 : 7))) fp_walk_kernel(FpParams P, int nblk) {
     extern __shared__ float fp_lds[];
     fp_block<RPW>(P, fp_lds, xcd_remap(blockIdx.x, nblk));
 }
@@ -844,7 +844,7 @@ __host__ __device__ inline int fp_blocks(const FpParams& p) {
 // workgroups spread evenly over the 8 XCDs (a contiguous run of each series per
 // XCD); padding workgroups exit at once.
 template <int RPW>
-__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(7))) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
+__global__ void __launch_bounds__(kFpWG) __attribute__((amdgpu_waves_per_eu(RPW == 2 ? 5 : 7))) fp_walk_batch_kernel(FpBatch Bt, int nblk) {
     extern __shared__ float fp_lds[];
     const int b = blockIdx.x;
     if (b >= nblk) return;
