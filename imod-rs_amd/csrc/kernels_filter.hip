@@ -429,6 +429,10 @@ __global__ void __launch_bounds__(256) filter_batch_kernel(FilterBatch B) {
 // of dynamic LDS per workgroup (two buffers of n complex values); a kernel opts
 // in to the size a launch needs the first time it is needed.
 static hipError_t allow_large_lds(const void* kernel, size_t lds) {
+    return lds_opt_in(kernel, lds);
+}
+
+hipError_t lds_opt_in(const void* kernel, size_t lds) {
     if (lds <= 64 * 1024) return hipSuccess;
     if (lds > kFilterMaxLds) return hipErrorInvalidValue;
     static std::mutex mu;

@@ -33,7 +33,7 @@ namespace tiltgpu {
 
 namespace {
 
-constexpr int kFpWG = 256;          // threads per workgroup: 4 waves = 4 views
+constexpr int kFpWG = 512;          // threads per workgroup: 8 waves = 8 views
 constexpr int kFpViewsPerWG = kFpWG / 64;
 constexpr float kWalkMinC = 0.05f;  // views with cos_t*cos_xt below this use fp_bin_scan
 
@@ -502,10 +502,10 @@ __device__ __forceinline__ void fp_walk_global(const FpParams& P, float* lds, in
     walk_store<B>(P, a, x0, iy, p, 2);
 }
 
-// Rows of at most kStageNx voxels are staged in LDS: the 4 waves of a workgroup
-// walk 4 views of the same volume row, which the workgroup reads from HBM once,
+// Rows of at most kStageNx voxels are staged in LDS: the 8 waves of a workgroup
+// walk 8 views of the same volume row, which the workgroup reads from HBM once,
 // coalesced, one z row ahead (registers) and double-buffered (LDS).
-constexpr int kStageRegs = 16;
+constexpr int kStageRegs = 4096 / kFpWG;
 constexpr int kStageNx = kStageRegs * kFpWG;
 __host__ __device__ constexpr int fp_stage_stride(int nx) { return nx + 4; }
 
@@ -780,9 +780,11 @@ __device__ __forceinline__ void fp_walk_block2(const FpParams& P, float* lds, in
         for (int k = 0; k < kStageRegs2; ++k)
             if (t + k * kFpWG < nx) stg[t + k * kFpWG] = make_float2(r0[k], r1[k]);
     };
-    // One staging buffer (a second barrier per z row): the two slot sets and a
-    // double buffer would take 39 KB at B = 8, four workgroups per CU; 31 KB
-    // allows five, which outweighs the barrier (1024^2 x 61 -> z300 launch: -5%).
+    // One staging buffer (a second barrier per z row): with 4 views per workgroup
+    // the two slot sets and a double buffer took 39 KB at B = 8 (four workgroups
+    // per CU) against 31 KB (five), which outweighed the barrier
+    // (1024^2 x 61 -> z300 launch: -5%); with 8 views, 52 against 60 KB at B = 8
+    // keeps three workgroups per CU instead of two.
     fetch(0);
     put();
     if (nz > 1) fetch(1);
@@ -1050,22 +1052,35 @@ hipError_t launch_forward_gather(const FpParams& p, hipStream_t st) {
         return hipErrorInvalidValue;
     if (p.rows_per_wg != 1 && p.rows_per_wg != 2) return hipErrorInvalidValue;
     const int nblk = fp_gather_blocks(p);
-    if (p.rows_per_wg == 2)
-        hipLaunchKernelGGL(fp_walk_kernel<2>, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
-    else
-        hipLaunchKernelGGL(fp_walk_kernel<1>, dim3(nblk), dim3(kFpWG), fp_lds_bytes(p), st, p, nblk);
+    const size_t lds = fp_lds_bytes(p);
+    if (p.rows_per_wg == 2) {
+        const hipError_t e = lds_opt_in(reinterpret_cast<const void*>(fp_walk_kernel<2>), lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(fp_walk_kernel<2>, dim3(nblk), dim3(kFpWG), lds, st, p, nblk);
+    } else {
+        const hipError_t e = lds_opt_in(reinterpret_cast<const void*>(fp_walk_kernel<1>), lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(fp_walk_kernel<1>, dim3(nblk), dim3(kFpWG), lds, st, p, nblk);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_forward_gather_batch(const FpBatch& b, int total_blocks, size_t lds,
                                        int rows_per_wg, hipStream_t st) {
     if (total_blocks <= 0) return hipSuccess;
-    if (rows_per_wg == 2)
+    if (rows_per_wg == 2) {
+        const hipError_t e =
+            lds_opt_in(reinterpret_cast<const void*>(fp_walk_batch_kernel<2>), lds);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(fp_walk_batch_kernel<2>, dim3(total_blocks), dim3(kFpWG), lds, st, b,
                            total_blocks);
-    else
+    } else {
+        const hipError_t e =
+            lds_opt_in(reinterpret_cast<const void*>(fp_walk_batch_kernel<1>), lds);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(fp_walk_batch_kernel<1>, dim3(total_blocks), dim3(kFpWG), lds, st, b,
                            total_blocks);
+    }
     return hipGetLastError();
 }
 

@@ -218,6 +218,9 @@ size_t filter_lds_bytes(int n, int pairs_per_wg);
 // less 256 bytes for the kernels' own static LDS (rows up to 10224 wide on the
 // mixed-radix path).
 constexpr size_t kFilterMaxLds = 160 * 1024 - 256;
+// Lets `kernel` launch with `lds` bytes of dynamic LDS (more than the default
+// 64 KiB, up to kFilterMaxLds); remembers what it granted (thread-safe).
+hipError_t lds_opt_in(const void* kernel, size_t lds);
 int filter_plan(int n, int* radix, int* nradix);  // 0 ok, -1 unsupported n
 
 hipError_t launch_log_means(const float* proj, float* logd, int64_t nxy, int nz, int do_log,
