@@ -577,10 +577,12 @@ __global__ void __launch_bounds__(kBpThreads)
                 }
                 __syncthreads();
                 __builtin_amdgcn_s_setprio(0);
+                // a wave whose z run lies wholly past the volume (the padded
+                // last tile) only stages: its sums would never be stored
 #pragma unroll
                 for (int v = 0; v < VPS; ++v) {
                     const int p = g * VPS + v;
-                    if (p >= P.nviews) break;
+                    if (p >= P.nviews || zw0 >= P.out_nz) break;
                     const Vs<XPL>& c = cur[v];
                     const int buf = set + v * P.lds_cap;
                     const TILTGPU_CONSTANT float* zc =
